@@ -1,0 +1,782 @@
+// 3D real <-> half-complex FFT for gfx950: Stockham autosort passes in LDS.
+//
+// A transform is three passes, one per axis, each reading and writing the mesh
+// once:
+//   forward : z (real -> half complex, packed as a length n2/2 complex FFT),
+//             y (complex, in place), x (complex, in place, times 1/N).
+//   inverse : x (complex, reads the input once and applies up to six k-space
+//             multipliers, writing one scratch mesh per output), y (in place),
+//             z (half complex -> real).
+// The strided y/x passes work on tiles of BZ adjacent z columns so each global
+// row segment is 128 bytes; the tile (n x BZ complex) lives in LDS and every
+// Stockham stage reads its radix-R inputs to registers, synchronises, and writes
+// the outputs back (in-place autosort).  Stage indexing follows the Stockham
+// formulation of Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors" (SC08): input j + r N/R, twiddle
+// w_N^{r (j mod Ns) N/(Ns R)}, output (j/Ns) Ns R + j mod Ns + r Ns.
+// Twiddles come from device tables computed in double on the host.
+#include "fpm_common.h"
+
+#include <cmath>
+#include <vector>
+
+namespace fpm {
+
+// ---------------------------------------------------------------- radix plans
+template <int N> struct Plan;
+#define FPM_PLAN(N_, NS_, ...)                       \
+  template <> struct Plan<N_> {                      \
+    static constexpr int ns = NS_;                   \
+    static constexpr int r[6] = {__VA_ARGS__};       \
+  };
+FPM_PLAN(2, 1, 2)
+FPM_PLAN(4, 1, 4)
+FPM_PLAN(8, 1, 8)
+FPM_PLAN(16, 2, 4, 4)
+FPM_PLAN(32, 2, 8, 4)
+FPM_PLAN(64, 2, 8, 8)
+FPM_PLAN(128, 3, 8, 4, 4)
+FPM_PLAN(256, 3, 8, 8, 4)
+FPM_PLAN(512, 3, 8, 8, 8)
+FPM_PLAN(1024, 4, 8, 8, 4, 4)
+FPM_PLAN(3, 1, 3)
+FPM_PLAN(6, 2, 3, 2)
+FPM_PLAN(12, 2, 3, 4)
+FPM_PLAN(24, 2, 3, 8)
+FPM_PLAN(48, 3, 3, 4, 4)
+FPM_PLAN(96, 3, 3, 8, 4)
+FPM_PLAN(192, 3, 3, 8, 8)
+FPM_PLAN(384, 4, 3, 8, 4, 4)
+FPM_PLAN(768, 4, 3, 8, 8, 4)
+#undef FPM_PLAN
+
+// ---------------------------------------------------------------- butterflies
+// Multiply by -i (forward) or +i (inverse).
+template <bool INV, typename V> __device__ __forceinline__ V rot90(V a) {
+  return INV ? cmk<V>(-a.y, a.x) : cmk<V>(a.y, -a.x);
+}
+
+template <int R, bool INV, typename V> struct Dft;
+
+template <bool INV, typename V> struct Dft<2, INV, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    V t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <bool INV, typename V> struct Dft<4, INV, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    V t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    V t2 = cadd(a[1], a[3]), t3 = rot90<INV>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <bool INV, typename V> struct Dft<8, INV, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    using T = decltype(V::x);
+    const T h = (T)0.70710678118654752440084436210485;
+    V e[4] = {a[0], a[2], a[4], a[6]};
+    V o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<4, INV, V>::run(e);
+    Dft<4, INV, V>::run(o);
+    // o1 *= w8, o2 *= w8^2, o3 *= w8^3 with w8 = exp(-+ i pi/4)
+    V w1 = INV ? cmk<V>((o[1].x - o[1].y) * h, (o[1].x + o[1].y) * h)
+               : cmk<V>((o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h);
+    V w2 = rot90<INV>(o[2]);
+    V w3i = INV ? cmk<V>((o[3].x - o[3].y) * h, (o[3].x + o[3].y) * h)
+                : cmk<V>((o[3].x + o[3].y) * h, (o[3].y - o[3].x) * h);
+    V w3 = rot90<INV>(w3i);
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], w1);
+    a[5] = csub(e[1], w1);
+    a[2] = cadd(e[2], w2);
+    a[6] = csub(e[2], w2);
+    a[3] = cadd(e[3], w3);
+    a[7] = csub(e[3], w3);
+  }
+};
+
+template <bool INV, typename V> struct Dft<3, INV, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    using T = decltype(V::x);
+    const T s3 = (T)0.86602540378443864676372317075294;
+    V t = cadd(a[1], a[2]);
+    V s = csub(a[1], a[2]);
+    V m = cmk<V>(a[0].x - (T)0.5 * t.x, a[0].y - (T)0.5 * t.y);
+    V u = rot90<INV>(cscale(s, s3));
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, u);
+    a[2] = csub(m, u);
+  }
+};
+
+// ---------------------------------------------------------------- one stage
+// NL lines of length N in LDS.  Element e of line l lives at l*LS + e*ES.
+// LFAST selects which index varies fastest across threads (line index for the
+// column tiles, whose lines are interleaved; element index for row tiles).
+// TS is the stride into the twiddle table (2 when the table is for length 2N).
+template <typename T, int N, int R, int NS, int NL, int NT, bool LFAST, int ES, int LS, int TS,
+          bool INV>
+__device__ __forceinline__ void stage(C<T>* s, const C<T>* tw) {
+  using V = C<T>;
+  constexpr int NB = NL * (N / R);
+  constexpr int PER = (NB + NT - 1) / NT;
+  V v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = threadIdx.x + q * NT;
+    if ((NB % NT == 0) || b < NB) {
+      int l, j;
+      if (LFAST) { l = b % NL; j = b / NL; } else { j = b % (N / R); l = b / (N / R); }
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = s[l * LS + (j + r * (N / R)) * ES];
+      if (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          V w = tw[(r * k * (N / (NS * R))) * TS];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      Dft<R, INV, V>::run(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = threadIdx.x + q * NT;
+    if ((NB % NT == 0) || b < NB) {
+      int l, j;
+      if (LFAST) { l = b % NL; j = b / NL; } else { j = b % (N / R); l = b / (N / R); }
+      const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[l * LS + (base + r * NS) * ES] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int S, int NS, int NL, int NT, bool LFAST, int ES, int LS, int TS,
+          bool INV>
+struct Stages {
+  __device__ __forceinline__ static void run(C<T>* s, const C<T>* tw) {
+    if constexpr (S < Plan<N>::ns) {
+      constexpr int R = Plan<N>::r[S];
+      stage<T, N, R, NS, NL, NT, LFAST, ES, LS, TS, INV>(s, tw);
+      Stages<T, N, S + 1, NS * R, NL, NT, LFAST, ES, LS, TS, INV>::run(s, tw);
+    }
+  }
+};
+
+constexpr int round64(int x) { return ((x + 63) / 64) * 64; }
+constexpr int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// Threads for a tile of E complex elements: about 8 elements per thread.
+constexpr int threads_for(int E) { return clampi(round64(E / 8), 64, 1024); }
+
+// Column tile width: 128-byte global row segments.
+template <typename T> struct ColW;
+template <> struct ColW<float> { static constexpr int BZ = 16; };
+template <> struct ColW<double> { static constexpr int BZ = 8; };
+
+// Lines per workgroup for the z pass: about 2048 complex elements.
+constexpr int z_lines(int M) {
+  int nl = 1;
+  while (nl * 2 * M <= 2048) nl *= 2;
+  return nl;
+}
+
+// ---------------------------------------------------------------- k-space multipliers
+struct KopPack {
+  int ncomp;
+  int laplace[6];
+  int ngrad[6];
+  int gdir[6][2];
+  double factor[6];
+  double r2[6];          // r_split^2
+  int grad_order[6];
+  int deconv[6];
+  int nm[3];
+  double box[3];
+};
+
+// Per-axis factors for mode index i on an axis of n points and side L.
+// Returns k^2, the gradient factor for both gradient orders and sinc(k h/2).
+__device__ __forceinline__ void axis_factors(int i, int n, double L, bool half, double& kk,
+                                             double& g0, double& g1, double& sn) {
+  const int nn = half ? i : (i < (n + 1) / 2 ? i : i - n);
+  const bool nyq = (n % 2 == 0) && (i == n / 2);
+  const double two_pi = 6.283185307179586476925286766559;
+  const double k = two_pi * (double)nn / L;
+  const double h = L / (double)n;
+  kk = k * k;
+  g0 = nyq ? 0.0 : k;
+  // 4-point finite difference: (8 sin w - sin 2w) / (6 h), w = 2 pi nn / n.
+  const double x = 2.0 * (double)nn / (double)n;
+  g1 = nyq ? 0.0 : (8.0 * sinpi(x) - sinpi(2.0 * x)) / (6.0 * h);
+  const double hx = (double)nn / (double)n;  // w/2 over pi
+  sn = (nn == 0) ? 1.0 : sinpi(hx) / (3.14159265358979323846264338327950 * hx);
+}
+
+template <typename T>
+struct AxisTab {
+  T kk, g0, g1, sn;
+};
+
+template <typename T>
+__device__ __forceinline__ AxisTab<T> axis_tab(int i, int n, double L, bool half) {
+  double kk, g0, g1, sn;
+  axis_factors(i, n, L, half, kk, g0, g1, sn);
+  return AxisTab<T>{(T)kk, (T)g0, (T)g1, (T)sn};
+}
+
+// Multiplier for component c; ax/ay/az are the axis factors of the mode.
+// Returns (m, ngrad) with the full multiplier = m * i^ngrad.
+template <typename T>
+__device__ __forceinline__ T kop_mult(const KopPack& K, int c, const AxisTab<T>& ax,
+                                      const AxisTab<T>& ay, const AxisTab<T>& az) {
+  T m = (T)K.factor[c];
+  const T kk = ax.kk + ay.kk + az.kk;
+  if (K.laplace[c]) m = (kk == (T)0) ? (T)0 : m / kk;
+  if (K.r2[c] != 0.0) m *= exp(-kk * (T)K.r2[c]);
+  for (int g = 0; g < K.ngrad[c]; ++g) {
+    const int d = K.gdir[c][g];
+    const AxisTab<T>& a = d == 0 ? ax : (d == 1 ? ay : az);
+    m *= K.grad_order[c] == 0 ? a.g0 : a.g1;
+  }
+  const int p = K.deconv[c];
+  if (p) {
+    const T w = ax.sn * ay.sn * az.sn;
+    T wp = w;
+    for (int q = 1; q < p; ++q) wp *= w;
+    m /= wp;
+  }
+  return m;
+}
+
+template <typename V>
+__device__ __forceinline__ V apply_mult(V v, decltype(V::x) m, int ngrad) {
+  switch (ngrad & 3) {
+    case 0: return cmk<V>(v.x * m, v.y * m);
+    case 1: return cmk<V>(-v.y * m, v.x * m);
+    case 2: return cmk<V>(-v.x * m, -v.y * m);
+    default: return cmk<V>(v.y * m, -v.x * m);
+  }
+}
+
+// ---------------------------------------------------------------- column pass
+// Transform along a strided axis of a complex mesh.  The tile is the N points
+// of the axis times BZ adjacent z columns; `outer` enumerates the remaining
+// axis.  Element (e, outer, z) is at outer*ostride + e*estride + z.
+template <typename T>
+struct ColArgs {
+  const C<T>* in;
+  C<T>* out[6];
+  int ncomp;
+  int64_t estride, ostride;
+  int nouter, ncols, ntz;
+  T scale;
+  const C<T>* tw;
+  int64_t nblk;
+  // k-space multipliers (only for the x pass of the inverse transform, where
+  // e is the x index, outer the y index and z the column).
+  KopPack kop;
+};
+
+template <typename T, int N, int BZ, int NT, bool INV, bool KOP>
+__global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
+  using V = C<T>;
+  constexpr int E = N * BZ;
+  constexpr int PER = (E + NT - 1) / NT;
+  // The x-axis factor table is kept in LDS when it fits beside the tile.
+  constexpr bool TXL = KOP && ((E + N) * sizeof(V) + N * sizeof(AxisTab<T>) <= 150 * 1024);
+  __shared__ V tile[E];
+  __shared__ V tw[N];
+  __shared__ AxisTab<T> tx[TXL ? N : 1];
+  __shared__ AxisTab<T> tz[KOP ? BZ : 1];
+
+  const int64_t bid = xcd_remap(blockIdx.x, a.nblk);
+  const int outer = (int)(bid / a.ntz);
+  const int z0 = (int)(bid % a.ntz) * BZ;
+
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = a.tw[i];
+  AxisTab<T> ty{};
+  if constexpr (KOP) {
+    if constexpr (TXL)
+      for (int i = threadIdx.x; i < N; i += NT) tx[i] = axis_tab<T>(i, a.kop.nm[0], a.kop.box[0], false);
+    for (int i = threadIdx.x; i < BZ; i += NT)
+      tz[i] = axis_tab<T>(z0 + i, a.kop.nm[2], a.kop.box[2], true);
+    ty = axis_tab<T>(outer, a.kop.nm[1], a.kop.box[1], false);
+  }
+
+  const V* src = a.in + (int64_t)outer * a.ostride + z0;
+  V reg[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    reg[q] = cmk<V>((T)0, (T)0);
+    if ((E % NT == 0) || idx < E) {
+      const int l = idx % BZ, e = idx / BZ;
+      if (z0 + l < a.ncols) reg[q] = src[(int64_t)e * a.estride + l];
+    }
+  }
+  if constexpr (KOP) __syncthreads();  // axis tables ready
+
+  for (int c = 0; c < a.ncomp; ++c) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = threadIdx.x + q * NT;
+      if ((E % NT == 0) || idx < E) {
+        V v = reg[q];
+        if constexpr (KOP) {
+          const int l = idx % BZ, e = idx / BZ;
+          const AxisTab<T> axe = TXL ? tx[e] : axis_tab<T>(e, a.kop.nm[0], a.kop.box[0], false);
+          const T m = kop_mult<T>(a.kop, c, axe, ty, tz[l]);
+          v = apply_mult(v, m, a.kop.ngrad[c]);
+        }
+        tile[idx] = v;
+      }
+    }
+    __syncthreads();
+    Stages<T, N, 0, 1, BZ, NT, true, BZ, 1, 1, INV>::run(tile, tw);
+    V* dst = a.out[c] + (int64_t)outer * a.ostride + z0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = threadIdx.x + q * NT;
+      if ((E % NT == 0) || idx < E) {
+        const int l = idx % BZ, e = idx / BZ;
+        if (z0 + l < a.ncols) dst[(int64_t)e * a.estride + l] = cscale(tile[idx], a.scale);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- z passes
+// Real rows of length 2M are transformed as length-M complex rows
+// (z_m = x_{2m} + i x_{2m+1}) and unpacked to M+1 half-spectrum modes:
+//   X_k = (Z_k + conj Z_{M-k})/2 - (i/2) w^k (Z_k - conj Z_{M-k}),  w = exp(-2 pi i/2M).
+// The inverse packs Z_k = (X_k + conj X_{M-k}) + i w^-k (X_k - conj X_{M-k})
+// (imaginary parts of X_0 and X_M ignored) and runs an unnormalised inverse.
+template <typename T>
+struct ZArgs {
+  const T* rin;      // r2c input rows [nrows][2M]
+  const C<T>* cin;   // c2r input rows [nrows][pitch2]
+  C<T>* cout;        // r2c output rows
+  T* rout;           // c2r output rows
+  int64_t nrows;
+  int64_t pitch2;
+  const C<T>* tw;    // table of length 2M
+};
+
+template <typename T, int M, int NL, int NT>
+__global__ __launch_bounds__(NT) void k_fft_z_r2c(ZArgs<T> a) {
+  using V = C<T>;
+  constexpr int LP = M + 1;
+  constexpr int NP = M / 2 + 1;                 // pairs (k, M-k), k = 0..M/2
+  constexpr int PERP = (NL * NP + NT - 1) / NT;
+  __shared__ V s[NL * LP];
+  __shared__ V tw[2 * M];
+  const int64_t row0 = (int64_t)blockIdx.x * NL;
+  for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = a.tw[i];
+  const V* in = reinterpret_cast<const V*>(a.rin);
+  for (int idx = threadIdx.x; idx < NL * M; idx += NT) {
+    const int l = idx / M, m = idx % M;
+    const int64_t row = row0 + l;
+    s[l * LP + m] = row < a.nrows ? in[row * M + m] : cmk<V>((T)0, (T)0);
+  }
+  __syncthreads();
+  Stages<T, M, 0, 1, NL, NT, false, 1, LP, 2, false>::run(s, tw);
+  V xa[PERP], xb[PERP];
+#pragma unroll
+  for (int q = 0; q < PERP; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    if (idx < NL * NP) {
+      const int l = idx / NP, k = idx % NP;
+      const V za = s[l * LP + (k % M)];
+      const V zb = s[l * LP + ((M - k) % M)];
+      const V cb = cconj(zb), ca = cconj(za);
+      // X_k
+      {
+        const V sum = cadd(za, cb), dif = csub(za, cb);
+        const V t = cmul(tw[k], dif);  // w^k (Z_k - conj Z_{M-k})
+        xa[q] = cmk<V>((T)0.5 * (sum.x + t.y), (T)0.5 * (sum.y - t.x));
+      }
+      // X_{M-k}
+      {
+        const V sum = cadd(zb, ca), dif = csub(zb, ca);
+        const V t = cmul(tw[M - k], dif);
+        xb[q] = cmk<V>((T)0.5 * (sum.x + t.y), (T)0.5 * (sum.y - t.x));
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PERP; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    if (idx < NL * NP) {
+      const int l = idx / NP, k = idx % NP;
+      s[l * LP + k] = xa[q];
+      s[l * LP + (M - k)] = xb[q];
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < NL * LP; idx += NT) {
+    const int l = idx / LP, k = idx % LP;
+    const int64_t row = row0 + l;
+    if (row < a.nrows) a.cout[row * a.pitch2 + k] = s[l * LP + k];
+  }
+}
+
+template <typename T, int M, int NL, int NT>
+__global__ __launch_bounds__(NT) void k_fft_z_c2r(ZArgs<T> a) {
+  using V = C<T>;
+  constexpr int LP = M + 1;
+  constexpr int NP = M / 2 + 1;
+  constexpr int PERP = (NL * NP + NT - 1) / NT;
+  __shared__ V s[NL * LP];
+  __shared__ V tw[2 * M];
+  const int64_t row0 = (int64_t)blockIdx.x * NL;
+  for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = a.tw[i];
+  for (int idx = threadIdx.x; idx < NL * LP; idx += NT) {
+    const int l = idx / LP, k = idx % LP;
+    const int64_t row = row0 + l;
+    s[l * LP + k] = row < a.nrows ? a.cin[row * a.pitch2 + k] : cmk<V>((T)0, (T)0);
+  }
+  __syncthreads();
+  V za[PERP], zb[PERP];
+#pragma unroll
+  for (int q = 0; q < PERP; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    if (idx < NL * NP) {
+      const int l = idx / NP, k = idx % NP;
+      V xa = s[l * LP + k];
+      V xb = s[l * LP + (M - k)];
+      if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
+      const V ca = cconj(xa), cb = cconj(xb);
+      {
+        const V sum = cadd(xa, cb), dif = csub(xa, cb);
+        const V t = cmul(cconj(tw[k]), dif);  // w^-k (X_k - conj X_{M-k})
+        za[q] = cmk<V>(sum.x - t.y, sum.y + t.x);
+      }
+      {
+        const V sum = cadd(xb, ca), dif = csub(xb, ca);
+        const V t = cmul(cconj(tw[M - k]), dif);
+        zb[q] = cmk<V>(sum.x - t.y, sum.y + t.x);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PERP; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    if (idx < NL * NP) {
+      const int l = idx / NP, k = idx % NP;
+      s[l * LP + k] = za[q];
+      if (k != 0) s[l * LP + (M - k)] = zb[q];
+    }
+  }
+  __syncthreads();
+  Stages<T, M, 0, 1, NL, NT, false, 1, LP, 2, true>::run(s, tw);
+  V* out = reinterpret_cast<V*>(a.rout);
+  for (int idx = threadIdx.x; idx < NL * M; idx += NT) {
+    const int l = idx / M, m = idx % M;
+    const int64_t row = row0 + l;
+    if (row < a.nrows) out[row * M + m] = s[l * LP + m];
+  }
+}
+
+// ---------------------------------------------------------------- dispatch
+#define FPM_LENGTHS(X) \
+  X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) \
+  X(3) X(6) X(12) X(24) X(48) X(96) X(192) X(384) X(768)
+
+static bool supported_len(int n) {
+  switch (n) {
+#define FPM_CASE(N_) case N_:
+    FPM_LENGTHS(FPM_CASE)
+#undef FPM_CASE
+    return true;
+    default: return false;
+  }
+}
+
+template <typename T, int N, bool INV, bool KOP>
+static void launch_col_n(const ColArgs<T>& a, hipStream_t st) {
+  constexpr int BZ = ColW<T>::BZ;
+  constexpr int NT = threads_for(N * BZ);
+  hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP>), dim3((unsigned)a.nblk), dim3(NT), 0,
+                     st, a);
+}
+
+template <typename T, bool INV, bool KOP>
+static int launch_col(int n, ColArgs<T> a, hipStream_t st) {
+  constexpr int BZ = ColW<T>::BZ;
+  a.ntz = (a.ncols + BZ - 1) / BZ;
+  a.nblk = (int64_t)a.ntz * a.nouter;
+  if (a.nblk <= 0) return FPM_OK;
+  switch (n) {
+#define FPM_CASE(N_) \
+  case N_: launch_col_n<T, N_, INV, KOP>(a, st); break;
+    FPM_LENGTHS(FPM_CASE)
+#undef FPM_CASE
+    default: return FPM_ERR_UNSUPPORTED;
+  }
+  return FPM_OK;
+}
+
+template <typename T, int M, bool INV>
+static void launch_z_m(const ZArgs<T>& a, hipStream_t st) {
+  constexpr int NL = z_lines(M);
+  constexpr int NT = threads_for(NL * M);
+  const unsigned nblk = (unsigned)((a.nrows + NL - 1) / NL);
+  if (INV)
+    hipLaunchKernelGGL((k_fft_z_c2r<T, M, NL, NT>), dim3(nblk), dim3(NT), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_fft_z_r2c<T, M, NL, NT>), dim3(nblk), dim3(NT), 0, st, a);
+}
+
+template <typename T, bool INV>
+static int launch_z(int n2, const ZArgs<T>& a, hipStream_t st) {
+  if (n2 % 2) return FPM_ERR_UNSUPPORTED;
+  if (a.nrows <= 0) return FPM_OK;
+  switch (n2 / 2) {
+#define FPM_CASE(N_) \
+  case N_: launch_z_m<T, N_, INV>(a, st); break;
+    FPM_LENGTHS(FPM_CASE)
+#undef FPM_CASE
+    default: return FPM_ERR_UNSUPPORTED;
+  }
+  return FPM_OK;
+}
+
+static int check_mesh(const fpm_mesh* m, int64_t pitch2) {
+  if (!m) return FPM_ERR_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (m->nmesh[d] <= 0 || !(m->box[d] > 0)) return FPM_ERR_ARG;
+  if (pitch2 < m->nmesh[2] / 2 + 1) return FPM_ERR_ARG;
+  if (!supported_len(m->nmesh[0]) || !supported_len(m->nmesh[1])) return FPM_ERR_UNSUPPORTED;
+  if (m->nmesh[2] % 2 || !supported_len(m->nmesh[2] / 2)) return FPM_ERR_UNSUPPORTED;
+  return FPM_OK;
+}
+
+template <typename T>
+static int r2c_impl(const void* in, void* out, const fpm_mesh* m, int64_t pitch2, const void* tw0,
+                    const void* tw1, const void* tw2, hipStream_t st) {
+  int rc = check_mesh(m, pitch2);
+  if (rc) return rc;
+  if (!in || !out || !tw0 || !tw1 || !tw2) return FPM_ERR_ARG;
+  const int n0 = m->nmesh[0], n1 = m->nmesh[1], n2 = m->nmesh[2];
+  const int nh = n2 / 2 + 1;
+  C<T>* cout = static_cast<C<T>*>(out);
+  ZArgs<T> z{};
+  z.rin = static_cast<const T*>(in);
+  z.cout = cout;
+  z.nrows = (int64_t)n0 * n1;
+  z.pitch2 = pitch2;
+  z.tw = static_cast<const C<T>*>(tw2);
+  if ((rc = launch_z<T, false>(n2, z, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  // y pass, in place
+  ColArgs<T> y{};
+  y.in = cout;
+  y.out[0] = cout;
+  y.ncomp = 1;
+  y.estride = pitch2;
+  y.ostride = (int64_t)n1 * pitch2;
+  y.nouter = n0;
+  y.ncols = nh;
+  y.scale = (T)1;
+  y.tw = static_cast<const C<T>*>(tw1);
+  if ((rc = launch_col<T, false, false>(n1, y, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  // x pass, in place, with 1/N
+  ColArgs<T> x{};
+  x.in = cout;
+  x.out[0] = cout;
+  x.ncomp = 1;
+  x.estride = (int64_t)n1 * pitch2;
+  x.ostride = pitch2;
+  x.nouter = n1;
+  x.ncols = nh;
+  x.scale = (T)(1.0 / ((double)n0 * n1 * n2));
+  x.tw = static_cast<const C<T>*>(tw0);
+  if ((rc = launch_col<T, false, false>(n0, x, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+static int fill_kop(KopPack& K, int ncomp, const fpm_kop* ops, const fpm_mesh* m) {
+  K = KopPack{};
+  K.ncomp = ncomp;
+  for (int d = 0; d < 3; ++d) {
+    K.nm[d] = m->nmesh[d];
+    K.box[d] = m->box[d];
+  }
+  for (int c = 0; c < ncomp; ++c) {
+    if (!ops) {
+      K.factor[c] = 1.0;
+      continue;
+    }
+    const fpm_kop& o = ops[c];
+    if (o.ngrad < 0 || o.ngrad > 2 || o.deconv < 0 || o.deconv > 8) return FPM_ERR_ARG;
+    if (o.grad_order != 0 && o.grad_order != 1) return FPM_ERR_ARG;
+    for (int g = 0; g < o.ngrad; ++g)
+      if (o.grad_dir[g] < 0 || o.grad_dir[g] > 2) return FPM_ERR_ARG;
+    K.factor[c] = o.factor;
+    K.laplace[c] = o.laplace ? 1 : 0;
+    K.ngrad[c] = o.ngrad;
+    K.gdir[c][0] = o.grad_dir[0];
+    K.gdir[c][1] = o.grad_dir[1];
+    K.r2[c] = o.r_split * o.r_split;
+    K.grad_order[c] = o.grad_order;
+    K.deconv[c] = o.deconv;
+  }
+  return FPM_OK;
+}
+
+template <typename T>
+static int c2r_impl(const void* in, int ncomp, const fpm_kop* ops, void* const* outs,
+                    void* const* work, const fpm_mesh* m, int64_t pitch2, const void* tw0,
+                    const void* tw1, const void* tw2, hipStream_t st) {
+  int rc = check_mesh(m, pitch2);
+  if (rc) return rc;
+  if (ncomp < 1 || ncomp > 6 || !in || !outs || !work || !tw0 || !tw1 || !tw2) return FPM_ERR_ARG;
+  for (int c = 0; c < ncomp; ++c)
+    if (!outs[c] || !work[c] || work[c] == in) return FPM_ERR_ARG;
+  const int n0 = m->nmesh[0], n1 = m->nmesh[1], n2 = m->nmesh[2];
+  const int nh = n2 / 2 + 1;
+  ColArgs<T> x{};
+  x.in = static_cast<const C<T>*>(in);
+  for (int c = 0; c < ncomp; ++c) x.out[c] = static_cast<C<T>*>(work[c]);
+  x.ncomp = ncomp;
+  x.estride = (int64_t)n1 * pitch2;
+  x.ostride = pitch2;
+  x.nouter = n1;
+  x.ncols = nh;
+  x.scale = (T)1;
+  x.tw = static_cast<const C<T>*>(tw0);
+  if ((rc = fill_kop(x.kop, ncomp, ops, m))) return rc;
+  if ((rc = launch_col<T, true, true>(n0, x, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  for (int c = 0; c < ncomp; ++c) {
+    C<T>* wc = static_cast<C<T>*>(work[c]);
+    ColArgs<T> y{};
+    y.in = wc;
+    y.out[0] = wc;
+    y.ncomp = 1;
+    y.estride = pitch2;
+    y.ostride = (int64_t)n1 * pitch2;
+    y.nouter = n0;
+    y.ncols = nh;
+    y.scale = (T)1;
+    y.tw = static_cast<const C<T>*>(tw1);
+    if ((rc = launch_col<T, true, false>(n1, y, st))) return rc;
+    FPM_LAUNCH_CHECK();
+    ZArgs<T> z{};
+    z.cin = wc;
+    z.rout = static_cast<T*>(outs[c]);
+    z.nrows = (int64_t)n0 * n1;
+    z.pitch2 = pitch2;
+    z.tw = static_cast<const C<T>*>(tw2);
+    if ((rc = launch_z<T, true>(n2, z, st))) return rc;
+    FPM_LAUNCH_CHECK();
+  }
+  return FPM_OK;
+}
+
+// In-place multiplier on a complex mesh (no transform).
+template <typename T>
+__global__ void k_kspace_apply(C<T>* f, int64_t pitch2, KopPack K) {
+  const int n0 = K.nm[0], n1 = K.nm[1], nh = K.nm[2] / 2 + 1;
+  const int64_t total = (int64_t)n0 * n1 * nh;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(t % nh);
+    const int64_t r = t / nh;
+    const int j = (int)(r % n1), i = (int)(r / n1);
+    const AxisTab<T> ax = axis_tab<T>(i, n0, K.box[0], false);
+    const AxisTab<T> ay = axis_tab<T>(j, n1, K.box[1], false);
+    const AxisTab<T> az = axis_tab<T>(k, K.nm[2], K.box[2], true);
+    const T mult = kop_mult<T>(K, 0, ax, ay, az);
+    C<T>* p = f + r * pitch2 + k;
+    *p = apply_mult(*p, mult, K.ngrad[0]);
+  }
+}
+
+template <typename T>
+static int kspace_apply_impl(void* field, const fpm_kop* op, const fpm_mesh* m, int64_t pitch2,
+                             hipStream_t st) {
+  if (!m || !field || !op || pitch2 < m->nmesh[2] / 2 + 1) return FPM_ERR_ARG;
+  KopPack K;
+  int rc = fill_kop(K, 1, op, m);
+  if (rc) return rc;
+  const int64_t total = (int64_t)m->nmesh[0] * m->nmesh[1] * (m->nmesh[2] / 2 + 1);
+  const int64_t nb = (total + 255) / 256;
+  hipLaunchKernelGGL(k_kspace_apply<T>, dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(256), 0, st,
+                     static_cast<C<T>*>(field), pitch2, K);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
+static int twiddles_impl(int n, T* out) {
+  if (!out || n <= 0) return FPM_ERR_ARG;
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int m = 0; m < n; ++m) {
+    // exact values at the quarter points keep the tables symmetric
+    const double ang = -two_pi * (double)m / (double)n;
+    double c = std::cos(ang), s = std::sin(ang);
+    if (4 * m == n) { c = 0.0; s = -1.0; }
+    if (2 * m == n) { c = -1.0; s = 0.0; }
+    if (4 * m == 3 * n) { c = 0.0; s = 1.0; }
+    out[2 * m] = (T)c;
+    out[2 * m + 1] = (T)s;
+  }
+  return FPM_OK;
+}
+
+}  // namespace fpm
+
+using namespace fpm;
+
+extern "C" {
+
+int fpm_fft_supported(int n) { return supported_len(n) ? 1 : 0; }
+int fpm_twiddles_f32(int n, float* o) { return twiddles_impl<float>(n, o); }
+int fpm_twiddles_f64(int n, double* o) { return twiddles_impl<double>(n, o); }
+
+int fpm_fft_r2c_f32(const void* in, void* out, const fpm_mesh* mesh, int64_t pitch2,
+                    const void* tw0, const void* tw1, const void* tw2, fpm_stream_t s) {
+  return r2c_impl<float>(in, out, mesh, pitch2, tw0, tw1, tw2, (hipStream_t)s);
+}
+int fpm_fft_r2c_f64(const void* in, void* out, const fpm_mesh* mesh, int64_t pitch2,
+                    const void* tw0, const void* tw1, const void* tw2, fpm_stream_t s) {
+  return r2c_impl<double>(in, out, mesh, pitch2, tw0, tw1, tw2, (hipStream_t)s);
+}
+int fpm_fft_c2r_f32(const void* in, int ncomp, const fpm_kop* ops, void* const* outs,
+                    void* const* work, const fpm_mesh* mesh, int64_t pitch2, const void* tw0,
+                    const void* tw1, const void* tw2, fpm_stream_t s) {
+  return c2r_impl<float>(in, ncomp, ops, outs, work, mesh, pitch2, tw0, tw1, tw2, (hipStream_t)s);
+}
+int fpm_fft_c2r_f64(const void* in, int ncomp, const fpm_kop* ops, void* const* outs,
+                    void* const* work, const fpm_mesh* mesh, int64_t pitch2, const void* tw0,
+                    const void* tw1, const void* tw2, fpm_stream_t s) {
+  return c2r_impl<double>(in, ncomp, ops, outs, work, mesh, pitch2, tw0, tw1, tw2, (hipStream_t)s);
+}
+int fpm_kspace_apply_f32(void* f, const fpm_kop* op, const fpm_mesh* m, int64_t p, fpm_stream_t s) {
+  return kspace_apply_impl<float>(f, op, m, p, (hipStream_t)s);
+}
+int fpm_kspace_apply_f64(void* f, const fpm_kop* op, const fpm_mesh* m, int64_t p, fpm_stream_t s) {
+  return kspace_apply_impl<double>(f, op, m, p, (hipStream_t)s);
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Shared device helpers for the FastPM particle-mesh kernels (gfx950 / CDNA4).
+//
+// Conventions used by every kernel in this library:
+//  * Real meshes are C-ordered [n0][n1][n2] with no padding.
+//  * Half-spectrum complex meshes are [n0][n1][pitch2] with pitch2 >= n2/2+1;
+//    mode (i, j, k) holds k_x = 2*pi*fftfreq(i), k_y = 2*pi*fftfreq(j),
+//    k_z = 2*pi*k over the box side.  The forward transform carries 1/(n0 n1 n2).
+//  * Particle positions are split as X = Q + S where Q is either an implicit
+//    Lagrangian lattice (particle p -> lattice cell (p/(ny nz), (p/nz)%ny, p%nz))
+//    or an explicit [n][3] array, and S is the displacement [n][3].
+//  * A grid point (i, j, k) sits at position (i, j, k) * BoxSize / Nmesh.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/fpm.h"
+
+namespace fpm {
+
+template <typename T> struct cvec;
+template <> struct cvec<float> { using type = float2; };
+template <> struct cvec<double> { using type = double2; };
+template <typename T> using C = typename cvec<T>::type;
+
+template <typename V> __host__ __device__ __forceinline__ V cmk(decltype(V::x) a, decltype(V::x) b) { V r; r.x = a; r.y = b; return r; }
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cmk<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return cmk<V>(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) { return cmk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename V> __device__ __forceinline__ V cconj(V a) { return cmk<V>(a.x, -a.y); }
+template <typename V> __device__ __forceinline__ V cscale(V a, decltype(V::x) s) { return cmk<V>(a.x * s, a.y * s); }
+
+// Map a linear block id so that consecutive tiles land on the same XCD.
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8);
+// this bijection gives XCD x a contiguous range of tiles, so neighbouring tiles
+// share that XCD's L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nblk) {
+  const int64_t q = nblk >> 3, r = nblk & 7;
+  const int64_t x = bid & 7, k = bid >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ int64_t wrapi(int64_t i, int64_t n) {
+  int64_t r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// Particle position in mesh-cell units along one axis, returned as an exact
+// integer cell plus a fractional offset in [0, 1).  For the implicit lattice the
+// integer part comes from integer arithmetic, so precision is set by the
+// displacement alone, not by the box size.
+template <typename T>
+struct PartDesc {
+  const T* q;         // explicit Lagrangian positions [n][3] or nullptr
+  const T* s;         // displacement [n][3] (may be nullptr: zero)
+  int64_t n;
+  int32_t lat[3];     // lattice particles per side
+  double lat_shift;   // lattice offset in lattice spacings
+  double ratio[3];    // mesh cells per lattice spacing (Nmesh / Nlattice)
+  T inv_cell[3];      // Nmesh / BoxSize
+};
+
+template <typename T>
+__device__ __forceinline__ void particle_cell(const PartDesc<T>& P, int64_t p, int64_t cell[3], T frac[3]) {
+  if (P.q == nullptr) {
+    const int32_t nz = P.lat[2], ny = P.lat[1];
+    const int64_t iz = p % nz;
+    const int64_t t = p / nz;
+    const int64_t iy = t % ny;
+    const int64_t ix = t / ny;
+    const int64_t li[3] = {ix, iy, iz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double base = ((double)li[d] + P.lat_shift) * P.ratio[d];
+      const double fb = floor(base);
+      T t2 = (T)(base - fb);
+      if (P.s) t2 += P.s[3 * p + d] * P.inv_cell[d];
+      const T ft = floor(t2);
+      cell[d] = (int64_t)fb + (int64_t)ft;
+      frac[d] = t2 - ft;
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T x = P.q[3 * p + d];
+      if (P.s) x += P.s[3 * p + d];
+      x *= P.inv_cell[d];
+      const T fx = floor(x);
+      cell[d] = (int64_t)fx;
+      frac[d] = x - fx;
+    }
+  }
+}
+
+}  // namespace fpm
+
+#define FPM_LAUNCH_CHECK()                                   \
+  do {                                                       \
+    hipError_t e_ = hipGetLastError();                       \
+    if (e_ != hipSuccess) return FPM_ERR_HIP;                \
+  } while (0)

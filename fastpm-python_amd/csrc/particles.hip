@@ -1,0 +1,127 @@
+// Particle-array updates: kick / drift (axpy with a double coefficient), the
+// 2LPT combination of displacement, momentum and force, and positions X = Q + S.
+#include "fpm_common.h"
+
+namespace fpm {
+
+template <typename T>
+int make_desc(const fpm_particles* part, const fpm_mesh* m, PartDesc<T>& P);
+
+static unsigned grid_elems(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+template <typename T>
+__global__ void k_axpy(T* y, const T* x, int64_t n, T a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+template <typename T>
+__global__ void k_lpt_combine(const T* dx1, const T* dx2, T* S, T* P, T* F, int64_t n, T c0, T c1,
+                              T c2, T c3, T c4, T c5) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T a = dx1[i];
+    const T b = dx2 ? dx2[i] : (T)0;
+    S[i] = c0 * a + c1 * b;
+    P[i] = c2 * a + c3 * b;
+    F[i] = c4 * a + c5 * b;
+  }
+}
+
+// Positions in box units.  For the lattice the integer lattice coordinate is
+// converted once; X = (i + shift) * L / Nlat + S.
+template <typename T>
+__global__ void k_positions(PartDesc<T> P, double sx, double sy, double sz, T* out) {
+  const double sp[3] = {sx, sy, sz};
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < P.n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    if (P.q == nullptr) {
+      const int64_t iz = p % P.lat[2];
+      const int64_t t = p / P.lat[2];
+      const int64_t li[3] = {t / P.lat[1], t % P.lat[1], iz};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T q = (T)(((double)li[d] + P.lat_shift) * sp[d]);
+        out[3 * p + d] = q + (P.s ? P.s[3 * p + d] : (T)0);
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out[3 * p + d] = P.q[3 * p + d] + (P.s ? P.s[3 * p + d] : (T)0);
+    }
+  }
+}
+
+template <typename T>
+static int axpy_impl(void* y, const void* x, int64_t n, double a, hipStream_t st) {
+  if (n < 0 || (n > 0 && (!y || !x))) return FPM_ERR_ARG;
+  if (n == 0) return FPM_OK;
+  hipLaunchKernelGGL(k_axpy<T>, dim3(grid_elems(n)), dim3(256), 0, st, static_cast<T*>(y),
+                     static_cast<const T*>(x), n, (T)a);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
+static int combine_impl(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,
+                        const double* c, hipStream_t st) {
+  if (n < 0 || !c || (n > 0 && (!dx1 || !S || !P || !F))) return FPM_ERR_ARG;
+  if (n == 0) return FPM_OK;
+  hipLaunchKernelGGL(k_lpt_combine<T>, dim3(grid_elems(n)), dim3(256), 0, st,
+                     static_cast<const T*>(dx1), static_cast<const T*>(dx2), static_cast<T*>(S),
+                     static_cast<T*>(P), static_cast<T*>(F), n, (T)c[0], (T)c[1], (T)c[2],
+                     (T)c[3], (T)c[4], (T)c[5]);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
+static int positions_impl(const fpm_particles* part, const fpm_mesh* m, void* out,
+                          hipStream_t st) {
+  PartDesc<T> P;
+  int rc = make_desc<T>(part, m, P);
+  if (rc) return rc;
+  if (!out) return FPM_ERR_ARG;
+  if (P.n == 0) return FPM_OK;
+  double sp[3] = {1, 1, 1};
+  if (!part->q)
+    for (int d = 0; d < 3; ++d) sp[d] = m->box[d] / (double)part->lattice[d];
+  hipLaunchKernelGGL(k_positions<T>, dim3(grid_elems(P.n)), dim3(256), 0, st, P, sp[0], sp[1],
+                     sp[2], static_cast<T*>(out));
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+}  // namespace fpm
+
+using namespace fpm;
+
+extern "C" {
+
+int fpm_axpy_f32(void* y, const void* x, int64_t n, double a, fpm_stream_t s) {
+  return axpy_impl<float>(y, x, n, a, (hipStream_t)s);
+}
+int fpm_axpy_f64(void* y, const void* x, int64_t n, double a, fpm_stream_t s) {
+  return axpy_impl<double>(y, x, n, a, (hipStream_t)s);
+}
+int fpm_lpt_combine_f32(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,
+                        const double* c, fpm_stream_t s) {
+  return combine_impl<float>(dx1, dx2, S, P, F, n, c, (hipStream_t)s);
+}
+int fpm_lpt_combine_f64(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,
+                        const double* c, fpm_stream_t s) {
+  return combine_impl<double>(dx1, dx2, S, P, F, n, c, (hipStream_t)s);
+}
+int fpm_positions_f32(const fpm_particles* part, const fpm_mesh* mesh, void* out, fpm_stream_t s) {
+  return positions_impl<float>(part, mesh, out, (hipStream_t)s);
+}
+int fpm_positions_f64(const fpm_particles* part, const fpm_mesh* mesh, void* out, fpm_stream_t s) {
+  return positions_impl<double>(part, mesh, out, (hipStream_t)s);
+}
+
+}  // extern "C"

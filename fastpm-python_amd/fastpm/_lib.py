@@ -1,0 +1,157 @@
+"""ctypes binding of libfpm.so, the C ABI over the gfx950 HIP kernels.
+
+The library is built in-tree (``make -C fastpm-python_amd/csrc``).  There is no
+fallback: if the library is missing, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfpm.so")
+
+FPM_OK = 0
+
+
+class FpmParticles(ctypes.Structure):
+    _fields_ = [
+        ("q", ctypes.c_void_p),
+        ("s", ctypes.c_void_p),
+        ("n", ctypes.c_int64),
+        ("lattice", ctypes.c_int32 * 3),
+        ("pad_", ctypes.c_int32),
+        ("lattice_shift", ctypes.c_double),
+    ]
+
+
+class FpmMesh(ctypes.Structure):
+    _fields_ = [
+        ("nmesh", ctypes.c_int32 * 3),
+        ("pad_", ctypes.c_int32),
+        ("box", ctypes.c_double * 3),
+    ]
+
+
+class FpmKop(ctypes.Structure):
+    _fields_ = [
+        ("factor", ctypes.c_double),
+        ("r_split", ctypes.c_double),
+        ("laplace", ctypes.c_int32),
+        ("ngrad", ctypes.c_int32),
+        ("grad_dir", ctypes.c_int32 * 2),
+        ("grad_order", ctypes.c_int32),
+        ("deconv", ctypes.c_int32),
+    ]
+
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+_PP = ctypes.POINTER(FpmParticles)
+_PM = ctypes.POINTER(FpmMesh)
+_PK = ctypes.POINTER(FpmKop)
+
+# name -> (restype, argtypes); every compute function exists as _f32 and _f64
+_TYPED = {
+    "fpm_paint": (_int, [_PP, _PM, _int, _dbl, _vp, _int, _vp]),
+    "fpm_readout": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _vp, _vp]),
+    "fpm_fft_r2c": (_int, [_vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
+    "fpm_fft_c2r": (_int, [_vp, _int, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
+    "fpm_kspace_apply": (_int, [_vp, _PK, _PM, _i64, _vp]),
+    "fpm_whitenoise": (_int, [_vp, _PM, _i64, ctypes.c_uint64, _int, _vp]),
+    "fpm_linear_scale": (_int, [_vp, _PM, _i64, _vp, _i64, _vp]),
+    "fpm_lpt2_source": (_int, [_vp, _vp, _i64, _vp]),
+    "fpm_axpy": (_int, [_vp, _vp, _i64, _dbl, _vp]),
+    "fpm_lpt_combine": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
+    "fpm_positions": (_int, [_PP, _PM, _vp, _vp]),
+    "fpm_power_bin": (_int, [_vp, _vp, _PM, _i64, _int, _dbl, _dbl, _vp, _i64, _vp, _vp]),
+    "fpm_twiddles": (_int, [_int, _vp]),
+}
+_PLAIN = {
+    "fpm_version": (ctypes.c_char_p, []),
+    "fpm_strerror": (ctypes.c_char_p, [_int]),
+    "fpm_fft_supported": (_int, [_int]),
+    "fpm_power_work_len": (_i64, [_PM, _int]),
+    "fpm_growth": (_int, [_dbl, _dbl, _dbl, _dbl, _vp, _int, _vp]),
+}
+
+
+def exported_names():
+    names = list(_PLAIN)
+    for n in _TYPED:
+        names += [n + "_f32", n + "_f64"]
+    return names
+
+
+_lib = None
+
+
+def lib():
+    """Load libfpm.so once; raise if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libfpm.so not found at %s: build it with `make -C fastpm-python_amd/csrc` "
+                "(there is no CPU fallback)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for n, (rt, at) in _PLAIN.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = rt, at
+        for n, (rt, at) in _TYPED.items():
+            for suf in ("_f32", "_f64"):
+                f = getattr(L, n + suf)
+                f.restype, f.argtypes = rt, at
+        _lib = L
+    return _lib
+
+
+def suffix(dtype):
+    dtype = np.dtype(dtype)
+    if dtype == np.float32:
+        return "_f32"
+    if dtype == np.float64:
+        return "_f64"
+    raise TypeError("only float32 and float64 storage are supported, got %s" % dtype)
+
+
+def call(name, dtype, *args):
+    """Call a typed entry point and raise on a non-zero status."""
+    f = getattr(lib(), name + suffix(dtype))
+    rc = f(*args)
+    if rc != FPM_OK:
+        raise RuntimeError("%s%s failed: %s (%d)" % (name, suffix(dtype),
+                                                      lib().fpm_strerror(rc).decode(), rc))
+
+
+def call_plain(name, *args):
+    f = getattr(lib(), name)
+    rc = f(*args)
+    if rc != FPM_OK:
+        raise RuntimeError("%s failed: %s (%d)" % (name, lib().fpm_strerror(rc).decode(), rc))
+
+
+def growth(omega_m, omega_lambda, omega_k, a_normalize, a):
+    """Growth table (host C++): rows of D1, D1', D1'', D2, D2', D2'', E, E', Om."""
+    a = np.ascontiguousarray(np.atleast_1d(np.asarray(a, dtype="f8")))
+    out = np.zeros((len(a), 9), dtype="f8")
+    call_plain("fpm_growth", float(omega_m), float(omega_lambda), float(omega_k),
+               float(a_normalize), a.ctypes.data, len(a), out.ctypes.data)
+    return out
+
+
+def twiddles(n, dtype):
+    out = np.zeros(2 * n, dtype=dtype)
+    call("fpm_twiddles", dtype, int(n), out.ctypes.data)
+    return out
+
+
+def fft_supported(n):
+    return bool(lib().fpm_fft_supported(int(n)))
+
+
+def ptr_array(ptrs):
+    arr = (ctypes.c_void_p * len(ptrs))(*[ctypes.c_void_p(int(p)) for p in ptrs])
+    return arr

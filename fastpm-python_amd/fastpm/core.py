@@ -1,0 +1,329 @@
+"""FastPM solver API on one MI355X (reference: fastpm/core.py).
+
+``Solver``, ``StateVector``, ``FastPMStep``, ``leapfrog``,
+``leapfrog_with_lpt`` and ``autostages`` keep the reference's names, argument
+meanings, ownership rules and errors; the arrays of a state are device tensors
+and every particle or mesh operation runs in libfpm's HIP kernels.
+"""
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import _lib
+from .background import MatterDominated
+from .pm import ParticleMesh, Particles, RealField, ComplexField, c2r_many, _stream
+from .force.kernels import laplace, gradient, compensation
+
+
+class StateVector:
+    """Particle state: Q (Lagrangian lattice or explicit positions), displacement
+    S, momentum P, force F ([N, 3] device tensors) and RHO ([N]); time tags in
+    ``a`` (core.py:7-89)."""
+
+    def __init__(self, solver, Q):
+        self.solver = solver
+        self.pm = solver.pm
+        self.Q = Q
+        self.csize = solver.pm.comm.allreduce(len(self.Q))
+        self.dtype = solver.pm.dtype
+        self.cosmology = solver.cosmology
+        self.H0 = 100.0  # km/s / (Mpc/h)
+        self.GM0 = (self.H0 ** 2 / (4 * np.pi) * 1.5 * self.cosmology.Om0
+                    * float(np.prod(self.pm.BoxSize)) / self.csize)
+        n = len(Q)
+        dev, dt = self.pm.device, self.pm.tdtype
+        self.S = torch.zeros((n, 3), dtype=dt, device=dev)
+        self.P = torch.zeros((n, 3), dtype=dt, device=dev)
+        self.F = torch.zeros((n, 3), dtype=dt, device=dev)
+        self.RHO = torch.zeros((n,), dtype=dt, device=dev)
+        self.a = dict(S=None, P=None, F=None)
+
+    def copy(self):
+        """Deep copy of S, P, F, RHO; Q is shared (core.py:26-34)."""
+        obj = object.__new__(type(self))
+        obj.__dict__.update(self.__dict__)
+        obj.S = self.S.clone()
+        obj.P = self.P.clone()
+        obj.F = self.F.clone()
+        obj.RHO = self.RHO.clone()
+        obj.a = dict(self.a)
+        return obj
+
+    @property
+    def synchronized(self):
+        a = self.a["S"]
+        return a == self.a["P"] and a == self.a["F"]
+
+    @property
+    def particles(self):
+        return Particles(self.Q, self.S)
+
+    @property
+    def X(self):
+        """Positions Q + S (a new tensor)."""
+        return self.pm.positions(self.particles)
+
+    @property
+    def V(self):
+        """Peculiar velocity P H0 / a_P (a new tensor)."""
+        return self.P * (self.H0 / self.a["P"])
+
+    def to_mesh(self):
+        """Paint X on the particle mesh with unit mass (core.py:50-55)."""
+        return self.pm.paint(self.particles, hold=False)
+
+    def save(self, filename, attrs={}):
+        """Snapshot directory: ``Header.json`` with the attributes written by
+        core.py:65-89 and ``1/Position.npy``, ``1/Velocity.npy`` (km/s),
+        ``1/Density.npy``."""
+        a = self.a["S"]
+        os.makedirs(os.path.join(filename, "1"), exist_ok=True)
+        header = {}
+        for key in ["Om0", "Tcmb0", "Neff", "Ob0", "Ode0", "m_nu"]:
+            v = getattr(self.cosmology, key, None)
+            if v is not None:
+                header[key] = v if np.isscalar(v) else list(np.ravel(v))
+        header["Time"] = a
+        header["h"] = self.cosmology.H0 / self.H0
+        header["RSDFactor"] = 1.0 / (self.H0 * a * self.cosmology.efunc(1.0 / a - 1))
+        for key in attrs:
+            try:
+                json.dumps(attrs[key])
+                header[key] = attrs[key]
+            except (TypeError, ValueError):
+                pass
+        with open(os.path.join(filename, "Header.json"), "w") as f:
+            json.dump(header, f, indent=1, default=float)
+        np.save(os.path.join(filename, "1", "Position.npy"), self.X.cpu().numpy())
+        np.save(os.path.join(filename, "1", "Velocity.npy"), self.V.cpu().numpy())
+        np.save(os.path.join(filename, "1", "Density.npy"), self.RHO.cpu().numpy())
+
+
+class Solver:
+    """PM solver (core.py:91-152).
+
+    pm : ParticleMesh of the particle lattice; the force mesh has Nmesh * B.
+    cosmology : any object with ``Om0`` (``efunc``/``H0``/... for ``save``).
+    a_linear : scale factor at which the linear field has D1 = 1.
+    compensate : divide the force by the mass-assignment window (off in the
+        reference; used by the window-compensated benchmark configurations).
+    """
+
+    def __init__(self, pm, cosmology, B=1, a_linear=1.0, compensate=False):
+        if not hasattr(cosmology, "Om0"):
+            raise TypeError("cosmology must provide Om0")
+        fpm = ParticleMesh(Nmesh=pm.Nmesh * B, BoxSize=pm.BoxSize, dtype=pm.dtype, comm=pm.comm,
+                           resampler=pm.resampler, device=pm.device)
+        self.pm = pm
+        self.fpm = fpm
+        self.B = B
+        self.cosmology = cosmology
+        self.a_linear = a_linear
+        self.compensate = bool(compensate)
+
+    @property
+    def nbodystep(self):
+        return FastPMStep(self)
+
+    def whitenoise(self, seed, unitary=False):
+        return self.pm.generate_whitenoise(seed, type="complex", unitary=unitary)
+
+    def linear(self, whitenoise, Pk):
+        """delta_k = sqrt(P(|k|)/V) w_k (core.py:116-118).  P is evaluated on the
+        host once per distinct integer |n|^2 and applied by table lookup."""
+        pm = whitenoise.pm
+        box = pm.BoxSize
+        if not np.allclose(box, box[0], rtol=0, atol=0):
+            raise NotImplementedError("linear() needs an isotropic box")
+        nmax = int(sum((int(n) // 2) ** 2 for n in pm.Nmesh))
+        n2 = np.arange(nmax + 1, dtype="f8")
+        k = 2 * np.pi * np.sqrt(n2) / box[0]
+        try:
+            pk = np.asarray(Pk(k), dtype="f8")
+        except Exception:
+            pk = np.array([float(Pk(x)) for x in k])
+        with np.errstate(invalid="ignore"):
+            tab = np.sqrt(pk / float(np.prod(box)))
+        tab[0] = 0.0
+        tab = np.nan_to_num(tab, nan=0.0, posinf=0.0)
+        tab_d = torch.from_numpy(tab).to(pm.device)
+        out = whitenoise.copy()
+        _lib.call("fpm_linear_scale", pm.dtype, out._t.data_ptr(), pm._desc, pm.pitch2,
+                  tab_d.data_ptr(), len(tab), _stream())
+        out._table = tab_d  # the table lives as long as the field
+        return out
+
+    def lpt(self, linear, Q, a, order=2):
+        """2LPT initial state at a (core.py:120-145)."""
+        assert order in (1, 2)
+        from .force.lpt import lpt1, lpt2source
+
+        state = StateVector(self, Q)
+        pt = MatterDominated(self.cosmology.Om0, a=[a], a_normalize=self.a_linear)
+        D1, D2 = pt.D1(a), pt.D2(a)
+        a2E = a ** 2 * pt.E(a)
+        dx1 = lpt1(linear, Q)
+        if order == 2:
+            dx2 = lpt1(lpt2source(linear), Q)
+            coef = [D1, D2, a2E * pt.f1(a) * D1, a2E * pt.f2(a) * D2, a2E * pt.gf(a),
+                    a2E * pt.gf2(a)]
+        else:
+            dx2 = None
+            coef = [D1, 0.0, a2E * pt.f1(a) * D1, 0.0, a2E * pt.gf(a), 0.0]
+        c = np.array(coef, dtype="f8")
+        _lib.call("fpm_lpt_combine", self.pm.dtype, dx1.data_ptr(),
+                  None if dx2 is None else dx2.data_ptr(), state.S.data_ptr(),
+                  state.P.data_ptr(), state.F.data_ptr(), state.S.numel(), c.ctypes.data,
+                  _stream())
+        state.a["S"] = a
+        state.a["P"] = a
+        return state
+
+    def nbody(self, state, stepping, monitor=None):
+        """Apply an action list; mutates and returns ``state`` (core.py:147-152)."""
+        step = self.nbodystep
+        for action, ai, ac, af in stepping:
+            step.run(action, ai, ac, af, state, monitor)
+        return state
+
+
+class FastPMStep:
+    """Kick, drift and PM force (core.py:155-210).  Mesh buffers are allocated
+    once per step object and reused across forces."""
+
+    def __init__(self, solver):
+        self.cosmology = solver.cosmology
+        self.pm = solver.fpm
+        self.solver = solver
+        self._ws = None
+        self._pt = MatterDominated(self.cosmology.Om0, a_normalize=solver.a_linear)
+
+    def run(self, action, ai, ac, af, state, monitor):
+        actions = dict(K=self.Kick, D=self.Drift, F=self.Force)
+        event = actions[action](state, ai, ac, af)
+        if monitor is not None:
+            monitor(action, ai, ac, af, state, event)
+
+    def Kick(self, state, ai, ac, af):
+        assert ac == state.a["F"]
+        fac = self._pt.kick_factor(ai, ac, af)
+        _lib.call("fpm_axpy", state.dtype, state.P.data_ptr(), state.F.data_ptr(),
+                  state.P.numel(), float(fac), _stream())
+        state.a["P"] = af
+
+    def Drift(self, state, ai, ac, af):
+        assert ac == state.a["P"]
+        fac = self._pt.drift_factor(ai, ac, af)
+        _lib.call("fpm_axpy", state.dtype, state.S.data_ptr(), state.P.data_ptr(),
+                  state.S.numel(), float(fac), _stream())
+        state.a["S"] = af
+
+    def workspace(self):
+        if self._ws is None:
+            pm = self.pm
+            self._ws = dict(
+                rho=RealField(pm),
+                delta_k=ComplexField(pm),
+                force=[RealField(pm) for _ in range(3)],
+                work=[pm._complex_tensor() for _ in range(3)],
+            )
+        return self._ws
+
+    def prepare_force(self, state, smoothing):
+        """Paint 1 + delta of the particles on the force mesh (core.py:182-193)."""
+        nbar = 1.0 * state.csize / float(np.prod(self.pm.Nmesh))
+        X1 = state.particles
+        layout = self.pm.decompose(X1, smoothing)
+        rho = self.pm.paint(X1, mass=1.0 / nbar, out=self.workspace()["rho"])
+        return layout, X1, rho
+
+    def force_ops(self):
+        factor = 1.5 * self.cosmology.Om0
+        ops = []
+        for d in range(3):
+            op = laplace * gradient(d, order=1) * factor
+            if self.solver.compensate:
+                op = op * compensation(self.pm.order)
+            ops.append(op)
+        return ops
+
+    def Force(self, state, ai, ac, af):
+        """RHO and F at the particles (core.py:195-210): paint, forward
+        transform, three fused multiplier + inverse transforms, and one fused
+        four-component readout."""
+        assert ac == state.a["S"]
+        ws = self.workspace()
+        layout, X1, rho = self.prepare_force(state, smoothing=None)
+        delta_k = rho.r2c(out=ws["delta_k"])
+        c2r_many(delta_k, self.force_ops(), outs=ws["force"], work=ws["work"])
+        F = state.F
+        self.pm.readout_many([rho] + ws["force"], X1, [state.RHO, F[:, 0], F[:, 1], F[:, 2]])
+        state.a["F"] = af
+        return dict(delta_k=delta_k)
+
+
+def autostages(knots, N, astart=None, N0=None):
+    """N stages that include every knot (core.py:212-259)."""
+    knots = np.array(knots, dtype="f8")
+    knots.sort()
+    stages = np.array([], dtype="f8")
+    if astart is not None and astart != knots.min():
+        assert astart < knots.min()
+        if N0 is None:
+            N0 = 1
+        knots = np.append([astart], knots)
+    else:
+        N0 = 1
+    for i in range(0, len(knots) - 1):
+        da = (knots[-1] - knots[i]) / (N - len(stages) - 1)
+        n_span = int((knots[i + 1] - knots[i]) / da + 0.5)
+        if i == 0 and n_span < N0:
+            n_span = N0
+        add = np.linspace(knots[i], knots[i + 1], n_span, endpoint=False)
+        stages = np.append(stages, add)
+    return np.append(stages, [knots[-1]])
+
+
+def leapfrog(stages):
+    """KDK action list: F at the first stage, then K, D, F, K per step
+    (core.py:261-288)."""
+    if len(stages) == 0:
+        return
+    ai = stages[0]
+    yield "F", ai, ai, ai
+    x, p, f = ai, ai, ai
+    for i in range(len(stages) - 1):
+        a0 = stages[i]
+        a1 = stages[i + 1]
+        ah = (a0 * a1) ** 0.5
+        yield "K", p, f, ah
+        p = ah
+        yield "D", x, p, a1
+        x = a1
+        yield "F", f, x, a1
+        f = a1
+        yield "K", p, f, a1
+        p = a1
+
+
+def leapfrog_with_lpt(stages):
+    """As leapfrog, without the first force (the LPT state carries it)
+    (core.py:290-317)."""
+    if len(stages) == 0:
+        return
+    ai = stages[0]
+    x, p, f = ai, ai, ai
+    for i in range(len(stages) - 1):
+        a0 = stages[i]
+        a1 = stages[i + 1]
+        ah = (a0 * a1) ** 0.5
+        yield "K", p, f, ah
+        p = ah
+        yield "D", x, p, a1
+        x = a1
+        yield "F", f, x, a1
+        f = a1
+        yield "K", p, f, a1
+        p = a1

@@ -1,0 +1,389 @@
+"""Particle mesh on one MI355X: fields as device tensors, operations via libfpm.
+
+This replaces the parts of ``pmesh.pm.ParticleMesh`` that the reference solver
+calls (fastpm/core.py:51-54,102,114,187-205; fastpm/force/lpt.py:11-22;
+fastpm/force/gravity.py:17-24): ``create``, ``paint``, ``readout``, ``r2c``,
+``c2r``, ``apply`` with the reference's k-space kernels,
+``generate_whitenoise``, ``generate_uniform_particle_grid`` and the (trivial,
+single-device) ``decompose`` / ``exchange`` / ``gather``.
+
+Storage: real fields are [n0, n1, n2] tensors; complex fields are
+[n0, n1, pitch2] complex tensors with pitch2 = n2/2+1 rounded up to 128 bytes;
+``ComplexField.value`` is the [n0, n1, n2/2+1] view.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("fastpm-mi355x runs on a ROCm GPU; none is visible (no CPU fallback)")
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+_TORCH = {np.dtype("f4"): torch.float32, np.dtype("f8"): torch.float64}
+_CTORCH = {np.dtype("f4"): torch.complex64, np.dtype("f8"): torch.complex128}
+
+
+class SelfComm:
+    """Stand-in communicator for one device (size 1, rank 0)."""
+
+    size = 1
+    rank = 0
+
+    def allreduce(self, x, op=None):
+        return x
+
+    def allgather(self, x):
+        return [x]
+
+    def bcast(self, x, root=0):
+        return x
+
+    def barrier(self):
+        pass
+
+
+class LatticeGrid:
+    """Uniform particle grid, one particle per cell of an n0 x n1 x n2 mesh, in
+    C order: Q_p = (idx(p) + shift) * BoxSize / Nmesh.  Kept implicit (integer
+    lattice index); ``value`` materialises positions on the device."""
+
+    def __init__(self, pm, shift=0.0):
+        self.pm = pm
+        self.nlat = tuple(int(n) for n in pm.Nmesh)
+        self.shift = float(shift)
+        self.dtype = pm.dtype
+
+    def __len__(self):
+        return int(np.prod(self.nlat))
+
+    @property
+    def shape(self):
+        return (len(self), 3)
+
+    @property
+    def size(self):
+        return 3 * len(self)
+
+    @property
+    def value(self):
+        return self.pm.positions(Particles(self, None))
+
+    def __array__(self, dtype=None):
+        a = self.value.cpu().numpy()
+        return a if dtype is None else a.astype(dtype)
+
+
+class Particles:
+    """Positions X = Q + S with Q a LatticeGrid or an explicit [N, 3] device
+    tensor, and S a [N, 3] device tensor (or None)."""
+
+    def __init__(self, Q, S=None):
+        self.Q = Q
+        self.S = S
+
+    def __len__(self):
+        return len(self.Q)
+
+    def desc(self, dtype):
+        p = _lib.FpmParticles()
+        p.n = len(self.Q)
+        if isinstance(self.Q, LatticeGrid):
+            p.q = None
+            for d in range(3):
+                p.lattice[d] = self.Q.nlat[d]
+            p.lattice_shift = self.Q.shift
+        else:
+            _check_tensor(self.Q, dtype, (p.n, 3))
+            p.q = self.Q.data_ptr()
+        if self.S is not None:
+            _check_tensor(self.S, dtype, (p.n, 3))
+            p.s = self.S.data_ptr()
+        return p
+
+
+def _check_tensor(t, dtype, shape):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a device tensor, got %s" % type(t))
+    if not t.is_cuda:
+        raise ValueError("tensor must live on the GPU")
+    if t.dtype != _TORCH[np.dtype(dtype)]:
+        raise TypeError("tensor dtype %s does not match the mesh dtype %s" % (t.dtype, dtype))
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError("tensor shape %s, expected %s" % (tuple(t.shape), tuple(shape)))
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+
+
+class Layout:
+    """Domain layout on one device: exchange and gather are identities
+    (pmesh decompose/exchange/gather, fastpm/core.py:187-189,203,207)."""
+
+    def exchange(self, x):
+        return x
+
+    def gather(self, x, mode="sum"):
+        return x
+
+
+class ParticleMesh:
+    def __init__(self, Nmesh, BoxSize, dtype="f4", resampler="cic", comm=None, device=None):
+        _require_gpu()
+        nm = np.array(Nmesh, dtype="i8").reshape(-1)
+        if nm.size == 1:
+            nm = np.repeat(nm, 3)
+        if nm.size != 3:
+            raise ValueError("only 3-dimensional meshes are supported")
+        box = np.array(BoxSize, dtype="f8").reshape(-1)
+        if box.size == 1:
+            box = np.repeat(box, 3)
+        self.Nmesh = nm
+        self.BoxSize = box
+        self.ndim = 3
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in _TORCH:
+            raise TypeError("dtype must be f4 or f8")
+        self.resampler = resampler
+        self.comm = SelfComm() if comm is None else comm
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None \
+            else torch.device(device)
+        for d in range(3):
+            if not _lib.fft_supported(int(nm[d])) and d < 2:
+                raise ValueError("unsupported mesh size %d" % nm[d])
+        if nm[2] % 2 or not _lib.fft_supported(int(nm[2] // 2)):
+            raise ValueError("unsupported mesh size %d (last axis must be even, n/2 supported)"
+                             % nm[2])
+        self.nh = int(nm[2] // 2 + 1)
+        align = 16 if self.dtype == np.float32 else 8
+        self.pitch2 = (self.nh + align - 1) // align * align
+        self._tw = None
+        self._power_work = None
+        self._desc = _lib.FpmMesh()
+        for d in range(3):
+            self._desc.nmesh[d] = int(nm[d])
+            self._desc.box[d] = float(box[d])
+
+    # ------------------------------------------------------------------ helpers
+    @property
+    def order(self):
+        return resampler_order(self.resampler)
+
+    @property
+    def tdtype(self):
+        return _TORCH[self.dtype]
+
+    def twiddles(self):
+        if self._tw is None:
+            self._tw = [torch.from_numpy(_lib.twiddles(int(n), self.dtype)).to(self.device)
+                        for n in self.Nmesh]
+        return self._tw
+
+    def _real_tensor(self):
+        return torch.empty(tuple(int(n) for n in self.Nmesh), dtype=self.tdtype, device=self.device)
+
+    def _complex_tensor(self):
+        return torch.zeros((int(self.Nmesh[0]), int(self.Nmesh[1]), self.pitch2),
+                           dtype=_CTORCH[self.dtype], device=self.device)
+
+    def create(self, mode="real", base=None, value=None):
+        if mode == "real":
+            f = RealField(self)
+        elif mode == "complex":
+            f = ComplexField(self)
+        else:
+            raise ValueError("mode must be 'real' or 'complex'")
+        if value is not None:
+            f.value[...] = value
+        return f
+
+    def decompose(self, pos, smoothing=None):
+        return Layout()
+
+    def generate_uniform_particle_grid(self, shift=0.0, dtype=None):
+        return LatticeGrid(self, shift)
+
+    def _particles(self, pos):
+        if isinstance(pos, Particles):
+            return pos
+        if isinstance(pos, LatticeGrid):
+            return Particles(pos, None)
+        if isinstance(pos, np.ndarray):
+            pos = torch.from_numpy(np.ascontiguousarray(pos, dtype=self.dtype)).to(self.device)
+        return Particles(pos.contiguous(), None)
+
+    def positions(self, parts):
+        """X = Q + S as a [N, 3] device tensor."""
+        parts = self._particles(parts)
+        out = torch.empty((len(parts), 3), dtype=self.tdtype, device=self.device)
+        d = parts.desc(self.dtype)
+        _lib.call("fpm_positions", self.dtype, d, self._desc, out.data_ptr(), _stream())
+        return out
+
+    # ------------------------------------------------------------------ operations
+    def paint(self, pos, mass=1.0, resampler=None, hold=False, out=None):
+        """pmesh ParticleMesh.paint (fastpm/core.py:191): mass assignment."""
+        if out is None:
+            out = RealField(self)
+        parts = self._particles(pos)
+        d = parts.desc(self.dtype)
+        _lib.call("fpm_paint", self.dtype, d, self._desc, resampler_order(resampler or self.resampler),
+                  float(mass), out.value.data_ptr(), 1 if hold else 0, _stream())
+        return out
+
+    def readout_many(self, fields, pos, outs, resampler=None, scales=None):
+        """Fused readout of up to four real fields into strided outputs."""
+        parts = self._particles(pos)
+        d = parts.desc(self.dtype)
+        n = len(fields)
+        meshes = _lib.ptr_array([f.value.data_ptr() for f in fields])
+        optrs = _lib.ptr_array([o.data_ptr() for o in outs])
+        strides = (ctypes_i64 * n)(*[int(o.stride(0)) for o in outs])
+        sc = (ctypes_f64 * n)(*([1.0] * n if scales is None else [float(s) for s in scales]))
+        _lib.call("fpm_readout", self.dtype, d, self._desc,
+                  resampler_order(resampler or self.resampler), n, meshes, optrs, strides, sc,
+                  _stream())
+
+    def generate_whitenoise(self, seed, type="complex", unitary=False, mean=0):
+        """Counter-based complex white noise (replaces pmesh generate_whitenoise,
+        fastpm/core.py:114)."""
+        f = ComplexField(self)
+        _lib.call("fpm_whitenoise", self.dtype, f._t.data_ptr(), self._desc, self.pitch2,
+                  int(seed) & 0xFFFFFFFFFFFFFFFF, 1 if unitary else 0, _stream())
+        if type == "complex":
+            return f
+        if type == "real":
+            return f.c2r()
+        raise ValueError("type must be 'complex' or 'real'")
+
+    def power_work(self, nbins):
+        n = int(_lib.lib().fpm_power_work_len(self._desc, int(nbins)))
+        if self._power_work is None or self._power_work.numel() < n:
+            self._power_work = torch.empty(n, dtype=torch.float64, device=self.device)
+        return self._power_work
+
+
+import ctypes  # noqa: E402
+
+ctypes_i64 = ctypes.c_int64
+ctypes_f64 = ctypes.c_double
+
+
+def resampler_order(resampler):
+    r = str(resampler).lower()
+    if r == "cic":
+        return 2
+    if r == "tsc":
+        return 3
+    raise ValueError("resampler must be 'cic' or 'tsc', got %r" % (resampler,))
+
+
+class RealField:
+    def __init__(self, pm, tensor=None):
+        self.pm = pm
+        self.value = pm._real_tensor() if tensor is None else tensor
+        self.Nmesh = pm.Nmesh
+        self.BoxSize = pm.BoxSize
+        self.ndim = 3
+
+    @property
+    def dtype(self):
+        return self.pm.dtype
+
+    def r2c(self, out=None):
+        """Forward transform with 1/N (pmesh r2c, fastpm/core.py:205).  The real
+        field is left intact; ``out=Ellipsis`` is accepted for API parity."""
+        if out is None or out is Ellipsis:
+            out = ComplexField(self.pm)
+        tw = self.pm.twiddles()
+        _lib.call("fpm_fft_r2c", self.pm.dtype, self.value.data_ptr(), out._t.data_ptr(),
+                  self.pm._desc, self.pm.pitch2, tw[0].data_ptr(), tw[1].data_ptr(),
+                  tw[2].data_ptr(), _stream())
+        return out
+
+    def readout(self, pos, out=None, resampler=None, layout=None):
+        parts = self.pm._particles(pos)
+        if out is None:
+            out = torch.empty(len(parts), dtype=self.pm.tdtype, device=self.pm.device)
+        self.pm.readout_many([self], parts, [out], resampler)
+        return out
+
+    def paint(self, pos, mass=1.0, resampler=None, hold=False, layout=None):
+        self.pm.paint(pos, mass, resampler, hold, out=self)
+        return self
+
+    def copy(self):
+        return RealField(self.pm, self.value.clone())
+
+
+class ComplexField:
+    def __init__(self, pm, tensor=None):
+        self.pm = pm
+        self._t = pm._complex_tensor() if tensor is None else tensor
+        self.Nmesh = pm.Nmesh
+        self.BoxSize = pm.BoxSize
+        self.ndim = 3
+
+    @property
+    def value(self):
+        return self._t[:, :, : self.pm.nh]
+
+    @property
+    def dtype(self):
+        return np.dtype("c8") if self.pm.dtype == np.float32 else np.dtype("c16")
+
+    def copy(self):
+        return ComplexField(self.pm, self._t.clone())
+
+    def apply(self, kernel, out=None, kind=None):
+        """Multiply by one of the k-space kernels of ``fastpm.force.kernels``
+        (evaluated on the device).  ``out=Ellipsis`` applies in place."""
+        from .force.kernels import KernelOp
+
+        if not isinstance(kernel, KernelOp):
+            raise TypeError("apply() takes a fastpm.force.kernels operator on the device path, "
+                            "got %r" % (kernel,))
+        if out is Ellipsis:
+            dst = self
+        else:
+            dst = self.copy() if out is None else out
+            if out is not None:
+                out._t.copy_(self._t)
+        op = kernel.fpm_kop()
+        _lib.call("fpm_kspace_apply", self.pm.dtype, dst._t.data_ptr(), op, self.pm._desc,
+                  self.pm.pitch2, _stream())
+        return dst
+
+    def c2r(self, out=None, ops=None):
+        """Inverse transform (pmesh c2r).  ``ops`` optionally fuses a list of
+        k-space operators into the first pass, giving one output per operator."""
+        return c2r_many(self, [None] if ops is None else ops, None if out in (None, Ellipsis)
+                        else [out])[0]
+
+
+def c2r_many(cfield, ops, outs=None, work=None):
+    """outs[c] = c2r(ops[c] * cfield) for up to six operators; the input is read
+    once and left intact."""
+    pm = cfield.pm
+    n = len(ops)
+    if outs is None:
+        outs = [RealField(pm) for _ in range(n)]
+    if work is None:
+        work = [pm._complex_tensor() for _ in range(n)]
+    kops = (_lib.FpmKop * n)()
+    for c, o in enumerate(ops):
+        if o is None:
+            kops[c].factor = 1.0
+        else:
+            kops[c] = o.fpm_kop() if not isinstance(o, _lib.FpmKop) else o
+    tw = pm.twiddles()
+    _lib.call("fpm_fft_c2r", pm.dtype, cfield._t.data_ptr(), n, kops,
+              _lib.ptr_array([o.value.data_ptr() for o in outs]),
+              _lib.ptr_array([w.data_ptr() for w in work]), pm._desc, pm.pitch2,
+              tw[0].data_ptr(), tw[1].data_ptr(), tw[2].data_ptr(), _stream())
+    return outs

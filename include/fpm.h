@@ -1,0 +1,207 @@
+/*
+ * fpm.h - C ABI of the MI355X particle-mesh library (libfpm.so).
+ *
+ * Every compute entry point enqueues work on the given hipStream_t and returns
+ * an int status (FPM_OK or a negative FPM_ERR_* code); none synchronises the
+ * stream and none allocates device memory.  Device pointers are owned by the
+ * caller.  Each entry point exists for float (_f32) and double (_f64) storage;
+ * the pointer arguments typed void* hold that real type (or its complex pair).
+ *
+ * Mesh layouts
+ *   real field     : [n0][n1][n2] C order, unpadded.
+ *   complex field  : [n0][n1][pitch2] interleaved (re, im) pairs,
+ *                    pitch2 >= n2/2 + 1; mode (i, j, k) has wave numbers
+ *                    2 pi fftfreq(i)/L0, 2 pi fftfreq(j)/L1, 2 pi k/L2.
+ *   The forward transform is normalised by 1/(n0 n1 n2), the inverse is not.
+ *
+ * Particle layout
+ *   Positions are X = Q + S with S a [n][3] displacement array and Q either an
+ *   explicit [n][3] array or, when q == NULL, the implicit lattice
+ *   Q_p = (idx(p) + shift) * BoxSize / lattice, idx(p) the C-order index of p
+ *   on a lattice[0] x lattice[1] x lattice[2] grid.
+ *
+ * Each declaration cites the reference operation it replaces
+ * (rainwoodman/fastpm-python; pmesh is the reference's mesh dependency).
+ */
+#ifndef FPM_H
+#define FPM_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FPM_OK 0
+#define FPM_ERR_ARG (-1)          /* bad shape, size, order or pointer */
+#define FPM_ERR_UNSUPPORTED (-2)  /* FFT length without a compiled plan */
+#define FPM_ERR_HIP (-3)          /* a HIP launch failed */
+#define FPM_ERR_WORKSPACE (-4)    /* workspace too small */
+
+typedef struct hipStream_t_opaque* fpm_stream_t; /* ABI-compatible with hipStream_t */
+
+typedef struct fpm_particles {
+  const void* q;          /* explicit Lagrangian positions [n][3], or NULL for the lattice */
+  const void* s;          /* displacement [n][3], or NULL for zero */
+  int64_t n;              /* particle count */
+  int32_t lattice[3];     /* lattice particles per side (used when q == NULL) */
+  int32_t pad_;
+  double lattice_shift;   /* lattice offset in lattice spacings */
+} fpm_particles;
+
+typedef struct fpm_mesh {
+  int32_t nmesh[3];
+  int32_t pad_;
+  double box[3];
+} fpm_mesh;
+
+/* A k-space multiplier, evaluated per mode and fused into the first pass of the
+ * inverse transform:  factor * [1/k^2 if laplace, 0 at k=0]
+ *                     * prod_g (i a_{dir_g}(k))  (g < ngrad)
+ *                     * exp(-k^2 r_split^2)       (if r_split != 0)
+ *                     / prod_d W(k_d)^deconv      (window compensation)
+ * with a_d(k) = k_d (grad_order 0, zero on the Nyquist plane) or the 4-point
+ * finite difference (8 sin(k h) - sin(2 k h)) / (6 h), h = L/N (grad_order 1),
+ * and W(k) = sinc(k h / 2).
+ * Reference: fastpm/force/kernels.py:3-35 (laplace, gradient, longrange). */
+typedef struct fpm_kop {
+  double factor;
+  double r_split;
+  int32_t laplace;
+  int32_t ngrad;
+  int32_t grad_dir[2];
+  int32_t grad_order;
+  int32_t deconv;
+} fpm_kop;
+
+const char* fpm_version(void);
+const char* fpm_strerror(int status);
+
+/* Twiddle table for a transform axis of length n: n entries exp(-2 pi i m / n),
+ * computed in double on the host and rounded to the storage type.  Writes to
+ * host memory; the caller uploads it.  Returns FPM_ERR_UNSUPPORTED for a
+ * length without a compiled plan. */
+int fpm_fft_supported(int n);
+int fpm_twiddles_f32(int n, float* host_out);
+int fpm_twiddles_f64(int n, double* host_out);
+
+/* Mass assignment.  Replaces pmesh ParticleMesh.paint / RealField.paint
+ * (fastpm/core.py:191 prepare_force, fastpm/core.py:54 to_mesh).
+ * order 2 = CIC, 3 = TSC.  Each particle deposits `mass` with periodic wrap.
+ * accumulate = 0 clears `out` first (hold=False); 1 adds to it (hold=True). */
+int fpm_paint_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, double mass,
+                  void* out, int accumulate, fpm_stream_t stream);
+int fpm_paint_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, double mass,
+                  void* out, int accumulate, fpm_stream_t stream);
+
+/* Window interpolation of ncomp (<= 4) meshes at the particle positions, fused
+ * into one gather.  Replaces RealField.readout (fastpm/core.py:203,
+ * fastpm/force/gravity.py:24, fastpm/force/lpt.py:21).
+ * outs[c][p * out_strides[c]] = scales[c] * sum_w W * meshes[c][cell]. */
+int fpm_readout_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, int ncomp,
+                    const void* const* meshes, void* const* outs, const int64_t* out_strides,
+                    const double* scales, fpm_stream_t stream);
+int fpm_readout_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, int ncomp,
+                    const void* const* meshes, void* const* outs, const int64_t* out_strides,
+                    const double* scales, fpm_stream_t stream);
+
+/* Real-to-half-complex 3D transform, normalised by 1/(n0 n1 n2).  `in` is left
+ * intact.  tw0/tw1/tw2: device twiddle tables for n0, n1, n2.
+ * Replaces RealField.r2c (fastpm/core.py:205, fastpm/force/lpt.py:64). */
+int fpm_fft_r2c_f32(const void* in, void* out, const fpm_mesh* mesh, int64_t pitch2,
+                    const void* tw0, const void* tw1, const void* tw2, fpm_stream_t stream);
+int fpm_fft_r2c_f64(const void* in, void* out, const fpm_mesh* mesh, int64_t pitch2,
+                    const void* tw0, const void* tw1, const void* tw2, fpm_stream_t stream);
+
+/* Multiplier-then-inverse transform for ncomp (<= 6) outputs from one input:
+ * outs[c] = c2r(ops[c](k) * in).  The input is read once; work[c] is a
+ * complex scratch mesh of the same shape per output.  `in` is left intact.
+ * ops == NULL means the identity multiplier.
+ * Replaces ComplexField.apply(...).c2r (fastpm/force/gravity.py:17-22,
+ * fastpm/force/lpt.py:18-20,39-57). */
+int fpm_fft_c2r_f32(const void* in, int ncomp, const fpm_kop* ops, void* const* outs,
+                    void* const* work, const fpm_mesh* mesh, int64_t pitch2,
+                    const void* tw0, const void* tw1, const void* tw2, fpm_stream_t stream);
+int fpm_fft_c2r_f64(const void* in, int ncomp, const fpm_kop* ops, void* const* outs,
+                    void* const* work, const fpm_mesh* mesh, int64_t pitch2,
+                    const void* tw0, const void* tw1, const void* tw2, fpm_stream_t stream);
+
+/* In-place k-space multiplier on a complex field (ComplexField.apply with one
+ * of the kernels of fastpm/force/kernels.py:3-35). */
+int fpm_kspace_apply_f32(void* field, const fpm_kop* op, const fpm_mesh* mesh, int64_t pitch2,
+                         fpm_stream_t stream);
+int fpm_kspace_apply_f64(void* field, const fpm_kop* op, const fpm_mesh* mesh, int64_t pitch2,
+                         fpm_stream_t stream);
+
+/* Complex white noise with <|w|^2> = 1 (|w| = 1 if unitary), Hermitian
+ * consistent on the k_z = 0 and Nyquist planes, zero mode 0.  Counter-based
+ * (Philox4x32-10 keyed by seed, counter = canonical mode index), so the field
+ * does not depend on launch shape.
+ * Replaces pmesh generate_whitenoise(seed, type='complex') (fastpm/core.py:114). */
+int fpm_whitenoise_f32(void* out, const fpm_mesh* mesh, int64_t pitch2, uint64_t seed,
+                       int unitary, fpm_stream_t stream);
+int fpm_whitenoise_f64(void* out, const fpm_mesh* mesh, int64_t pitch2, uint64_t seed,
+                       int unitary, fpm_stream_t stream);
+
+/* delta_k *= table[nx^2 + ny^2 + nz^2] (integer wave numbers), with the device
+ * table holding sqrt(P(k)/V) at k = 2 pi sqrt(n^2)/L for an isotropic box.
+ * Replaces Solver.linear (fastpm/core.py:116-118). */
+int fpm_linear_scale_f32(void* field, const fpm_mesh* mesh, int64_t pitch2, const double* table,
+                         int64_t ntable, fpm_stream_t stream);
+int fpm_linear_scale_f64(void* field, const fpm_mesh* mesh, int64_t pitch2, const double* table,
+                         int64_t ntable, fpm_stream_t stream);
+
+/* 2LPT source in real space from the six second derivatives
+ * phi = {xx, yy, zz, xy, yz, zx}:
+ * out = 3/7 (xx yy + yy zz + zz xx - xy^2 - yz^2 - zx^2).
+ * Replaces the numpy products of fastpm/force/lpt.py:45-62. */
+int fpm_lpt2_source_f32(const void* const* phi, void* out, int64_t ncells, fpm_stream_t stream);
+int fpm_lpt2_source_f64(const void* const* phi, void* out, int64_t ncells, fpm_stream_t stream);
+
+/* y += a * x over n elements (a in double).  Kick and drift updates,
+ * fastpm/core.py:172 (P += fac F) and fastpm/core.py:179 (S += fac P). */
+int fpm_axpy_f32(void* y, const void* x, int64_t n, double a, fpm_stream_t stream);
+int fpm_axpy_f64(void* y, const void* x, int64_t n, double a, fpm_stream_t stream);
+
+/* S = c0 dx1 + c1 dx2; P = c2 dx1 + c3 dx2; F = c4 dx1 + c5 dx2 over n
+ * elements (dx2 may be NULL for first order).  fastpm/core.py:128-141. */
+int fpm_lpt_combine_f32(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,
+                        const double* coef, fpm_stream_t stream);
+int fpm_lpt_combine_f64(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,
+                        const double* coef, fpm_stream_t stream);
+
+/* out[p][d] = Q_p[d] + S_p[d] (StateVector.X, fastpm/core.py:41-43). */
+int fpm_positions_f32(const fpm_particles* part, const fpm_mesh* mesh, void* out,
+                      fpm_stream_t stream);
+int fpm_positions_f64(const fpm_particles* part, const fpm_mesh* mesh, void* out,
+                      fpm_stream_t stream);
+
+/* Shell-binned spectra with double accumulation, deterministic (fixed
+ * reduction order, no atomics).  Bins: [kmin + b dk, kmin + (b+1) dk),
+ * b < nbins; the zero mode is excluded; half-spectrum modes with
+ * 0 < k_z index < n2/2 count twice.  out[b][0..4] = sum w, sum w |k|,
+ * sum w Re(f1 f2*), sum w |f1|^2, sum w |f2|^2 (f2 == NULL: f2 = f1).
+ * work holds fpm_power_work_len() doubles.
+ * Replaces the estimator of fastpm/utils.py:77-164 and the
+ * nbodykit FFTPower call of fastpm/main.py:81-92. */
+int64_t fpm_power_work_len(const fpm_mesh* mesh, int nbins);
+int fpm_power_bin_f32(const void* f1, const void* f2, const fpm_mesh* mesh, int64_t pitch2,
+                      int nbins, double kmin, double dk, double* work, int64_t work_len,
+                      double* out, fpm_stream_t stream);
+int fpm_power_bin_f64(const void* f1, const void* f2, const fpm_mesh* mesh, int64_t pitch2,
+                      int nbins, double kmin, double dk, double* work, int64_t work_len,
+                      double* out, fpm_stream_t stream);
+
+/* Growth functions of a matter + Lambda (+ curvature) universe, host-side
+ * double precision.  For each a[i] writes 9 values to out[9 i ...]:
+ * D1, dD1/dlna, d2D1/dlna2, D2, dD2/dlna, d2D2/dlna2, E, dE/dlna, Omega_m(a),
+ * with D1 and D2 each normalised to 1 at a_normalize.
+ * Replaces fastpm/background.py:4-256 (Perturbation / MatterDominated). */
+int fpm_growth(double omega_m, double omega_lambda, double omega_k, double a_normalize,
+               const double* a, int n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FPM_H */

@@ -1,0 +1,263 @@
+"""Per-kernel parity of the HIP path against the FP64 oracle (GPU only).
+
+Tolerances: float64 storage must agree to rounding (rtol 1e-11 relative to the
+field's scale); float32 storage to a few float32 ulps of the field's scale
+(documented per test)."""
+import numpy as np
+import pytest
+import torch
+
+import fastpm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _pm(nmesh, box, dtype, resampler="cic"):
+    from fastpm.pm import ParticleMesh
+
+    return ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype=dtype, resampler=resampler)
+
+
+def _tol(dtype):
+    return 1e-11 if np.dtype(dtype) == np.float64 else 3e-6
+
+
+def _rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+MESHES = [(8, 8, 8), (16, 16, 16), (32, 16, 8), (12, 24, 48), (64, 64, 64), (96, 48, 32),
+          (128, 128, 128), (256, 8, 16), (512, 4, 8), (8, 512, 4), (4, 8, 1024)]
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+@pytest.mark.parametrize("nmesh", MESHES)
+def test_r2c(gpu, nmesh, dtype):
+    print("r2c", nmesh, dtype, flush=True)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(nmesh)
+    pm = _pm(nmesh, (100.0, 80.0, 60.0), dtype)
+    f = pm.create("real", value=torch.from_numpy(x.astype(dtype)).to(gpu))
+    c = f.r2c()
+    ref = O.r2c(x)
+    got = c.value.cpu().numpy()
+    assert _rel(got, ref) < _tol(dtype) * 4
+    # the input is left intact
+    assert np.array_equal(f.value.cpu().numpy(), x.astype(dtype))
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+@pytest.mark.parametrize("nmesh", MESHES)
+def test_c2r_ops(gpu, nmesh, dtype):
+    print("c2r", nmesh, dtype, flush=True)
+    from fastpm.pm import c2r_many
+    from fastpm.force.kernels import laplace, gradient, longrange, compensation, identity
+
+    rng = np.random.default_rng(2)
+    box = (100.0, 80.0, 60.0)
+    x = rng.standard_normal(nmesh)
+    xk = O.r2c(x)
+    pm = _pm(nmesh, box, dtype)
+    f = pm.create("real", value=torch.from_numpy(x.astype(dtype)).to(gpu))
+    ck = f.r2c()
+    ops = [identity, laplace * gradient(0, 1) * 1.7, laplace * gradient(1, 1) * gradient(2, 1),
+           gradient(2, 0) * compensation(2), laplace * longrange(3.0), compensation(3) * 0.5]
+    refs = [
+        O.c2r(xk, nmesh),
+        O.c2r(xk * O.k_laplace(nmesh, box) * O.k_gradient(0, nmesh, box, 1) * 1.7, nmesh),
+        O.c2r(xk * O.k_laplace(nmesh, box) * O.k_gradient(1, nmesh, box, 1)
+              * O.k_gradient(2, nmesh, box, 1), nmesh),
+        O.c2r(xk * O.k_gradient(2, nmesh, box, 0) * O.k_deconv(nmesh, box, 2), nmesh),
+        O.c2r(xk * O.k_laplace(nmesh, box) * O.k_longrange(nmesh, box, 3.0), nmesh),
+        O.c2r(xk * O.k_deconv(nmesh, box, 3) * 0.5, nmesh),
+    ]
+    outs = c2r_many(ck, ops)
+    for o, r in zip(outs, refs):
+        assert _rel(o.value.cpu().numpy(), r) < _tol(dtype) * 8
+    # identity round trip
+    assert _rel(outs[0].value.cpu().numpy(), x) < _tol(dtype) * 8
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+def test_kspace_apply(gpu, dtype):
+    from fastpm.force.kernels import laplace, gradient
+
+    nmesh, box = (16, 16, 16), (50.0,) * 3
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(nmesh)
+    pm = _pm(nmesh, box, dtype)
+    ck = pm.create("real", value=torch.from_numpy(x.astype(dtype)).to(gpu)).r2c()
+    out = ck.apply(laplace).apply(gradient(1, order=1), out=Ellipsis)
+    ref = O.r2c(x) * O.k_laplace(nmesh, box) * O.k_gradient(1, nmesh, box, 1)
+    assert _rel(out.value.cpu().numpy(), ref) < _tol(dtype) * 4
+
+
+def _particles(n, box, rng, spread):
+    q = rng.uniform(0, box, size=(n, 3))
+    s = rng.normal(0, spread, size=(n, 3))
+    return q, s
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+@pytest.mark.parametrize("order", [2, 3])
+def test_paint_explicit(gpu, dtype, order):
+    from fastpm.pm import Particles
+
+    nmesh, box = (16, 16, 16), 64.0
+    rng = np.random.default_rng(4)
+    q, s = _particles(3000, box, rng, 20.0)  # displacements wrap across the box
+    pm = _pm(nmesh, box, dtype, "cic" if order == 2 else "tsc")
+    parts = Particles(torch.from_numpy(q.astype(dtype)).to(gpu), torch.from_numpy(s.astype(dtype)).to(gpu))
+    rho = pm.paint(parts, mass=0.7)
+    ref = O.paint(q + s, nmesh, (box,) * 3, order, 0.7)
+    assert _rel(rho.value.cpu().numpy(), ref) < (1e-12 if dtype == "f8" else 2e-5)
+    # hold=True accumulates
+    pm.paint(parts, mass=0.3, hold=True, out=rho)
+    ref2 = ref + O.paint(q + s, nmesh, (box,) * 3, order, 0.3)
+    assert _rel(rho.value.cpu().numpy(), ref2) < (1e-12 if dtype == "f8" else 2e-5)
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("B", [1, 2, 3])
+def test_paint_readout_lattice(gpu, dtype, order, B):
+    from fastpm.pm import Particles
+
+    nc, box = 8, 32.0
+    nmesh = (nc * B,) * 3
+    rng = np.random.default_rng(5 + B)
+    pm_p = _pm((nc,) * 3, box, dtype)
+    pm = _pm(nmesh, box, dtype, "cic" if order == 2 else "tsc")
+    Q = pm_p.generate_uniform_particle_grid(shift=0.5 if B == 3 else 0.0)
+    s = rng.normal(0, 3.0, size=(nc ** 3, 3))
+    parts = Particles(Q, torch.from_numpy(s.astype(dtype)).to(gpu))
+    q = O.lattice((nc,) * 3, (box,) * 3, Q.shift)
+    rho = pm.paint(parts)
+    ref = O.paint(q + s, nmesh, (box,) * 3, order)
+    assert _rel(rho.value.cpu().numpy(), ref) < (1e-12 if dtype == "f8" else 2e-5)
+    # positions
+    X = pm.positions(parts).cpu().numpy()
+    assert _rel(X, q + s) < (1e-14 if dtype == "f8" else 1e-6)
+    # fused 4-component readout with scales, into strided outputs
+    meshes = [rng.standard_normal(nmesh) for _ in range(4)]
+    fields = [pm.create("real", value=torch.from_numpy(m.astype(dtype)).to(gpu)) for m in meshes]
+    F = torch.empty((nc ** 3, 3), dtype=pm.tdtype, device=gpu)
+    R = torch.empty((nc ** 3,), dtype=pm.tdtype, device=gpu)
+    pm.readout_many(fields, parts, [R, F[:, 0], F[:, 1], F[:, 2]], scales=[1.0, 2.0, -1.0, 0.5])
+    sc = [1.0, 2.0, -1.0, 0.5]
+    refs = [O.readout(m, q + s, (box,) * 3, order) * c for m, c in zip(meshes, sc)]
+    got = [R.cpu().numpy()] + [F[:, d].cpu().numpy() for d in range(3)]
+    for g, r in zip(got, refs):
+        assert _rel(g, r) < (1e-12 if dtype == "f8" else 2e-5)
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+@pytest.mark.parametrize("unitary", [False, True])
+@pytest.mark.parametrize("nmesh", [(8, 8, 8), (16, 12, 6), (32, 32, 32)])
+def test_whitenoise(gpu, dtype, unitary, nmesh):
+    pm = _pm(nmesh, 100.0, dtype)
+    w = pm.generate_whitenoise(1234, unitary=unitary).value.cpu().numpy()
+    ref = O.whitenoise(nmesh, 1234, unitary)
+    assert _rel(w, ref) < (1e-13 if dtype == "f8" else 1e-6)
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+def test_linear(gpu, dtype):
+    from fastpm.core import Solver
+    from fastpm.cosmology import Cosmology, LinearPower
+
+    cosmo = Cosmology()
+    Pk = LinearPower(cosmo)
+    nmesh, box = (32,) * 3, 200.0
+    pm = _pm(nmesh, box, dtype)
+    solver = Solver(pm, cosmo, B=1)
+    wn = solver.whitenoise(7)
+    d = solver.linear(wn, Pk).value.cpu().numpy()
+    ref = O.linear(O.whitenoise(nmesh, 7), Pk, nmesh, (box,) * 3)
+    assert _rel(d, ref) < (1e-12 if dtype == "f8" else 1e-6)
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+def test_power_bins(gpu, dtype):
+    from fastpm.utils import power_sums
+
+    nmesh, box = (32, 32, 32), 100.0
+    rng = np.random.default_rng(9)
+    a = rng.standard_normal(nmesh)
+    b = a + 0.3 * rng.standard_normal(nmesh)
+    pm = _pm(nmesh, box, dtype)
+    fa = pm.create("real", value=torch.from_numpy(a.astype(dtype)).to(gpu)).r2c()
+    fb = pm.create("real", value=torch.from_numpy(b.astype(dtype)).to(gpu)).r2c()
+    kmax = np.pi * 32 / box
+    got = power_sums(fa, fb, nbins=20, kmin=0.0, kmax=kmax)
+    ref = O.power_bins(fa.value.cpu().numpy().astype("c16"), nmesh, (box,) * 3, 20, 0.0, kmax / 20,
+                       fb.value.cpu().numpy().astype("c16"))
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
+    # determinism: bitwise identical on a repeat
+    got2 = power_sums(fa, fb, nbins=20, kmin=0.0, kmax=kmax)
+    assert np.array_equal(got, got2)
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+def test_axpy_combine(gpu, dtype):
+    from fastpm import _lib
+    from fastpm.pm import _stream
+
+    rng = np.random.default_rng(11)
+    n = 1000
+    x, y, z = (rng.standard_normal(n) for _ in range(3))
+    tx, ty, tz = (torch.from_numpy(v.astype(dtype)).cuda() for v in (x, y, z))
+    _lib.call("fpm_axpy", dtype, ty.data_ptr(), tx.data_ptr(), n, 0.37, _stream())
+    assert _rel(ty.cpu().numpy(), y.astype(dtype) + 0.37 * x.astype(dtype)) < _tol(dtype)
+    S, P, F = (torch.empty(n, dtype=tx.dtype, device="cuda") for _ in range(3))
+    c = np.array([1.5, -2.0, 0.25, 3.0, 7.0, -1.0])
+    _lib.call("fpm_lpt_combine", dtype, tx.data_ptr(), tz.data_ptr(), S.data_ptr(), P.data_ptr(),
+              F.data_ptr(), n, c.ctypes.data, _stream())
+    xs, zs = x.astype(dtype).astype("f8"), z.astype(dtype).astype("f8")
+    for t, (c0, c1) in zip((S, P, F), [(c[0], c[1]), (c[2], c[3]), (c[4], c[5])]):
+        assert _rel(t.cpu().numpy(), c0 * xs + c1 * zs) < _tol(dtype) * 4
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+def test_lpt_ops(gpu, dtype):
+    from fastpm.force.lpt import lpt1, lpt2source
+
+    nmesh, box = (16, 16, 16), 64.0
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(nmesh)
+    pm = _pm(nmesh, box, dtype)
+    dk = pm.create("real", value=torch.from_numpy(x.astype(dtype)).to(gpu)).r2c()
+    dref = O.r2c(x)
+    src = lpt2source(dk).value.cpu().numpy()
+    sref = O.lpt2source(dref, nmesh, (box,) * 3)
+    assert _rel(src, sref) < (1e-11 if dtype == "f8" else 2e-5)
+    Q = pm.generate_uniform_particle_grid(shift=0.25)
+    q = O.lattice(nmesh, (box,) * 3, 0.25)
+    disp = lpt1(dk, Q).cpu().numpy()
+    ref = O.lpt1(dref, q, nmesh, (box,) * 3)
+    assert _rel(disp, ref) < (1e-11 if dtype == "f8" else 1e-5)
+
+
+def test_errors(gpu):
+    from fastpm import _lib
+    from fastpm.pm import ParticleMesh, _stream
+
+    with pytest.raises(ValueError):
+        ParticleMesh(Nmesh=[10, 10, 10], BoxSize=1.0)
+    pm = ParticleMesh(Nmesh=[8, 8, 8], BoxSize=1.0, dtype="f4")
+    with pytest.raises(ValueError):
+        pm.paint(np.zeros((4, 3), "f4"), resampler="pcs")
+    # bad order straight through the ABI
+    f = pm.create("real")
+    d = _lib.FpmParticles()
+    d.n = 8
+    d.lattice[0] = d.lattice[1] = d.lattice[2] = 2
+    with pytest.raises(RuntimeError):
+        _lib.call("fpm_paint", "f4", d, pm._desc, 5, 1.0, f.value.data_ptr(), 0, _stream())
+    # lattice size inconsistent with n
+    d.n = 9
+    with pytest.raises(RuntimeError):
+        _lib.call("fpm_paint", "f4", d, pm._desc, 2, 1.0, f.value.data_ptr(), 0, _stream())

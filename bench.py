@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the FastPM particle-mesh step on one MI355X.
+
+Workload: a single box of side 1000 Mpc/h with 256^3 particles on a 512^3 force
+mesh (B = 2), flat LCDM (Om0 = 0.3, h = 0.7, ns = 0.96, sigma8 = 0.8) with
+the no-wiggle Eisenstein-Hu linear power, white-noise seed 7, 2LPT at a = 0.1
+and 40 kick-drift-kick steps linear in a to a = 1, CIC paint/readout with
+window compensation.  Inputs are synthetic (generated on the device).
+
+A step is one K, D, F, K cycle.  The float32 run (integer lattice index plus
+float32 displacement) is timed over --steps steps after --warmup untimed
+steps, then run on to a = 1; the float64 plain run (one transform and one
+readout per force component) is timed the same way.  The final states are
+compared: P(k) relative error for k < k_Nyq/2, r(k), and the largest periodic
+displacement difference.  One JSON line is printed.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fastpm-python_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nc", type=int, default=256)
+    ap.add_argument("--B", type=int, default=2)
+    ap.add_argument("--box", type=float, default=1000.0)
+    ap.add_argument("--nsteps", type=int, default=40, help="steps from a=0.1 to a=1")
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the float64 plain run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
+    return ap.parse_args()
+
+
+def run_sim(args, dtype, fused, timed):
+    """Runs the whole simulation; times `timed` steps after `args.warmup`."""
+    import numpy as np
+    import torch
+    from fastpm.core import Solver, leapfrog, KernelTimer
+    from fastpm.pm import ParticleMesh
+    from fastpm.cosmology import Cosmology, LinearPower
+
+    cosmo = Cosmology(Om0=0.3, h=0.7, n_s=0.96, sigma8=0.8, Ob0=0.05, Tcmb0=2.7255)
+    Pk = LinearPower(cosmo)
+    pm = ParticleMesh(Nmesh=[args.nc] * 3, BoxSize=args.box, dtype=dtype, resampler="cic")
+    solver = Solver(pm, cosmo, B=args.B, compensate=True, fused=fused)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dlin = solver.linear(solver.whitenoise(args.seed), Pk)
+    Q = pm.generate_uniform_particle_grid(shift=0.0)
+    state = solver.lpt(dlin, Q, a=0.1, order=2)
+    del dlin
+    torch.cuda.synchronize()
+    t_ic = time.perf_counter() - t0
+    stages = np.linspace(0.1, 1.0, args.nsteps + 1)
+    actions = list(leapfrog(stages))
+    step = solver.nbodystep
+    step.run(*actions[0], state, None)  # initial force
+    body = actions[1:]
+    W, K = args.warmup, timed
+    if W + K > args.nsteps:
+        raise SystemExit("--warmup + --steps exceeds the %d steps of the run" % args.nsteps)
+    for act in body[: 4 * W]:
+        step.run(*act, state, None)
+    timer = KernelTimer()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step.timer = timer
+    for act in body[4 * W: 4 * (W + K)]:
+        step.run(*act, state, None)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    step.timer = None
+    kern = timer.summary()
+    for act in body[4 * (W + K):]:
+        step.run(*act, state, None)
+    torch.cuda.synchronize()
+    return dict(state=state, solver=solver, elapsed=elapsed, kern=kern, t_ic=t_ic,
+                npart=len(Q), nmesh=int(solver.fpm.Nmesh[0]))
+
+
+def gates(r32, r64, args):
+    """P(k) error, r(k) and displacement error of float32 against float64."""
+    import numpy as np
+    import torch
+    from fastpm.pm import RealField
+    from fastpm.utils import power
+
+    s64, s32 = r64["state"], r32["state"]
+    fpm64 = r64["solver"].fpm
+    m64 = fpm64.paint(s64.particles, mass=1.0)
+    m32 = r32["solver"].fpm.paint(s32.particles, mass=1.0)
+    m32d = RealField(fpm64, m32.value.double().contiguous())
+    del m32
+    c64, c32 = m64.r2c(), m32d.r2c()
+    k, p64, _ = power(c64, nbins=32)
+    _, p32, _ = power(c32, nbins=32)
+    _, _, _, r = power(c32, c64, nbins=32)
+    knyq = np.pi * r64["nmesh"] / args.box
+    sel = (k < 0.5 * knyq) & np.isfinite(p64)
+    dx = (s32.X.double() - s64.X + args.box / 2) % args.box - args.box / 2
+    return dict(pk_rel_err_max=float(np.max(np.abs(p32[sel] / p64[sel] - 1))),
+                rk_min=float(np.min(r[sel])), max_disp_err_mpch=float(dx.abs().max()),
+                max_disp_err_cells=float(dx.abs().max()) * r64["nmesh"] / args.box)
+
+
+def cpu_baseline(args, r64):
+    """One KDK step of the FP64 oracle at the same workload, from the GPU
+    float64 state, on this host's cores."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import fastpm_oracle as O
+
+    st = r64["state"]
+    cores = os.cpu_count() or 1
+    O.set_fft_workers(min(cores, 16))
+    osolver = O.Solver(args.nc, args.box, 0.3, B=args.B, resampler="cic", deconv=2)
+    ost = O.State(O.lattice((args.nc,) * 3, (args.box,) * 3))
+    ost.S[...] = st.S.cpu().numpy()
+    ost.P[...] = st.P.cpu().numpy()
+    ost.F[...] = st.F.cpu().numpy()
+    ost.a = dict(S=1.0, P=1.0, F=1.0)
+    a0, a1 = 1.0, 1.0 + 1.0 / args.nsteps
+    acts = list(O.leapfrog([a0, a1]))[1:]
+    t0 = time.perf_counter()
+    osolver.nbody(ost, acts)
+    el = time.perf_counter() - t0
+    return dict(value=len(ost.Q) / el, unit="particle-steps/s", cores=min(cores, 16), kind="port",
+                sample="one KDK step (K, D, F, K) of the float64 numpy/scipy oracle at 256^3 "
+                       "particles / 512^3 mesh, %.1f s wall" % el)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (requested --gpus %d)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    print("bench: float32 fused run", file=sys.stderr, flush=True)
+    r32 = run_sim(args, "f4", True, args.steps)
+    ms32 = r32["elapsed"] / args.steps * 1e3
+    Np = r32["npart"]
+    Nc = r32["nmesh"] ** 3
+    value = Np * args.steps / r32["elapsed"]
+    out = {
+        "metric": "particle-steps/s at 256³ ptcl/512³ mesh, FP32 vs FP64 GPU, P(k) rel err ≤1e-3",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "single box L=1000 Mpc/h, %d^3 particles, %d^3 mesh (B=%d), CIC "
+                               "with window compensation, 2LPT at a=0.1, %d KDK steps to a=1"
+                               % (args.nc, r32["nmesh"], args.B, args.nsteps),
+                   "particles": Np, "mesh": Nc, "seed": args.seed},
+    }
+    # roofline of the dominant kernel (readout: one kernel per call)
+    kern = r32["kern"]
+    out["kernels_ms"] = {k: {"calls": c, "mean_ms": m} for k, (c, m) in kern.items()}
+    ro_calls, ro_ms = kern["readout"]
+    ro_bytes = 16 * Nc + 28 * Np  # 4 meshes read once + S in + RHO/F out
+    ach = ro_bytes / (ro_ms * 1e-3) / 1e9
+    out["roofline"] = {"kernel": "k_readout<float,2,4> (fused RHO+F CIC readout)", "bound": "hbm",
+                       "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                       "algorithmic_bytes": ro_bytes}
+    step_bytes = 36 * Nc + 32 * (r32["nmesh"] ** 2 * (r32["nmesh"] // 2 + 1)) + 148 * Np
+    out["step_roofline"] = {"algorithmic_bytes": step_bytes,
+                            "achieved_GBs": step_bytes / (ms32 * 1e-3) / 1e9,
+                            "frac": step_bytes / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    out["ic_seconds"] = r32["t_ic"]
+    if not args.no_fp64:
+        print("bench: float64 plain run", file=sys.stderr, flush=True)
+        r64 = run_sim(args, "f8", False, args.steps)
+        ms64 = r64["elapsed"] / args.steps * 1e3
+        out["fp64_plain"] = {"ms_per_step": ms64, "value": Np * args.steps / r64["elapsed"],
+                             "kernels_ms": {k: {"calls": c, "mean_ms": m}
+                                            for k, (c, m) in r64["kern"].items()}}
+        out["speedup_vs_fp64_plain"] = ms64 / ms32
+        print("bench: accuracy gates", file=sys.stderr, flush=True)
+        out["accuracy"] = gates(r32, r64, args)
+        del r32
+        torch.cuda.empty_cache()
+        if not args.no_cpu:
+            print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
+            out["cpu_baseline"] = cpu_baseline(args, r64)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -109,9 +109,12 @@ class Solver:
     a_linear : scale factor at which the linear field has D1 = 1.
     compensate : divide the force by the mass-assignment window (off in the
         reference; used by the window-compensated benchmark configurations).
+    fused : True runs the force as one multi-output inverse transform and one
+        four-component readout; False runs the plain sequence of the reference
+        (one inverse transform and one readout per component).
     """
 
-    def __init__(self, pm, cosmology, B=1, a_linear=1.0, compensate=False):
+    def __init__(self, pm, cosmology, B=1, a_linear=1.0, compensate=False, fused=True):
         if not hasattr(cosmology, "Om0"):
             raise TypeError("cosmology must provide Om0")
         fpm = ParticleMesh(Nmesh=pm.Nmesh * B, BoxSize=pm.BoxSize, dtype=pm.dtype, comm=pm.comm,
@@ -122,6 +125,7 @@ class Solver:
         self.cosmology = cosmology
         self.a_linear = a_linear
         self.compensate = bool(compensate)
+        self.fused = bool(fused)
 
     @property
     def nbodystep(self):
@@ -199,6 +203,10 @@ class FastPMStep:
         self.solver = solver
         self._ws = None
         self._pt = MatterDominated(self.cosmology.Om0, a_normalize=solver.a_linear)
+        self.timer = None  # optional KernelTimer recording each library call
+
+    def _mark(self, name):
+        return self.timer.region(name) if self.timer is not None else _NullRegion()
 
     def run(self, action, ai, ac, af, state, monitor):
         actions = dict(K=self.Kick, D=self.Drift, F=self.Force)
@@ -209,15 +217,17 @@ class FastPMStep:
     def Kick(self, state, ai, ac, af):
         assert ac == state.a["F"]
         fac = self._pt.kick_factor(ai, ac, af)
-        _lib.call("fpm_axpy", state.dtype, state.P.data_ptr(), state.F.data_ptr(),
-                  state.P.numel(), float(fac), _stream())
+        with self._mark("kick"):
+            _lib.call("fpm_axpy", state.dtype, state.P.data_ptr(), state.F.data_ptr(),
+                      state.P.numel(), float(fac), _stream())
         state.a["P"] = af
 
     def Drift(self, state, ai, ac, af):
         assert ac == state.a["P"]
         fac = self._pt.drift_factor(ai, ac, af)
-        _lib.call("fpm_axpy", state.dtype, state.S.data_ptr(), state.P.data_ptr(),
-                  state.S.numel(), float(fac), _stream())
+        with self._mark("drift"):
+            _lib.call("fpm_axpy", state.dtype, state.S.data_ptr(), state.P.data_ptr(),
+                      state.S.numel(), float(fac), _stream())
         state.a["S"] = af
 
     def workspace(self):
@@ -236,7 +246,8 @@ class FastPMStep:
         nbar = 1.0 * state.csize / float(np.prod(self.pm.Nmesh))
         X1 = state.particles
         layout = self.pm.decompose(X1, smoothing)
-        rho = self.pm.paint(X1, mass=1.0 / nbar, out=self.workspace()["rho"])
+        with self._mark("paint"):
+            rho = self.pm.paint(X1, mass=1.0 / nbar, out=self.workspace()["rho"])
         return layout, X1, rho
 
     def force_ops(self):
@@ -256,12 +267,65 @@ class FastPMStep:
         assert ac == state.a["S"]
         ws = self.workspace()
         layout, X1, rho = self.prepare_force(state, smoothing=None)
-        delta_k = rho.r2c(out=ws["delta_k"])
-        c2r_many(delta_k, self.force_ops(), outs=ws["force"], work=ws["work"])
+        with self._mark("r2c"):
+            delta_k = rho.r2c(out=ws["delta_k"])
         F = state.F
-        self.pm.readout_many([rho] + ws["force"], X1, [state.RHO, F[:, 0], F[:, 1], F[:, 2]])
+        ops = self.force_ops()
+        if self.solver.fused:
+            with self._mark("c2r"):
+                c2r_many(delta_k, ops, outs=ws["force"], work=ws["work"])
+            with self._mark("readout"):
+                self.pm.readout_many([rho] + ws["force"], X1,
+                                     [state.RHO, F[:, 0], F[:, 1], F[:, 2]])
+        else:
+            with self._mark("readout"):
+                self.pm.readout_many([rho], X1, [state.RHO])
+            for d in range(3):
+                with self._mark("c2r"):
+                    c2r_many(delta_k, [ops[d]], outs=ws["force"][d:d + 1], work=ws["work"][d:d + 1])
+                with self._mark("readout"):
+                    self.pm.readout_many([ws["force"][d]], X1, [F[:, d]])
         state.a["F"] = af
         return dict(delta_k=delta_k)
+
+
+class _NullRegion:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        return False
+
+
+class KernelTimer:
+    """Records HIP events around library calls on the current stream (the
+    stream every libfpm call is enqueued on) and reports per-region averages."""
+
+    def __init__(self):
+        self.events = {}
+
+    def region(self, name):
+        timer = self
+
+        class _R:
+            def __enter__(self_):
+                self_.a = torch.cuda.Event(enable_timing=True)
+                self_.a.record()
+                return self_
+
+            def __exit__(self_, *exc):
+                b = torch.cuda.Event(enable_timing=True)
+                b.record()
+                timer.events.setdefault(name, []).append((self_.a, b))
+                return False
+
+        return _R()
+
+    def summary(self):
+        """name -> (count, mean ms); synchronises the device."""
+        torch.cuda.synchronize()
+        return {k: (len(v), float(np.mean([a.elapsed_time(b) for a, b in v])))
+                for k, v in self.events.items()}
 
 
 def autostages(knots, N, astart=None, N0=None):

@@ -88,14 +88,27 @@ def kvectors(nmesh, box):
     return kx[:, None, None], ky[None, :, None], kz[None, None, :]
 
 
+_FFT_WORKERS = 1
+
+
+def set_fft_workers(n):
+    """Threads used by the oracle's transforms (scipy.fft workers)."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = int(n)
+
+
 def r2c(real):
     """pmesh RealField.r2c: forward transform normalised by 1/N (core.py:205)."""
-    return np.fft.rfftn(real) / real.size
+    import scipy.fft
+
+    return scipy.fft.rfftn(real, workers=_FFT_WORKERS) / real.size
 
 
 def c2r(cplx, nmesh):
     """pmesh ComplexField.c2r: unnormalised inverse of r2c."""
-    return np.fft.irfftn(cplx, s=tuple(nmesh)) * float(np.prod(nmesh))
+    import scipy.fft
+
+    return scipy.fft.irfftn(cplx, s=tuple(nmesh), workers=_FFT_WORKERS) * float(np.prod(nmesh))
 
 
 def whitenoise(nmesh, seed, unitary=False):

@@ -1,0 +1,97 @@
+"""Host-side logic of the product that runs without a GPU: stepping
+generators, kick/drift factors and k-space operator composition."""
+import numpy as np
+import pytest
+
+import fastpm_oracle as O
+
+
+def test_leapfrog_matches_oracle_and_reference_counts():
+    from fastpm.core import leapfrog, leapfrog_with_lpt
+
+    for stages in ([0.1, 1.0], [1.0], np.linspace(0.1, 1.0, 11), [1.0, 0.5, 0.1]):
+        assert list(leapfrog(stages)) == list(O.leapfrog(stages))
+        assert list(leapfrog_with_lpt(stages)) == list(O.leapfrog_with_lpt(stages))
+    assert len(list(leapfrog(np.linspace(0.1, 1.0, 2)))) == 1 + 4 * (2 - 1)
+    assert len(list(leapfrog([1.0]))) == 1
+    assert list(leapfrog([])) == []
+
+
+def test_autostages_matches():
+    from fastpm.core import autostages
+
+    for knots in [[0.1, 1.0], [1.0], [0.5], [0.1, 0.2, 1.0]]:
+        a = autostages(knots, astart=0.1, N=12)
+        assert len(a) == 12
+        for k in knots:
+            assert k in a
+        np.testing.assert_array_equal(a, O.autostages(knots, astart=0.1, N=12))
+
+
+def test_kick_drift_factors_match_oracle():
+    from fastpm.background import MatterDominated
+
+    pt = MatterDominated(0.3075, a_normalize=1.0)
+    osol = O.Solver(8, 100.0, 0.3075)
+    for ai, ac, af in [(0.1, 0.1, 0.1 ** 0.5 * 0.2 ** 0.5), (0.2, 0.25, 0.3), (0.5, 0.7, 1.0)]:
+        assert pt.kick_factor(ai, ac, af) == pytest.approx(osol.kick_factor(ai, ac, af), rel=1e-7)
+        assert pt.drift_factor(ai, ac, af) == pytest.approx(osol.drift_factor(ai, ac, af), rel=1e-7)
+
+
+def test_growth_api_matches_oracle():
+    from fastpm.background import MatterDominated
+
+    a = np.array([0.05, 0.3, 1.0])
+    p = MatterDominated(0.3, a=a, a_normalize=0.5)
+    g = O.Growth(0.3, a=a, a_normalize=0.5)
+    for name in ("D1", "D2", "f1", "f2", "Gp", "gp", "Gf", "gf", "gf2", "E"):
+        np.testing.assert_allclose(getattr(p, name)(a), getattr(g, name)(a), rtol=2e-7,
+                                   err_msg=name)
+
+
+def test_kernel_composition():
+    from fastpm.force.kernels import laplace, gradient, longrange, compensation, KernelOp
+
+    op = laplace * gradient(0, 1) * 1.5 * gradient(2, 1) * compensation(2) * longrange(3.0)
+    k = op.fpm_kop()
+    assert k.laplace == 1 and k.ngrad == 2 and list(k.grad_dir) == [0, 2]
+    assert k.grad_order == 1 and k.deconv == 2 and k.r_split == 3.0 and k.factor == 1.5
+    with pytest.raises(ValueError):
+        laplace * laplace
+    with pytest.raises(ValueError):
+        gradient(0, 0) * gradient(1, 1)
+    with pytest.raises(ValueError):
+        gradient(0) * gradient(1) * gradient(2)
+    with pytest.raises(ValueError):
+        gradient(0, 3)
+    assert isinstance(2.0 * laplace, KernelOp)
+
+
+def test_linear_power_normalisation():
+    from fastpm.cosmology import Cosmology, LinearPower, eh_nowiggle_transfer
+    from scipy.integrate import simpson
+
+    c = Cosmology(sigma8=0.8)
+    Pk = LinearPower(c)
+    # independent sigma_8 by Simpson on a log grid
+    lk = np.linspace(np.log(1e-5), np.log(1e2), 200001)
+    k = np.exp(lk)
+    x = 8.0 * k
+    w = 3 * (np.sin(x) - x * np.cos(x)) / x ** 3
+    s8 = np.sqrt(simpson(k ** 3 * Pk(k) * w * w / (2 * np.pi ** 2), x=lk))
+    assert s8 == pytest.approx(0.8, rel=1e-6)
+    # transfer -> 1 at k -> 0, decreasing, ~ ln(k)/k^2 at large k
+    T = eh_nowiggle_transfer(np.array([1e-6, 1e-3, 1e-1, 1.0, 10.0]), c)
+    assert T[0] == pytest.approx(1.0, abs=1e-4)
+    assert np.all(np.diff(T) < 0)
+    assert Pk(0.0) == 0.0
+
+
+def test_gpu_required():
+    import torch
+    from fastpm.pm import ParticleMesh
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(RuntimeError):
+        ParticleMesh(Nmesh=8, BoxSize=1.0)

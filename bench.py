@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=38)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nc", type=int, default=256)
     ap.add_argument("--B", type=int, default=2)
@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-fp64", action="store_true", help="skip the float64 plain run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device copy (reference kernel for HBM counters)")
     return ap.parse_args()
 
 
@@ -154,6 +155,13 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print("bench: float32 fused run", file=sys.stderr, flush=True)
     r32 = run_sim(args, "f4", True, args.steps)
     ms32 = r32["elapsed"] / args.steps * 1e3

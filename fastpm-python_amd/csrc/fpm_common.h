@@ -60,6 +60,9 @@ struct PartDesc {
 
 template <typename T>
 __device__ __forceinline__ void particle_cell(const PartDesc<T>& P, int64_t p, int64_t cell[3], T frac[3]) {
+  // No FMA contraction: every kernel that places a particle (binning, paint,
+  // readout) must round its position identically.
+#pragma clang fp contract(off)
   if (P.q == nullptr) {
     const int32_t nz = P.lat[2], ny = P.lat[1];
     const int64_t iz = p % nz;

@@ -75,6 +75,12 @@ _PLAIN = {
     "fpm_fft_supported": (_int, [_int]),
     "fpm_power_work_len": (_i64, [_PM, _int]),
     "fpm_growth": (_int, [_dbl, _dbl, _dbl, _dbl, _vp, _int, _vp]),
+    "fpm_tiles_supported": (_int, [_PM]),
+    "fpm_tile_workspace_size": (_i64, [_PM, _int, _i64]),
+    "fpm_tile_misplaced_offset": (_i64, [_PM, _int, _i64]),
+    "fpm_bin_particles_f32": (_int, [_PP, _PM, _int, _vp, _i64, _vp]),
+    "fpm_paint_tiled_f32": (_int, [_PP, _PM, _int, _dbl, _vp, _vp, _i64, _vp]),
+    "fpm_readout_tiled_f32": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
 }
 
 

@@ -204,6 +204,9 @@ class FastPMStep:
         self._ws = None
         self._pt = MatterDominated(self.cosmology.Om0, a_normalize=solver.a_linear)
         self.timer = None  # optional KernelTimer recording each library call
+        # fused float32 path on tile-divisible meshes: binned deterministic
+        # paint and LDS-staged readout
+        self.tiled = solver.fused and self.pm.tiles_supported()
 
     def _mark(self, name):
         return self.timer.region(name) if self.timer is not None else _NullRegion()
@@ -246,8 +249,15 @@ class FastPMStep:
         nbar = 1.0 * state.csize / float(np.prod(self.pm.Nmesh))
         X1 = state.particles
         layout = self.pm.decompose(X1, smoothing)
-        with self._mark("paint"):
-            rho = self.pm.paint(X1, mass=1.0 / nbar, out=self.workspace()["rho"])
+        rho = self.workspace()["rho"]
+        if self.tiled:
+            with self._mark("bin"):
+                self.pm.bin_particles(X1)
+            with self._mark("paint"):
+                self.pm.paint_tiled(X1, mass=1.0 / nbar, out=rho)
+        else:
+            with self._mark("paint"):
+                self.pm.paint(X1, mass=1.0 / nbar, out=rho)
         return layout, X1, rho
 
     def force_ops(self):
@@ -275,8 +285,8 @@ class FastPMStep:
             with self._mark("c2r"):
                 c2r_many(delta_k, ops, outs=ws["force"], work=ws["work"])
             with self._mark("readout"):
-                self.pm.readout_many([rho] + ws["force"], X1,
-                                     [state.RHO, F[:, 0], F[:, 1], F[:, 2]])
+                readout = self.pm.readout_tiled if self.tiled else self.pm.readout_many
+                readout([rho] + ws["force"], X1, [state.RHO, F[:, 0], F[:, 1], F[:, 2]])
         else:
             with self._mark("readout"):
                 self.pm.readout_many([rho], X1, [state.RHO])

@@ -249,6 +249,60 @@ class ParticleMesh:
                   resampler_order(resampler or self.resampler), n, meshes, optrs, strides, sc,
                   _stream())
 
+    # ------------------------------------------------------------------ tiled path
+    def tiles_supported(self):
+        return self.dtype == np.float32 and bool(_lib.lib().fpm_tiles_supported(self._desc))
+
+    def tile_workspace(self, n, order):
+        """Device workspace for binning n particles (cached, grown on demand)."""
+        need = int(_lib.lib().fpm_tile_workspace_size(self._desc, int(order), int(n)))
+        if need < 0:
+            raise ValueError("mesh does not support the tiled path")
+        ws = getattr(self, "_tile_ws", None)
+        if ws is None or ws.numel() < need:
+            ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+            self._tile_ws = ws
+        return ws
+
+    def tile_misplaced(self, n, order):
+        """Particles that paint/readout found outside their binned tile since
+        the last bin call (synchronises)."""
+        off = int(_lib.lib().fpm_tile_misplaced_offset(self._desc, int(order), int(n)))
+        ws = self._tile_ws
+        return int(ws[off:off + 4].view(torch.int32).item())
+
+    def bin_particles(self, pos, resampler=None):
+        parts = self._particles(pos)
+        order = resampler_order(resampler or self.resampler)
+        ws = self.tile_workspace(len(parts), order)
+        _lib.call_plain("fpm_bin_particles_f32", parts.desc(self.dtype), self._desc, order,
+                        ws.data_ptr(), ws.numel(), _stream())
+        return parts
+
+    def paint_tiled(self, pos, mass=1.0, resampler=None, out=None):
+        """Deterministic tiled paint; `pos` must have been binned with
+        bin_particles (same particles and resampler)."""
+        if out is None:
+            out = RealField(self)
+        parts = self._particles(pos)
+        order = resampler_order(resampler or self.resampler)
+        ws = self.tile_workspace(len(parts), order)
+        _lib.call_plain("fpm_paint_tiled_f32", parts.desc(self.dtype), self._desc, order,
+                        float(mass), out.value.data_ptr(), ws.data_ptr(), ws.numel(), _stream())
+        return out
+
+    def readout_tiled(self, fields, pos, outs, resampler=None, scales=None):
+        parts = self._particles(pos)
+        order = resampler_order(resampler or self.resampler)
+        ws = self.tile_workspace(len(parts), order)
+        n = len(fields)
+        meshes = _lib.ptr_array([f.value.data_ptr() for f in fields])
+        optrs = _lib.ptr_array([o.data_ptr() for o in outs])
+        strides = (ctypes_i64 * n)(*[int(o.stride(0)) for o in outs])
+        sc = (ctypes_f64 * n)(*([1.0] * n if scales is None else [float(s) for s in scales]))
+        _lib.call_plain("fpm_readout_tiled_f32", parts.desc(self.dtype), self._desc, order, n,
+                        meshes, optrs, strides, sc, ws.data_ptr(), ws.numel(), _stream())
+
     def generate_whitenoise(self, seed, type="complex", unitary=False, mean=0):
         """Counter-based complex white noise (replaces pmesh generate_whitenoise,
         fastpm/core.py:114)."""

@@ -106,6 +106,28 @@ int fpm_readout_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, 
                     const void* const* meshes, void* const* outs, const int64_t* out_strides,
                     const double* scales, fpm_stream_t stream);
 
+/* Tile-binned paint and readout (float32 storage, meshes whose sides are
+ * multiples of 16).  The mesh is cut into 16^3-cell tiles; fpm_bin_particles
+ * sorts particle indices into per-tile lists held in the caller's workspace
+ * (fpm_tile_workspace_size bytes).  fpm_paint_tiled and fpm_readout_tiled use
+ * those lists, so they must follow a bin call with the same particles, mesh
+ * and order.  The tiled paint accumulates in 64-bit fixed point, is bitwise
+ * reproducible, overwrites `out` (hold=False) and needs no clearing pass.
+ * Same contracts as fpm_paint / fpm_readout (fastpm/core.py:191,203;
+ * fastpm/force/gravity.py:24).  fpm_tile_misplaced_offset gives the byte
+ * offset in the workspace of an int32 counter of particles that paint or
+ * readout found outside their binned tile (always expected to stay 0). */
+int fpm_tiles_supported(const fpm_mesh* mesh);
+int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n);
+int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n);
+int fpm_bin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,
+                          int64_t ws_bytes, fpm_stream_t stream);
+int fpm_paint_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, double mass,
+                        void* out, void* ws, int64_t ws_bytes, fpm_stream_t stream);
+int fpm_readout_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, int ncomp,
+                          const void* const* meshes, void* const* outs, const int64_t* out_strides,
+                          const double* scales, void* ws, int64_t ws_bytes, fpm_stream_t stream);
+
 /* Real-to-half-complex 3D transform, normalised by 1/(n0 n1 n2).  `in` is left
  * intact.  tw0/tw1/tw2: device twiddle tables for n0, n1, n2.
  * Replaces RealField.r2c (fastpm/core.py:205, fastpm/force/lpt.py:64). */

@@ -179,13 +179,30 @@ struct Stages {
 constexpr int round64(int x) { return ((x + 63) / 64) * 64; }
 constexpr int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-// Threads for a tile of E complex elements: about 8 elements per thread.
-constexpr int threads_for(int E) { return clampi(round64(E / 8), 64, 1024); }
+#ifndef FPM_COL_BYTES
+#define FPM_COL_BYTES 64  // bytes of each global row segment of a column tile
+#endif
+#ifndef FPM_EPT_F32
+#define FPM_EPT_F32 8  // complex elements per thread in a column tile (float)
+#endif
+#ifndef FPM_EPT_F64
+#define FPM_EPT_F64 4  // (double)
+#endif
+// Tile shapes chosen by a sweep of row-segment width (64/128/256 bytes) and
+// elements per thread (4/8/16) at 512^3 on gfx950.
 
-// Column tile width: 128-byte global row segments.
-template <typename T> struct ColW;
-template <> struct ColW<float> { static constexpr int BZ = 16; };
-template <> struct ColW<double> { static constexpr int BZ = 8; };
+// Threads for a tile of E complex elements of type T.
+template <typename T> constexpr int threads_for(int E) {
+  return clampi(round64(E / (sizeof(T) == 4 ? FPM_EPT_F32 : FPM_EPT_F64)), 64, 1024);
+}
+
+// Column tile width (z columns per tile), capped so the tile stays within
+// 128 KiB of LDS.
+template <typename T, int N> struct ColW {
+  static constexpr int a = FPM_COL_BYTES / (int)sizeof(C<T>);
+  static constexpr int b = (128 * 1024 / (int)sizeof(C<T>)) / N;
+  static constexpr int BZ = a < b ? a : b;
+};
 
 // Lines per workgroup for the z pass: about 2048 complex elements.
 constexpr int z_lines(int M) {
@@ -247,10 +264,11 @@ __device__ __forceinline__ T kop_mult(const KopPack& K, int c, const AxisTab<T>&
   const T kk = ax.kk + ay.kk + az.kk;
   if (K.laplace[c]) m = (kk == (T)0) ? (T)0 : m / kk;
   if (K.r2[c] != 0.0) m *= exp(-kk * (T)K.r2[c]);
+  const bool o0 = K.grad_order[c] == 0;
   for (int g = 0; g < K.ngrad[c]; ++g) {
     const int d = K.gdir[c][g];
-    const AxisTab<T>& a = d == 0 ? ax : (d == 1 ? ay : az);
-    m *= K.grad_order[c] == 0 ? a.g0 : a.g1;
+    const T gx = o0 ? ax.g0 : ax.g1, gy = o0 ? ay.g0 : ay.g1, gz = o0 ? az.g0 : az.g1;
+    m *= d == 0 ? gx : (d == 1 ? gy : gz);
   }
   const int p = K.deconv[c];
   if (p) {
@@ -286,9 +304,10 @@ struct ColArgs {
   T scale;
   const C<T>* tw;
   int64_t nblk;
-  // k-space multipliers (only for the x pass of the inverse transform, where
-  // e is the x index, outer the y index and z the column).
+  // k-space multipliers, applied before the transform: e runs along mesh axis
+  // eaxis, outer along oaxis (x and y passes), the column along z.
   KopPack kop;
+  int eaxis, oaxis;
 };
 
 template <typename T, int N, int BZ, int NT, bool INV, bool KOP>
@@ -311,10 +330,11 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   AxisTab<T> ty{};
   if constexpr (KOP) {
     if constexpr (TXL)
-      for (int i = threadIdx.x; i < N; i += NT) tx[i] = axis_tab<T>(i, a.kop.nm[0], a.kop.box[0], false);
+      for (int i = threadIdx.x; i < N; i += NT)
+        tx[i] = axis_tab<T>(i, a.kop.nm[a.eaxis], a.kop.box[a.eaxis], false);
     for (int i = threadIdx.x; i < BZ; i += NT)
       tz[i] = axis_tab<T>(z0 + i, a.kop.nm[2], a.kop.box[2], true);
-    ty = axis_tab<T>(outer, a.kop.nm[1], a.kop.box[1], false);
+    ty = axis_tab<T>(outer, a.kop.nm[a.oaxis], a.kop.box[a.oaxis], false);
   }
 
   const V* src = a.in + (int64_t)outer * a.ostride + z0;
@@ -338,8 +358,10 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
         V v = reg[q];
         if constexpr (KOP) {
           const int l = idx % BZ, e = idx / BZ;
-          const AxisTab<T> axe = TXL ? tx[e] : axis_tab<T>(e, a.kop.nm[0], a.kop.box[0], false);
-          const T m = kop_mult<T>(a.kop, c, axe, ty, tz[l]);
+          const AxisTab<T> axe =
+              TXL ? tx[e] : axis_tab<T>(e, a.kop.nm[a.eaxis], a.kop.box[a.eaxis], false);
+          const T m = a.eaxis == 0 ? kop_mult<T>(a.kop, c, axe, ty, tz[l])
+                                   : kop_mult<T>(a.kop, c, ty, axe, tz[l]);
           v = apply_mult(v, m, a.kop.ngrad[c]);
         }
         tile[idx] = v;
@@ -375,6 +397,10 @@ struct ZArgs {
   int64_t nrows;
   int64_t pitch2;
   const C<T>* tw;    // table of length 2M
+  // c2r only: multiply mode k by i a_z(k) first (zgrad = 1), a_z of order
+  // zorder on an axis of n2 points and side L2
+  int zgrad, zorder;
+  double L2;
 };
 
 template <typename T, int M, int NL, int NT>
@@ -449,7 +475,12 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(ZArgs<T> a) {
   for (int idx = threadIdx.x; idx < NL * LP; idx += NT) {
     const int l = idx / LP, k = idx % LP;
     const int64_t row = row0 + l;
-    s[l * LP + k] = row < a.nrows ? a.cin[row * a.pitch2 + k] : cmk<V>((T)0, (T)0);
+    V v = row < a.nrows ? a.cin[row * a.pitch2 + k] : cmk<V>((T)0, (T)0);
+    if (a.zgrad) {
+      const AxisTab<T> az = axis_tab<T>(k, 2 * M, a.L2, true);
+      v = apply_mult(v, a.zorder == 0 ? az.g0 : az.g1, 1);
+    }
+    s[l * LP + k] = v;
   }
   __syncthreads();
   V za[PERP], zb[PERP];
@@ -510,19 +541,18 @@ static bool supported_len(int n) {
 }
 
 template <typename T, int N, bool INV, bool KOP>
-static void launch_col_n(const ColArgs<T>& a, hipStream_t st) {
-  constexpr int BZ = ColW<T>::BZ;
-  constexpr int NT = threads_for(N * BZ);
+static void launch_col_n(ColArgs<T> a, hipStream_t st) {
+  constexpr int BZ = ColW<T, N>::BZ;
+  constexpr int NT = threads_for<T>(N * BZ);
+  a.ntz = (a.ncols + BZ - 1) / BZ;
+  a.nblk = (int64_t)a.ntz * a.nouter;
+  if (a.nblk <= 0) return;
   hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP>), dim3((unsigned)a.nblk), dim3(NT), 0,
                      st, a);
 }
 
 template <typename T, bool INV, bool KOP>
 static int launch_col(int n, ColArgs<T> a, hipStream_t st) {
-  constexpr int BZ = ColW<T>::BZ;
-  a.ntz = (a.ncols + BZ - 1) / BZ;
-  a.nblk = (int64_t)a.ntz * a.nouter;
-  if (a.nblk <= 0) return FPM_OK;
   switch (n) {
 #define FPM_CASE(N_) \
   case N_: launch_col_n<T, N_, INV, KOP>(a, st); break;
@@ -536,7 +566,7 @@ static int launch_col(int n, ColArgs<T> a, hipStream_t st) {
 template <typename T, int M, bool INV>
 static void launch_z_m(const ZArgs<T>& a, hipStream_t st) {
   constexpr int NL = z_lines(M);
-  constexpr int NT = threads_for(NL * M);
+  constexpr int NT = clampi(round64(NL * M / 8), 64, 1024);
   const unsigned nblk = (unsigned)((a.nrows + NL - 1) / NL);
   if (INV)
     hipLaunchKernelGGL((k_fft_z_c2r<T, M, NL, NT>), dim3(nblk), dim3(NT), 0, st, a);
@@ -664,6 +694,8 @@ static int c2r_impl(const void* in, int ncomp, const fpm_kop* ops, void* const* 
   x.ncols = nh;
   x.scale = (T)1;
   x.tw = static_cast<const C<T>*>(tw0);
+  x.eaxis = 0;
+  x.oaxis = 1;
   if ((rc = fill_kop(x.kop, ncomp, ops, m))) return rc;
   if ((rc = launch_col<T, true, true>(n0, x, st))) return rc;
   FPM_LAUNCH_CHECK();
@@ -687,6 +719,96 @@ static int c2r_impl(const void* in, int ncomp, const fpm_kop* ops, void* const* 
     z.nrows = (int64_t)n0 * n1;
     z.pitch2 = pitch2;
     z.tw = static_cast<const C<T>*>(tw2);
+    if ((rc = launch_z<T, true>(n2, z, st))) return rc;
+    FPM_LAUNCH_CHECK();
+  }
+  return FPM_OK;
+}
+
+// outs[d] = c2r(i a_d(k_d) * common(k) * in), d = 0, 1, 2, using that a_d
+// depends on k_d alone: the x pass reads `in` once and writes the common
+// product before and after the x gradient (A, B); the y pass turns A into
+// A and C = (y gradient of A); the z pass applies the z gradient to A on the
+// fly.  Eight mesh passes instead of ten.
+template <typename T>
+static int c2r_grad3_impl(const void* in, const fpm_kop* common, void* const* outs,
+                          void* const* work, const fpm_mesh* m, int64_t pitch2, const void* tw0,
+                          const void* tw1, const void* tw2, hipStream_t st) {
+  int rc = check_mesh(m, pitch2);
+  if (rc) return rc;
+  if (!in || !common || !outs || !work || !tw0 || !tw1 || !tw2) return FPM_ERR_ARG;
+  if (common->ngrad != 0) return FPM_ERR_ARG;
+  for (int c = 0; c < 3; ++c) {
+    if (!outs[c] || !work[c] || work[c] == in) return FPM_ERR_ARG;
+    for (int d = 0; d < c; ++d)
+      if (work[c] == work[d]) return FPM_ERR_ARG;
+  }
+  const int n0 = m->nmesh[0], n1 = m->nmesh[1], n2 = m->nmesh[2];
+  const int nh = n2 / 2 + 1;
+  C<T>* A = static_cast<C<T>*>(work[0]);
+  C<T>* B = static_cast<C<T>*>(work[1]);
+  C<T>* Cw = static_cast<C<T>*>(work[2]);
+  fpm_kop xo[2] = {*common, *common};
+  xo[1].ngrad = 1;
+  xo[1].grad_dir[0] = 0;
+  ColArgs<T> x{};
+  x.in = static_cast<const C<T>*>(in);
+  x.out[0] = A;
+  x.out[1] = B;
+  x.ncomp = 2;
+  x.estride = (int64_t)n1 * pitch2;
+  x.ostride = pitch2;
+  x.nouter = n1;
+  x.ncols = nh;
+  x.scale = (T)1;
+  x.tw = static_cast<const C<T>*>(tw0);
+  x.eaxis = 0;
+  x.oaxis = 1;
+  if ((rc = fill_kop(x.kop, 2, xo, m))) return rc;
+  if ((rc = launch_col<T, true, true>(n0, x, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  // y pass on A: identity (in place) and the y gradient (into C)
+  fpm_kop yo[2] = {};
+  yo[0].factor = 1.0;
+  yo[1].factor = 1.0;
+  yo[1].ngrad = 1;
+  yo[1].grad_dir[0] = 1;
+  yo[1].grad_order = common->grad_order;
+  ColArgs<T> y{};
+  y.in = A;
+  y.out[0] = A;
+  y.out[1] = Cw;
+  y.ncomp = 2;
+  y.estride = pitch2;
+  y.ostride = (int64_t)n1 * pitch2;
+  y.nouter = n0;
+  y.ncols = nh;
+  y.scale = (T)1;
+  y.tw = static_cast<const C<T>*>(tw1);
+  y.eaxis = 1;
+  y.oaxis = 0;
+  if ((rc = fill_kop(y.kop, 2, yo, m))) return rc;
+  if ((rc = launch_col<T, true, true>(n1, y, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  // y pass on B, in place
+  ColArgs<T> yb = y;
+  yb.in = B;
+  yb.out[0] = B;
+  yb.ncomp = 1;
+  if ((rc = launch_col<T, true, false>(n1, yb, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  // z passes: B -> F_x, C -> F_y, A with the z gradient -> F_z
+  const C<T>* src[3] = {B, Cw, A};
+  for (int d = 0; d < 3; ++d) {
+    ZArgs<T> z{};
+    z.cin = src[d];
+    z.rout = static_cast<T*>(outs[d]);
+    z.nrows = (int64_t)n0 * n1;
+    z.pitch2 = pitch2;
+    z.tw = static_cast<const C<T>*>(tw2);
+    z.zgrad = d == 2 ? 1 : 0;
+    z.zorder = common->grad_order;
+    z.L2 = m->box[2];
     if ((rc = launch_z<T, true>(n2, z, st))) return rc;
     FPM_LAUNCH_CHECK();
   }
@@ -771,6 +893,17 @@ int fpm_fft_c2r_f64(const void* in, int ncomp, const fpm_kop* ops, void* const* 
                     void* const* work, const fpm_mesh* mesh, int64_t pitch2, const void* tw0,
                     const void* tw1, const void* tw2, fpm_stream_t s) {
   return c2r_impl<double>(in, ncomp, ops, outs, work, mesh, pitch2, tw0, tw1, tw2, (hipStream_t)s);
+}
+int fpm_fft_c2r_grad3_f32(const void* in, const fpm_kop* common, void* const* outs,
+                          void* const* work, const fpm_mesh* mesh, int64_t pitch2, const void* tw0,
+                          const void* tw1, const void* tw2, fpm_stream_t s) {
+  return c2r_grad3_impl<float>(in, common, outs, work, mesh, pitch2, tw0, tw1, tw2, (hipStream_t)s);
+}
+int fpm_fft_c2r_grad3_f64(const void* in, const fpm_kop* common, void* const* outs,
+                          void* const* work, const fpm_mesh* mesh, int64_t pitch2, const void* tw0,
+                          const void* tw1, const void* tw2, fpm_stream_t s) {
+  return c2r_grad3_impl<double>(in, common, outs, work, mesh, pitch2, tw0, tw1, tw2,
+                                (hipStream_t)s);
 }
 int fpm_kspace_apply_f32(void* f, const fpm_kop* op, const fpm_mesh* m, int64_t p, fpm_stream_t s) {
   return kspace_apply_impl<float>(f, op, m, p, (hipStream_t)s);

@@ -28,7 +28,7 @@ template <typename T>
 int make_desc(const fpm_particles* part, const fpm_mesh* m, PartDesc<T>& P);
 
 constexpr int TILE = 16;
-constexpr int kTileThreads = 256;
+constexpr int kTileThreads = 512;
 
 struct TileGeom {
   int nm[3];
@@ -141,16 +141,32 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
     p[i] = 0;
 }
 
+// Lanes of a wave that fall in the same tile are merged before touching the
+// counters (consecutive lattice particles mostly share tiles), so each
+// distinct tile in a wave costs one atomic.
 template <typename T, int ORDER>
 __global__ void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w) {
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < P.n;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    int i0[3];
-    T wt[3][ORDER];
-    window_of<T, ORDER>(P, p, g.nm, i0, wt);
-    const int32_t t = (int32_t)tile_of(i0, g);
-    w.key[p] = t;
-    atomicAdd(&w.count[t], 1);
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < P.n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = base + threadIdx.x;
+    const bool valid = p < P.n;
+    int32_t t = -1;
+    if (valid) {
+      int i0[3];
+      T wt[3][ORDER];
+      window_of<T, ORDER>(P, p, g.nm, i0, wt);
+      t = (int32_t)tile_of(i0, g);
+      w.key[p] = t;
+    }
+    unsigned long long active = __ballot(valid);
+    while (active) {
+      const int leader = __ffsll(active) - 1;
+      const int32_t tl = __shfl(t, leader);
+      const unsigned long long same = __ballot(valid && t == tl);
+      if (lane == leader) atomicAdd(&w.count[tl], (int32_t)__popcll(same));
+      active &= ~same;
+    }
   }
 }
 
@@ -182,10 +198,24 @@ __global__ __launch_bounds__(1024) void k_bin_scan(Workspace w, int64_t ntiles) 
 }
 
 __global__ void k_bin_scatter(int64_t n, Workspace w) {
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t slot = atomicAdd(&w.cursor[w.key[p]], 1);
-    w.list[slot] = (int32_t)p;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = base + threadIdx.x;
+    const bool valid = p < n;
+    const int32_t t = valid ? w.key[p] : -1;
+    unsigned long long active = __ballot(valid);
+    while (active) {
+      const int leader = __ffsll(active) - 1;
+      const int32_t tl = __shfl(t, leader);
+      const unsigned long long same = __ballot(valid && t == tl);
+      int32_t slot0 = 0;
+      if (lane == leader) slot0 = atomicAdd(&w.cursor[tl], (int32_t)__popcll(same));
+      slot0 = __shfl(slot0, leader);
+      if (valid && t == tl) w.list[slot0 + __popcll(same & below)] = (int32_t)p;
+      active &= ~same;
+    }
   }
 }
 
@@ -306,15 +336,33 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
   const int lo[3] = {tx * TILE, ty * TILE, tz * TILE};
   const int32_t b = w.offs[t], e = w.offs[t + 1];
   if (b == e) return;  // uniform across the workgroup
-  for (int i = threadIdx.x; i < NB; i += kTileThreads) {
-    const int z = i % TB, y = (i / TB) % TB, x = i / (TB * TB);
-    int gx = lo[0] + x, gy = lo[1] + y, gz = lo[2] + z;
+  // Stage the box: each (component, x, y) row is TILE aligned cells loaded as
+  // TILE/4 four-wide vectors plus H wrapped halo cells.
+  constexpr int NQ = TILE / 4 + 1;
+  for (int i = threadIdx.x; i < NC * TB * TB * NQ; i += kTileThreads) {
+    const int q = i % NQ;
+    const int r = i / NQ;
+    const int y = r % TB, x = (r / TB) % TB, c = r / (TB * TB);
+    int gx = lo[0] + x, gy = lo[1] + y;
     if (gx >= g.nm[0]) gx -= g.nm[0];
     if (gy >= g.nm[1]) gy -= g.nm[1];
-    if (gz >= g.nm[2]) gz -= g.nm[2];
-    const int64_t off = ((int64_t)gx * g.nm[1] + gy) * g.nm[2] + gz;
+    const T* row = R.mesh[c] + ((int64_t)gx * g.nm[1] + gy) * g.nm[2];
+    T* dst = &box[c][(x * TB + y) * TB];
+    if (q < TILE / 4) {
+      typedef T V4 __attribute__((ext_vector_type(4)));
+      const V4 v = *reinterpret_cast<const V4*>(row + lo[2] + 4 * q);
+      dst[4 * q + 0] = v.x;
+      dst[4 * q + 1] = v.y;
+      dst[4 * q + 2] = v.z;
+      dst[4 * q + 3] = v.w;
+    } else {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) box[c][i] = R.mesh[c][off];
+      for (int h = 0; h < H; ++h) {
+        int gz = lo[2] + TILE + h;
+        if (gz >= g.nm[2]) gz -= g.nm[2];
+        dst[TILE + h] = row[gz];
+      }
+    }
   }
   __syncthreads();
   for (int32_t s = b + threadIdx.x; s < e; s += kTileThreads) {

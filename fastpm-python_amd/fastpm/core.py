@@ -13,7 +13,7 @@ import torch
 
 from . import _lib
 from .background import MatterDominated
-from .pm import ParticleMesh, Particles, RealField, ComplexField, c2r_many, _stream
+from .pm import ParticleMesh, Particles, RealField, ComplexField, c2r_many, c2r_grad3, _stream
 from .force.kernels import laplace, gradient, compensation
 
 
@@ -260,6 +260,14 @@ class FastPMStep:
                 self.pm.paint(X1, mass=1.0 / nbar, out=rho)
         return layout, X1, rho
 
+    def force_common(self):
+        """The factor shared by the three force components:
+        3/2 Om0 / k^2 [/ W(k)^p]."""
+        op = laplace * (1.5 * self.cosmology.Om0)
+        if self.solver.compensate:
+            op = op * compensation(self.pm.order)
+        return op
+
     def force_ops(self):
         factor = 1.5 * self.cosmology.Om0
         ops = []
@@ -283,7 +291,7 @@ class FastPMStep:
         ops = self.force_ops()
         if self.solver.fused:
             with self._mark("c2r"):
-                c2r_many(delta_k, ops, outs=ws["force"], work=ws["work"])
+                c2r_grad3(delta_k, self.force_common(), outs=ws["force"], work=ws["work"])
             with self._mark("readout"):
                 readout = self.pm.readout_tiled if self.tiled else self.pm.readout_many
                 readout([rho] + ws["force"], X1, [state.RHO, F[:, 0], F[:, 1], F[:, 2]])

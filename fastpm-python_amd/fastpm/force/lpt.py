@@ -2,7 +2,7 @@
 fastpm/force/lpt.py:4-64)."""
 import torch
 
-from ..pm import c2r_many, RealField, _stream
+from ..pm import c2r_many, c2r_grad3, RealField, _stream
 from .. import _lib
 from .kernels import laplace, gradient
 
@@ -13,8 +13,7 @@ def lpt1(dlin_k, q, resampler="cic"):
     the three readouts are one fused gather.  Returns a [N, 3] device tensor in
     the mesh dtype."""
     pm = dlin_k.pm
-    ops = [laplace * gradient(d, order=1) for d in range(3)]
-    meshes = c2r_many(dlin_k, ops)
+    meshes = c2r_grad3(dlin_k, laplace, grad_order=1)
     parts = pm._particles(q)
     out = torch.empty((len(parts), 3), dtype=pm.tdtype, device=pm.device)
     pm.readout_many(meshes, parts, [out[:, 0], out[:, 1], out[:, 2]], resampler)

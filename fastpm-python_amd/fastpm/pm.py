@@ -420,6 +420,27 @@ class ComplexField:
                         else [out])[0]
 
 
+def c2r_grad3(cfield, common, outs=None, work=None, grad_order=1):
+    """outs[d] = c2r(i a_d(k) * common(k) * cfield) for d = 0, 1, 2 in one
+    call (the three force or displacement components); a_d is the exact
+    (grad_order 0) or 4-point (1) gradient.  The input is left intact."""
+    pm = cfield.pm
+    if outs is None:
+        outs = [RealField(pm) for _ in range(3)]
+    if work is None:
+        work = [pm._complex_tensor() for _ in range(3)]
+    k = common.fpm_kop() if not isinstance(common, _lib.FpmKop) else common
+    if k.ngrad != 0:
+        raise ValueError("the common factor of c2r_grad3 carries no gradient")
+    k.grad_order = int(grad_order)
+    tw = pm.twiddles()
+    _lib.call("fpm_fft_c2r_grad3", pm.dtype, cfield._t.data_ptr(), k,
+              _lib.ptr_array([o.value.data_ptr() for o in outs]),
+              _lib.ptr_array([w.data_ptr() for w in work]), pm._desc, pm.pitch2,
+              tw[0].data_ptr(), tw[1].data_ptr(), tw[2].data_ptr(), _stream())
+    return outs
+
+
 def c2r_many(cfield, ops, outs=None, work=None):
     """outs[c] = c2r(ops[c] * cfield) for up to six operators; the input is read
     once and left intact."""

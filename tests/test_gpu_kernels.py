@@ -299,9 +299,9 @@ def test_tiled_paint_readout(gpu, order, nmesh, kind):
     b = pm.paint_tiled(parts, mass=0.75).value.clone()
     assert torch.equal(a, b)
     ref = O.paint(X, nmesh, box, order, 0.75)
-    assert _rel(a.cpu().numpy(), ref) < 2e-6
+    assert _rel(a.cpu().numpy(), ref) < 5e-6
     plain = pm.paint(parts, mass=0.75).value
-    assert _rel(a.cpu().numpy(), plain.cpu().numpy().astype("f8")) < 2e-6
+    assert _rel(a.cpu().numpy(), plain.cpu().numpy().astype("f8")) < 5e-6
     meshes = [rng.standard_normal(nmesh) for _ in range(4)]
     fields = [pm.create("real", value=torch.from_numpy(m.astype("f4")).to(gpu)) for m in meshes]
     F = torch.empty((len(parts), 3), dtype=torch.float32, device=gpu)
@@ -336,3 +336,25 @@ def test_tiled_solver_matches_plain(gpu):
         out[fused] = (st.X.cpu().numpy(), st.F.cpu().numpy(), st.RHO.cpu().numpy())
     for a, b in zip(out[True], out[False]):
         assert _rel(a, b) < 1e-4
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+@pytest.mark.parametrize("nmesh", [(8, 8, 8), (16, 32, 64), (96, 48, 32), (64, 64, 64), (8, 512, 4)])
+@pytest.mark.parametrize("order", [0, 1])
+def test_c2r_grad3(gpu, dtype, nmesh, order):
+    from fastpm.pm import c2r_grad3
+    from fastpm.force.kernels import laplace, compensation
+
+    box = (100.0, 80.0, 60.0)
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal(nmesh)
+    xk = O.r2c(x)
+    pm = _pm(nmesh, box, dtype)
+    ck = pm.create("real", value=torch.from_numpy(x.astype(dtype)).to(gpu)).r2c()
+    outs = c2r_grad3(ck, laplace * 0.45 * compensation(2), grad_order=order)
+    common = xk * O.k_laplace(nmesh, box) * 0.45 * O.k_deconv(nmesh, box, 2)
+    for d in range(3):
+        ref = O.c2r(common * O.k_gradient(d, nmesh, box, order), nmesh)
+        assert _rel(outs[d].value.cpu().numpy(), ref) < _tol(dtype) * 8, d
+    # input untouched
+    assert _rel(ck.value.cpu().numpy(), xk) < _tol(dtype) * 4

@@ -1,23 +1,24 @@
 // Tile-binned, deterministic paint and LDS-staged readout (float32 storage).
 //
-// The force mesh is cut into TILE^3-cell tiles.  Each particle belongs to the
-// tile holding the first point of its window (the CIC floor cell, or the TSC
-// nearest point minus one), so its deposit lies in that tile's box of
-// (TILE + order - 1)^3 cells.
+// The force mesh is cut into TILE^3-cell tiles.  A particle's window starts at
+// i0 (the CIC floor cell, or the TSC nearest point minus one) and covers
+// i0 .. i0 + order - 1 per axis.  It belongs to the tile holding i0, in one of
+// eight sub-bins given by a 3-bit mask: bit d is set when its window crosses
+// the tile's upper face along axis d (i0 mod TILE > TILE - order).
 //
-//   bin      : count particles per tile (integer atomics), exclusive scan,
-//              scatter particle indices into per-tile lists.
-//   paint    : one workgroup per tile accumulates its particles' deposits in
-//              LDS as 64-bit fixed point (integer adds, so the sum does not
-//              depend on the order particles arrive in), writes the tile's own
-//              cells once (no clearing pass, no global atomics) and the
-//              overflow cells (the box beyond the tile) to a per-tile halo
-//              buffer; a second kernel adds the neighbours' halos to each
-//              tile's low faces in a fixed neighbour order.  The result is
-//              bitwise reproducible from run to run.
-//   readout  : one workgroup per tile stages the box of every requested mesh
-//              in LDS (each mesh cell read once, plus the (1/TILE) halo) and
-//              interpolates for the tile's particles.
+//   bin      : count particles per (tile, mask) with integer atomics merged
+//              per wave, exclusive scan, scatter particle indices into the
+//              per-sub-bin lists.
+//   paint    : one workgroup per tile gathers every deposit that lands in its
+//              cells: its own particles, plus the particles of the seven lower
+//              neighbour tiles whose mask says they cross into it.  Deposits
+//              are summed in LDS as 64-bit fixed point (integer adds, so the
+//              sum does not depend on the order particles arrive in) and each
+//              cell is written exactly once: no clearing pass, no global
+//              atomics, no halo pass, and bitwise reproducible results.
+//   readout  : one workgroup per tile stages the (TILE + order - 1)^3 box of
+//              every requested mesh in LDS (each cell read once plus the
+//              1/TILE overhang) and interpolates for the tile's particles.
 #include "fpm_common.h"
 
 #include <cmath>
@@ -29,17 +30,16 @@ int make_desc(const fpm_particles* part, const fpm_mesh* m, PartDesc<T>& P);
 
 constexpr int TILE = 16;
 constexpr int kTileThreads = 512;
+constexpr int kSub = 8;  // sub-bins per tile
 
 struct TileGeom {
   int nm[3];
   int nt[3];       // tiles per axis
   int64_t ntiles;
-  int H;           // order - 1: box overhang
-  int TB;          // TILE + H
-  int64_t halo;    // overflow cells per tile
+  int64_t nbins;   // ntiles * kSub
 };
 
-static TileGeom tile_geom(const fpm_mesh* m, int order) {
+static TileGeom tile_geom(const fpm_mesh* m) {
   TileGeom g{};
   g.ntiles = 1;
   for (int d = 0; d < 3; ++d) {
@@ -47,49 +47,37 @@ static TileGeom tile_geom(const fpm_mesh* m, int order) {
     g.nt[d] = m->nmesh[d] / TILE;
     g.ntiles *= g.nt[d];
   }
-  g.H = order - 1;
-  g.TB = TILE + g.H;
-  g.halo = (int64_t)g.H * g.TB * g.TB + (int64_t)TILE * g.H * g.TB + (int64_t)TILE * TILE * g.H;
+  g.nbins = g.ntiles * kSub;
   return g;
-}
-
-// Index of an overflow cell (at least one local coordinate >= TILE) in the
-// tile's halo record: x-slab, then y-slab, then z-slab.
-__device__ __forceinline__ int halo_index(int x, int y, int z, int H, int TB) {
-  if (x >= TILE) return ((x - TILE) * TB + y) * TB + z;
-  if (y >= TILE) return H * TB * TB + (x * H + (y - TILE)) * TB + z;
-  return H * TB * TB + TILE * H * TB + (x * TILE + y) * H + (z - TILE);
 }
 
 struct Workspace {
   int32_t* key;
   int32_t* list;
   int32_t* count;
-  int32_t* offs;
+  int32_t* offs;       // nbins + 1
   int32_t* cursor;
-  int32_t* misplaced;  // count of particles found outside their binned tile
-  void* halo;
+  int32_t* misplaced;  // particles found outside their binned tile
   int64_t bytes;
 };
 
 static int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
 // Byte offsets of the workspace sections: key, list, count, offs, cursor,
-// misplaced, halo; off[7] is the total size.
-static void ws_offsets(const TileGeom& g, int64_t n, size_t real, int64_t off[8]) {
-  const int64_t sizes[7] = {n * 4, n * 4, g.ntiles * 4, (g.ntiles + 1) * 4, g.ntiles * 4, 4,
-                            g.ntiles * g.halo * (int64_t)real};
+// misplaced; off[6] is the total size.
+static void ws_offsets(const TileGeom& g, int64_t n, int64_t off[7]) {
+  const int64_t sizes[6] = {n * 4, n * 4, g.nbins * 4, (g.nbins + 1) * 4, g.nbins * 4, 4};
   int64_t o = 0;
-  for (int i = 0; i < 7; ++i) {
+  for (int i = 0; i < 6; ++i) {
     off[i] = o;
     o += align256(sizes[i]);
   }
-  off[7] = o;
+  off[6] = o;
 }
 
-static int64_t layout_ws(const TileGeom& g, int64_t n, size_t real, char* base, Workspace* w) {
-  int64_t off[8];
-  ws_offsets(g, n, real, off);
+static int64_t layout_ws(const TileGeom& g, int64_t n, char* base, Workspace* w) {
+  int64_t off[7];
+  ws_offsets(g, n, off);
   if (w) {
     w->key = reinterpret_cast<int32_t*>(base + off[0]);
     w->list = reinterpret_cast<int32_t*>(base + off[1]);
@@ -97,10 +85,9 @@ static int64_t layout_ws(const TileGeom& g, int64_t n, size_t real, char* base, 
     w->offs = reinterpret_cast<int32_t*>(base + off[3]);
     w->cursor = reinterpret_cast<int32_t*>(base + off[4]);
     w->misplaced = reinterpret_cast<int32_t*>(base + off[5]);
-    w->halo = base + off[6];
-    w->bytes = off[7];
+    w->bytes = off[6];
   }
-  return off[7];
+  return off[6];
 }
 
 // Window origin (wrapped) and weights, identical to the plain paint/readout.
@@ -130,8 +117,20 @@ __device__ __forceinline__ void window_of(const PartDesc<T>& P, int64_t p, const
   }
 }
 
-__device__ __forceinline__ int64_t tile_of(const int i0[3], const TileGeom& g) {
-  return ((int64_t)(i0[0] / TILE) * g.nt[1] + i0[1] / TILE) * g.nt[2] + i0[2] / TILE;
+template <int ORDER>
+__device__ __forceinline__ int32_t bin_of(const int i0[3], const TileGeom& g) {
+  const int64_t t = ((int64_t)(i0[0] / TILE) * g.nt[1] + i0[1] / TILE) * g.nt[2] + i0[2] / TILE;
+  int mask = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (i0[d] % TILE > TILE - ORDER) mask |= 1 << d;
+  return (int32_t)(t * kSub + mask);
+}
+
+__device__ __forceinline__ void tile_coords(int64_t t, const TileGeom& g, int lo[3]) {
+  lo[2] = (int)(t % g.nt[2]) * TILE;
+  lo[1] = (int)((t / g.nt[2]) % g.nt[1]) * TILE;
+  lo[0] = (int)(t / ((int64_t)g.nt[2] * g.nt[1])) * TILE;
 }
 
 // ---------------------------------------------------------------- binning
@@ -141,9 +140,9 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
     p[i] = 0;
 }
 
-// Lanes of a wave that fall in the same tile are merged before touching the
-// counters (consecutive lattice particles mostly share tiles), so each
-// distinct tile in a wave costs one atomic.
+// Lanes of a wave that fall in the same sub-bin are merged before touching
+// the counters (consecutive lattice particles mostly share one), so each
+// distinct sub-bin in a wave costs one atomic.
 template <typename T, int ORDER>
 __global__ void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w) {
   const int lane = threadIdx.x & 63;
@@ -156,7 +155,7 @@ __global__ void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w) {
       int i0[3];
       T wt[3][ORDER];
       window_of<T, ORDER>(P, p, g.nm, i0, wt);
-      t = (int32_t)tile_of(i0, g);
+      t = bin_of<ORDER>(i0, g);
       w.key[p] = t;
     }
     unsigned long long active = __ballot(valid);
@@ -170,33 +169,55 @@ __global__ void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w) {
   }
 }
 
-// Exclusive scan of the tile counts in one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_bin_scan(Workspace w, int64_t ntiles) {
-  __shared__ int32_t part[1024];
+// Exclusive scan of the sub-bin counts in one workgroup, kScanChunk counts at
+// a time staged in LDS with coalesced loads and stores.
+constexpr int kScanThreads = 1024;
+constexpr int kScanChunk = 16384;
+
+__global__ __launch_bounds__(kScanThreads) void k_bin_scan(Workspace w, int64_t nbins) {
+  __shared__ int32_t buf[kScanChunk];
+  __shared__ int32_t part[kScanThreads];
+  constexpr int PER = kScanChunk / kScanThreads;
   const int tid = threadIdx.x;
-  const int64_t per = (ntiles + 1023) / 1024;
-  const int64_t b = tid * per;
-  const int64_t e = b + per < ntiles ? b + per : ntiles;
-  int32_t s = 0;
-  for (int64_t i = b; i < e; ++i) s += w.count[i];
-  part[tid] = s;
-  __syncthreads();
-  // Hillis-Steele inclusive scan over the 1024 partial sums
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int32_t v = tid >= o ? part[tid - o] : 0;
+  int32_t carry = 0;
+  for (int64_t c0 = 0; c0 < nbins; c0 += kScanChunk) {
+    for (int i = tid; i < kScanChunk; i += kScanThreads)
+      buf[i] = c0 + i < nbins ? w.count[c0 + i] : 0;
     __syncthreads();
-    part[tid] += v;
+    int32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) s += buf[tid * PER + i];
+    part[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < kScanThreads; o <<= 1) {
+      const int32_t v = tid >= o ? part[tid - o] : 0;
+      __syncthreads();
+      part[tid] += v;
+      __syncthreads();
+    }
+    int32_t run = carry + (tid ? part[tid - 1] : 0);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int32_t c = buf[tid * PER + i];
+      buf[tid * PER + i] = run;
+      run += c;
+    }
+    const int32_t total = part[kScanThreads - 1];
+    __syncthreads();
+    for (int i = tid; i < kScanChunk; i += kScanThreads)
+      if (c0 + i < nbins) {
+        w.offs[c0 + i] = buf[i];
+        w.cursor[c0 + i] = buf[i];
+      }
+    carry += total;
     __syncthreads();
   }
-  int32_t run = tid ? part[tid - 1] : 0;
-  for (int64_t i = b; i < e; ++i) {
-    w.offs[i] = run;
-    w.cursor[i] = run;
-    run += w.count[i];
-  }
-  if (tid == 1023) w.offs[ntiles] = part[1023];
+  if (tid == 0) w.offs[nbins] = carry;
 }
 
+// Each lane is in exactly one same-sub-bin group, so only a group's leader
+// issues an atomic, and the returned slot base is consumed after the merge
+// loop: the atomics of all groups of a wave are in flight together.
 __global__ void k_bin_scatter(int64_t n, Workspace w) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -206,110 +227,126 @@ __global__ void k_bin_scatter(int64_t n, Workspace w) {
     const bool valid = p < n;
     const int32_t t = valid ? w.key[p] : -1;
     unsigned long long active = __ballot(valid);
+    int myleader = 0, myrank = 0;
+    int32_t slot0 = 0;
     while (active) {
       const int leader = __ffsll(active) - 1;
       const int32_t tl = __shfl(t, leader);
       const unsigned long long same = __ballot(valid && t == tl);
-      int32_t slot0 = 0;
       if (lane == leader) slot0 = atomicAdd(&w.cursor[tl], (int32_t)__popcll(same));
-      slot0 = __shfl(slot0, leader);
-      if (valid && t == tl) w.list[slot0 + __popcll(same & below)] = (int32_t)p;
+      if (valid && t == tl) {
+        myleader = leader;
+        myrank = __popcll(same & below);
+      }
       active &= ~same;
     }
+    const int32_t b0 = __shfl(slot0, myleader);
+    if (valid) w.list[b0 + myrank] = (int32_t)p;
   }
 }
 
 // ---------------------------------------------------------------- paint
+// Deposits of the particle at list slot s that land in the target tile at lo.
+// Every window point is wrapped and tested against the tile, so a deposit
+// never leaves the tile's LDS accumulator.
+template <typename T, int ORDER>
+__device__ __forceinline__ void deposit_one(const PartDesc<T>& P, const TileGeom& g,
+                                            const Workspace& w, int32_t s, const int lo[3],
+                                            unsigned long long* acc, double mass_fx) {
+  const int64_t p = w.list[s];
+  int i0[3];
+  T wt[3][ORDER];
+  window_of<T, ORDER>(P, p, g.nm, i0, wt);
+  int lx[ORDER], ly[ORDER], lz[ORDER];
+#pragma unroll
+  for (int a = 0; a < ORDER; ++a) {
+    int cx = i0[0] + a, cy = i0[1] + a, cz = i0[2] + a;
+    if (cx >= g.nm[0]) cx -= g.nm[0];
+    if (cy >= g.nm[1]) cy -= g.nm[1];
+    if (cz >= g.nm[2]) cz -= g.nm[2];
+    lx[a] = cx - lo[0];
+    ly[a] = cy - lo[1];
+    lz[a] = cz - lo[2];
+  }
+#pragma unroll
+  for (int a = 0; a < ORDER; ++a) {
+    if ((unsigned)lx[a] >= (unsigned)TILE) continue;
+#pragma unroll
+    for (int c = 0; c < ORDER; ++c) {
+      if ((unsigned)ly[c] >= (unsigned)TILE) continue;
+      const T wac = wt[0][a] * wt[1][c];
+#pragma unroll
+      for (int d = 0; d < ORDER; ++d) {
+        if ((unsigned)lz[d] >= (unsigned)TILE) continue;
+        const double v = (double)(wac * wt[2][d]) * mass_fx;
+        const long long q = __double2ll_rn(v);
+        atomicAdd(&acc[(lx[a] * TILE + ly[c]) * TILE + lz[d]], (unsigned long long)q);
+      }
+    }
+  }
+}
+
+// The up to 64 (source tile, sub-bin) ranges feeding a tile are gathered
+// first (one lane each) and scanned, then all their particles are processed
+// as one flat index space, so their loads are issued together.
 template <typename T, int ORDER>
 __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, TileGeom g,
                                                                Workspace w, T* mesh, double mass_fx,
                                                                double inv_fx) {
-  constexpr int H = ORDER - 1;
-  constexpr int TB = TILE + H;
-  constexpr int NB = TB * TB * TB;
+  constexpr int NB = TILE * TILE * TILE;
   __shared__ unsigned long long acc[NB];
+  __shared__ int32_t rstart[64];
+  __shared__ int32_t rpre[65];
   const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
-  const int tz = (int)(t % g.nt[2]);
-  const int ty = (int)((t / g.nt[2]) % g.nt[1]);
-  const int tx = (int)(t / ((int64_t)g.nt[2] * g.nt[1]));
-  const int lo[3] = {tx * TILE, ty * TILE, tz * TILE};
+  int lo[3];
+  tile_coords(t, g, lo);
   for (int i = threadIdx.x; i < NB; i += kTileThreads) acc[i] = 0ull;
-  __syncthreads();
-  const int32_t b = w.offs[t], e = w.offs[t + 1];
-  for (int32_t s = b + threadIdx.x; s < e; s += kTileThreads) {
-    const int64_t p = w.list[s];
-    int i0[3];
-    T wt[3][ORDER];
-    window_of<T, ORDER>(P, p, g.nm, i0, wt);
-    int lx = i0[0] - lo[0], ly = i0[1] - lo[1], lz = i0[2] - lo[2];
-    if ((unsigned)lx >= (unsigned)TILE || (unsigned)ly >= (unsigned)TILE ||
-        (unsigned)lz >= (unsigned)TILE) {
-      // the particle was placed in another tile when binned: keep the LDS
-      // access in bounds and report it (never expected: same arithmetic)
-      atomicAdd(w.misplaced, 1);
-      lx = lx < 0 ? 0 : (lx >= TILE ? TILE - 1 : lx);
-      ly = ly < 0 ? 0 : (ly >= TILE ? TILE - 1 : ly);
-      lz = lz < 0 ? 0 : (lz >= TILE ? TILE - 1 : lz);
+  if (threadIdx.x < 64) {
+    // lane = o * 8 + m: source tile t - o (o in {0,1}^3; along an axis with a
+    // single tile only o = 0, the neighbour being the tile itself), sub-bin m
+    // whose crossing mask must contain the direction o
+    const int lane = threadIdx.x;
+    const int o = lane >> 3, m = lane & 7;
+    const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
+    const int need = ox | (oy << 1) | (oz << 2);
+    int32_t b = 0, len = 0;
+    const bool ok = !((ox && g.nt[0] == 1) || (oy && g.nt[1] == 1) || (oz && g.nt[2] == 1)) &&
+                    (m & need) == need;
+    if (ok) {
+      const int tx = lo[0] / TILE, ty = lo[1] / TILE, tz = lo[2] / TILE;
+      const int sx = (tx - ox + g.nt[0]) % g.nt[0];
+      const int sy = (ty - oy + g.nt[1]) % g.nt[1];
+      const int sz = (tz - oz + g.nt[2]) % g.nt[2];
+      const int64_t src = ((int64_t)sx * g.nt[1] + sy) * g.nt[2] + sz;
+      b = w.offs[src * kSub + m];
+      len = w.offs[src * kSub + m + 1] - b;
     }
+    // inclusive scan of the lengths across the 64 lanes
+    int32_t inc = len;
 #pragma unroll
-    for (int a = 0; a < ORDER; ++a) {
-#pragma unroll
-      for (int c = 0; c < ORDER; ++c) {
-        const T wac = wt[0][a] * wt[1][c];
-#pragma unroll
-        for (int d = 0; d < ORDER; ++d) {
-          const double v = (double)(wac * wt[2][d]) * mass_fx;
-          const long long q = __double2ll_rn(v);
-          atomicAdd(&acc[((lx + a) * TB + (ly + c)) * TB + (lz + d)], (unsigned long long)q);
-        }
-      }
+    for (int d = 1; d < 64; d <<= 1) {
+      const int32_t v = __shfl_up(inc, d);
+      if (lane >= d) inc += v;
     }
+    rstart[lane] = b;
+    rpre[lane + 1] = inc;
+    if (lane == 0) rpre[0] = 0;
   }
   __syncthreads();
-  T* halo = static_cast<T*>(w.halo) + t * g.halo;
+  const int32_t total = rpre[64];
+  for (int32_t f = threadIdx.x; f < total; f += kTileThreads) {
+    // range r with rpre[r] <= f < rpre[r + 1]
+    int r = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+      if (rpre[r + step] <= f) r += step;
+    deposit_one<T, ORDER>(P, g, w, rstart[r] + (f - rpre[r]), lo, acc, mass_fx);
+  }
+  __syncthreads();
   for (int i = threadIdx.x; i < NB; i += kTileThreads) {
-    const int z = i % TB, y = (i / TB) % TB, x = i / (TB * TB);
-    const T v = (T)((double)(long long)acc[i] * inv_fx);
-    if (x < TILE && y < TILE && z < TILE)
-      mesh[((int64_t)(lo[0] + x) * g.nm[1] + (lo[1] + y)) * g.nm[2] + (lo[2] + z)] = v;
-    else
-      halo[halo_index(x, y, z, H, TB)] = v;
-  }
-}
-
-// Adds the overflow of the up-to-7 lower neighbours into each tile's cells.
-template <typename T, int ORDER>
-__global__ __launch_bounds__(kTileThreads) void k_paint_halo(TileGeom g, Workspace w, T* mesh) {
-  constexpr int H = ORDER - 1;
-  constexpr int TB = TILE + H;
-  const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
-  const int tz = (int)(t % g.nt[2]);
-  const int ty = (int)((t / g.nt[2]) % g.nt[1]);
-  const int tx = (int)(t / ((int64_t)g.nt[2] * g.nt[1]));
-  const int lo[3] = {tx * TILE, ty * TILE, tz * TILE};
-  const T* halo = static_cast<const T*>(w.halo);
-  // receiving cells: some local coordinate < H
-  for (int i = threadIdx.x; i < TILE * TILE * TILE; i += kTileThreads) {
     const int z = i % TILE, y = (i / TILE) % TILE, x = i / (TILE * TILE);
-    if (x >= H && y >= H && z >= H) continue;
-    T sum = (T)0;
-    bool any = false;
-    // neighbour offsets (ox, oy, oz) in {0, -1}^3 \ {0}, fixed order
-#pragma unroll
-    for (int o = 1; o < 8; ++o) {
-      const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
-      if ((ox && x >= H) || (oy && y >= H) || (oz && z >= H)) continue;
-      const int nx = (tx - ox + g.nt[0]) % g.nt[0];
-      const int ny = (ty - oy + g.nt[1]) % g.nt[1];
-      const int nz = (tz - oz + g.nt[2]) % g.nt[2];
-      const int64_t nt = ((int64_t)nx * g.nt[1] + ny) * g.nt[2] + nz;
-      sum += halo[nt * g.halo + halo_index(x + ox * TILE, y + oy * TILE, z + oz * TILE, H, TB)];
-      any = true;
-    }
-    if (any) {
-      T* cell = mesh + ((int64_t)(lo[0] + x) * g.nm[1] + (lo[1] + y)) * g.nm[2] + (lo[2] + z);
-      *cell = *cell + sum;
-    }
+    mesh[((int64_t)(lo[0] + x) * g.nm[1] + (lo[1] + y)) * g.nm[2] + (lo[2] + z)] =
+        (T)((double)(long long)acc[i] * inv_fx);
   }
 }
 
@@ -330,38 +367,67 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
   constexpr int NB = TB * TB * TB;
   __shared__ T box[NC][NB];
   const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
-  const int tz = (int)(t % g.nt[2]);
-  const int ty = (int)((t / g.nt[2]) % g.nt[1]);
-  const int tx = (int)(t / ((int64_t)g.nt[2] * g.nt[1]));
-  const int lo[3] = {tx * TILE, ty * TILE, tz * TILE};
-  const int32_t b = w.offs[t], e = w.offs[t + 1];
+  int lo[3];
+  tile_coords(t, g, lo);
+  const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
   if (b == e) return;  // uniform across the workgroup
-  // Stage the box: each (component, x, y) row is TILE aligned cells loaded as
-  // TILE/4 four-wide vectors plus H wrapped halo cells.
-  constexpr int NQ = TILE / 4 + 1;
-  for (int i = threadIdx.x; i < NC * TB * TB * NQ; i += kTileThreads) {
-    const int q = i % NQ;
-    const int r = i / NQ;
-    const int y = r % TB, x = (r / TB) % TB, c = r / (TB * TB);
-    int gx = lo[0] + x, gy = lo[1] + y;
-    if (gx >= g.nm[0]) gx -= g.nm[0];
-    if (gy >= g.nm[1]) gy -= g.nm[1];
-    const T* row = R.mesh[c] + ((int64_t)gx * g.nm[1] + gy) * g.nm[2];
-    T* dst = &box[c][(x * TB + y) * TB];
-    if (q < TILE / 4) {
-      typedef T V4 __attribute__((ext_vector_type(4)));
-      const V4 v = *reinterpret_cast<const V4*>(row + lo[2] + 4 * q);
-      dst[4 * q + 0] = v.x;
-      dst[4 * q + 1] = v.y;
-      dst[4 * q + 2] = v.z;
-      dst[4 * q + 3] = v.w;
-    } else {
+  // Stage the box: each (component, x, y) row holds TILE aligned cells loaded
+  // as TILE/4 four-wide vectors, plus H wrapped overhang cells.  All of a
+  // thread's loads are issued before any LDS store.
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  constexpr int NQ = TILE / 4;
+  constexpr int NV = NC * TB * TB * NQ;
+  constexpr int PV = (NV + kTileThreads - 1) / kTileThreads;
+  V4 vr[PV];
 #pragma unroll
-      for (int h = 0; h < H; ++h) {
-        int gz = lo[2] + TILE + h;
-        if (gz >= g.nm[2]) gz -= g.nm[2];
-        dst[TILE + h] = row[gz];
-      }
+  for (int k = 0; k < PV; ++k) {
+    const int i = threadIdx.x + k * kTileThreads;
+    if (i < NV) {
+      const int q = i % NQ, r = i / NQ;
+      const int y = r % TB, x = (r / TB) % TB, c = r / (TB * TB);
+      int gx = lo[0] + x, gy = lo[1] + y;
+      if (gx >= g.nm[0]) gx -= g.nm[0];
+      if (gy >= g.nm[1]) gy -= g.nm[1];
+      vr[k] = *reinterpret_cast<const V4*>(R.mesh[c] + ((int64_t)gx * g.nm[1] + gy) * g.nm[2] +
+                                           lo[2] + 4 * q);
+    }
+  }
+  constexpr int NH = NC * TB * TB * H;
+  constexpr int PH = (NH + kTileThreads - 1) / kTileThreads;
+  T hr[PH];
+#pragma unroll
+  for (int k = 0; k < PH; ++k) {
+    const int i = threadIdx.x + k * kTileThreads;
+    if (i < NH) {
+      const int h = i % H, r = i / H;
+      const int y = r % TB, x = (r / TB) % TB, c = r / (TB * TB);
+      int gx = lo[0] + x, gy = lo[1] + y, gz = lo[2] + TILE + h;
+      if (gx >= g.nm[0]) gx -= g.nm[0];
+      if (gy >= g.nm[1]) gy -= g.nm[1];
+      if (gz >= g.nm[2]) gz -= g.nm[2];
+      hr[k] = R.mesh[c][((int64_t)gx * g.nm[1] + gy) * g.nm[2] + gz];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PV; ++k) {
+    const int i = threadIdx.x + k * kTileThreads;
+    if (i < NV) {
+      const int q = i % NQ, r = i / NQ;
+      const int y = r % TB, x = (r / TB) % TB, c = r / (TB * TB);
+      T* dst = &box[c][(x * TB + y) * TB + 4 * q];
+      dst[0] = vr[k].x;
+      dst[1] = vr[k].y;
+      dst[2] = vr[k].z;
+      dst[3] = vr[k].w;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PH; ++k) {
+    const int i = threadIdx.x + k * kTileThreads;
+    if (i < NH) {
+      const int h = i % H, r = i / H;
+      const int y = r % TB, x = (r / TB) % TB, c = r / (TB * TB);
+      box[c][(x * TB + y) * TB + TILE + h] = hr[k];
     }
   }
   __syncthreads();
@@ -373,6 +439,8 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
     int lx = i0[0] - lo[0], ly = i0[1] - lo[1], lz = i0[2] - lo[2];
     if ((unsigned)lx >= (unsigned)TILE || (unsigned)ly >= (unsigned)TILE ||
         (unsigned)lz >= (unsigned)TILE) {
+      // placed in another tile when binned: keep the LDS access in bounds and
+      // report it (never expected: binning uses the same arithmetic)
       atomicAdd(w.misplaced, 1);
       lx = lx < 0 ? 0 : (lx >= TILE ? TILE - 1 : lx);
       ly = ly < 0 ? 0 : (ly >= TILE ? TILE - 1 : ly);
@@ -400,6 +468,49 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
   }
 }
 
+// Readout without staging: the tile's particles gather straight from the
+// meshes (through L1/L2), in tile order so that a wave's gathers share lines.
+template <typename T, int ORDER, int NC>
+__global__ __launch_bounds__(256) void k_readout_list(PartDesc<T> P, TileGeom g, Workspace w,
+                                                      TileRead<T> R) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < P.n;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = w.list[s];
+    int i0[3];
+    T wt[3][ORDER];
+    window_of<T, ORDER>(P, p, g.nm, i0, wt);
+    T acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = (T)0;
+#pragma unroll
+    for (int a = 0; a < ORDER; ++a) {
+      int ix = i0[0] + a;
+      if (ix >= g.nm[0]) ix -= g.nm[0];
+#pragma unroll
+      for (int c2 = 0; c2 < ORDER; ++c2) {
+        int iy = i0[1] + c2;
+        if (iy >= g.nm[1]) iy -= g.nm[1];
+        const T wac = wt[0][a] * wt[1][c2];
+        const int64_t row = ((int64_t)ix * g.nm[1] + iy) * g.nm[2];
+#pragma unroll
+        for (int d = 0; d < ORDER; ++d) {
+          int iz = i0[2] + d;
+          if (iz >= g.nm[2]) iz -= g.nm[2];
+          const T wv = wac * wt[2][d];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) acc[c] += wv * R.mesh[c][row + iz];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) R.out[c][p * R.stride[c]] = acc[c] * R.scale[c];
+  }
+}
+
+#ifndef FPM_READOUT_LDS
+#define FPM_READOUT_LDS 1
+#endif
+
 // ---------------------------------------------------------------- host side
 static bool tiles_ok(const fpm_mesh* m) {
   if (!m) return false;
@@ -422,8 +533,9 @@ static int prep(const fpm_particles* part, const fpm_mesh* m, int order, void* w
   if (rc) return rc;
   if (!tiles_ok(m) || (order != 2 && order != 3) || !ws) return FPM_ERR_ARG;
   if (P.n >= (int64_t)1 << 31) return FPM_ERR_ARG;
-  g = tile_geom(m, order);
-  const int64_t need = layout_ws(g, P.n, sizeof(T), static_cast<char*>(ws), &w);
+  g = tile_geom(m);
+  if (g.nbins >= (int64_t)1 << 31) return FPM_ERR_ARG;
+  const int64_t need = layout_ws(g, P.n, static_cast<char*>(ws), &w);
   if (ws_bytes < need) return FPM_ERR_WORKSPACE;
   return FPM_OK;
 }
@@ -436,7 +548,7 @@ static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, voi
   Workspace w;
   int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_zero_i32, dim3(grid_parts(g.ntiles)), dim3(256), 0, st, w.count, g.ntiles);
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_parts(g.nbins)), dim3(256), 0, st, w.count, g.nbins);
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, w.misplaced, (int64_t)1);
   if (P.n > 0) {
     if (order == 2)
@@ -444,7 +556,7 @@ static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, voi
     else
       hipLaunchKernelGGL((k_bin_count<T, 3>), dim3(grid_parts(P.n)), dim3(256), 0, st, P, g, w);
   }
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, w, g.ntiles);
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanThreads), 0, st, w, g.nbins);
   if (P.n > 0)
     hipLaunchKernelGGL(k_bin_scatter, dim3(grid_parts(P.n)), dim3(256), 0, st, P.n, w);
   FPM_LAUNCH_CHECK();
@@ -468,13 +580,10 @@ static int paint_tiles_impl(const fpm_particles* part, const fpm_mesh* m, int or
   const double inv_fx = std::ldexp(1.0, -S);
   T* mesh = static_cast<T*>(out);
   const dim3 grid((unsigned)g.ntiles), blk(kTileThreads);
-  if (order == 2) {
+  if (order == 2)
     hipLaunchKernelGGL((k_paint_tiles<T, 2>), grid, blk, 0, st, P, g, w, mesh, mass_fx, inv_fx);
-    hipLaunchKernelGGL((k_paint_halo<T, 2>), grid, blk, 0, st, g, w, mesh);
-  } else {
+  else
     hipLaunchKernelGGL((k_paint_tiles<T, 3>), grid, blk, 0, st, P, g, w, mesh, mass_fx, inv_fx);
-    hipLaunchKernelGGL((k_paint_halo<T, 3>), grid, blk, 0, st, g, w, mesh);
-  }
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }
@@ -482,6 +591,16 @@ static int paint_tiles_impl(const fpm_particles* part, const fpm_mesh* m, int or
 template <typename T, int ORDER>
 static void launch_rt(const PartDesc<T>& P, const TileGeom& g, const Workspace& w,
                       const TileRead<T>& R, int nc, hipStream_t st) {
+  if (!FPM_READOUT_LDS) {
+    const dim3 gl(grid_parts(P.n)), bl(256);
+    switch (nc) {
+      case 1: hipLaunchKernelGGL((k_readout_list<T, ORDER, 1>), gl, bl, 0, st, P, g, w, R); break;
+      case 2: hipLaunchKernelGGL((k_readout_list<T, ORDER, 2>), gl, bl, 0, st, P, g, w, R); break;
+      case 3: hipLaunchKernelGGL((k_readout_list<T, ORDER, 3>), gl, bl, 0, st, P, g, w, R); break;
+      default: hipLaunchKernelGGL((k_readout_list<T, ORDER, 4>), gl, bl, 0, st, P, g, w, R); break;
+    }
+    return;
+  }
   const dim3 grid((unsigned)g.ntiles), blk(kTileThreads);
   switch (nc) {
     case 1: hipLaunchKernelGGL((k_readout_tiles<T, ORDER, 1>), grid, blk, 0, st, P, g, w, R); break;
@@ -524,16 +643,16 @@ extern "C" {
 
 int fpm_tiles_supported(const fpm_mesh* mesh) { return tiles_ok(mesh) ? 1 : 0; }
 
-int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n) {
-  if (!tiles_ok(mesh) || (order != 2 && order != 3) || n < 0) return -1;
-  int64_t off[8];
-  ws_offsets(tile_geom(mesh, order), n, sizeof(float), off);
-  return off[5];
-}
-
 int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n) {
   if (!tiles_ok(mesh) || (order != 2 && order != 3) || n < 0) return -1;
-  return layout_ws(tile_geom(mesh, order), n, sizeof(float), nullptr, nullptr);
+  return layout_ws(tile_geom(mesh), n, nullptr, nullptr);
+}
+
+int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n) {
+  if (!tiles_ok(mesh) || (order != 2 && order != 3) || n < 0) return -1;
+  int64_t off[7];
+  ws_offsets(tile_geom(mesh), n, off);
+  return off[5];
 }
 
 int fpm_bin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,

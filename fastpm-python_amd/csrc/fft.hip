@@ -204,10 +204,17 @@ template <typename T, int N> struct ColW {
   static constexpr int BZ = a < b ? a : b;
 };
 
-// Lines per workgroup for the z pass: about 2048 complex elements.
+#ifndef FPM_Z_ELEMS
+#define FPM_Z_ELEMS 2048  // complex elements per z-pass workgroup
+#endif
+#ifndef FPM_Z_EPT
+#define FPM_Z_EPT 8  // complex elements per thread in the z pass
+#endif
+
+// Lines per workgroup for the z pass: about FPM_Z_ELEMS complex elements.
 constexpr int z_lines(int M) {
   int nl = 1;
-  while (nl * 2 * M <= 2048) nl *= 2;
+  while (nl * 2 * M <= FPM_Z_ELEMS) nl *= 2;
   return nl;
 }
 
@@ -310,14 +317,19 @@ struct ColArgs {
   int eaxis, oaxis;
 };
 
-template <typename T, int N, int BZ, int NT, bool INV, bool KOP>
+// NCP outputs are transformed together: their tiles sit side by side in LDS
+// ([N][NCP * BZ]) and every Stockham stage runs over all NCP * BZ lines, so a
+// multi-output pass costs one set of stage barriers per NCP outputs.
+template <typename T, int N, int BZ, int NT, bool INV, bool KOP, int NCP>
 __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   using V = C<T>;
   constexpr int E = N * BZ;
+  constexpr int W = NCP * BZ;  // LDS lines
   constexpr int PER = (E + NT - 1) / NT;
-  // The x-axis factor table is kept in LDS when it fits beside the tile.
-  constexpr bool TXL = KOP && ((E + N) * sizeof(V) + N * sizeof(AxisTab<T>) <= 150 * 1024);
-  __shared__ V tile[E];
+  // The e-axis factor table is kept in LDS when it fits beside the tiles.
+  constexpr bool TXL =
+      KOP && ((NCP * E + N) * sizeof(V) + N * sizeof(AxisTab<T>) <= 150 * 1024);
+  __shared__ V tile[NCP * E];
   __shared__ V tw[N];
   __shared__ AxisTab<T> tx[TXL ? N : 1];
   __shared__ AxisTab<T> tz[KOP ? BZ : 1];
@@ -350,32 +362,44 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   }
   if constexpr (KOP) __syncthreads();  // axis tables ready
 
-  for (int c = 0; c < a.ncomp; ++c) {
+  for (int c0 = 0; c0 < a.ncomp; c0 += NCP) {
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = threadIdx.x + q * NT;
-      if ((E % NT == 0) || idx < E) {
-        V v = reg[q];
-        if constexpr (KOP) {
+    for (int j = 0; j < NCP; ++j) {
+      const int c = c0 + j;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        if ((E % NT == 0) || idx < E) {
           const int l = idx % BZ, e = idx / BZ;
-          const AxisTab<T> axe =
-              TXL ? tx[e] : axis_tab<T>(e, a.kop.nm[a.eaxis], a.kop.box[a.eaxis], false);
-          const T m = a.eaxis == 0 ? kop_mult<T>(a.kop, c, axe, ty, tz[l])
-                                   : kop_mult<T>(a.kop, c, ty, axe, tz[l]);
-          v = apply_mult(v, m, a.kop.ngrad[c]);
+          V v = reg[q];
+          if constexpr (KOP) {
+            if (c < a.ncomp) {
+              const AxisTab<T> axe =
+                  TXL ? tx[e] : axis_tab<T>(e, a.kop.nm[a.eaxis], a.kop.box[a.eaxis], false);
+              const T m = a.eaxis == 0 ? kop_mult<T>(a.kop, c, axe, ty, tz[l])
+                                       : kop_mult<T>(a.kop, c, ty, axe, tz[l]);
+              v = apply_mult(v, m, a.kop.ngrad[c]);
+            }
+          }
+          tile[e * W + j * BZ + l] = v;
         }
-        tile[idx] = v;
       }
     }
     __syncthreads();
-    Stages<T, N, 0, 1, BZ, NT, true, BZ, 1, 1, INV>::run(tile, tw);
-    V* dst = a.out[c] + (int64_t)outer * a.ostride + z0;
+    Stages<T, N, 0, 1, W, NT, true, W, 1, 1, INV>::run(tile, tw);
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = threadIdx.x + q * NT;
-      if ((E % NT == 0) || idx < E) {
-        const int l = idx % BZ, e = idx / BZ;
-        if (z0 + l < a.ncols) dst[(int64_t)e * a.estride + l] = cscale(tile[idx], a.scale);
+    for (int j = 0; j < NCP; ++j) {
+      const int c = c0 + j;
+      if (c >= a.ncomp) break;
+      V* dst = a.out[c] + (int64_t)outer * a.ostride + z0;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        if ((E % NT == 0) || idx < E) {
+          const int l = idx % BZ, e = idx / BZ;
+          if (z0 + l < a.ncols)
+            dst[(int64_t)e * a.estride + l] = cscale(tile[e * W + j * BZ + l], a.scale);
+        }
       }
     }
     __syncthreads();
@@ -547,7 +571,17 @@ static void launch_col_n(ColArgs<T> a, hipStream_t st) {
   a.ntz = (a.ncols + BZ - 1) / BZ;
   a.nblk = (int64_t)a.ntz * a.nouter;
   if (a.nblk <= 0) return;
-  hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP>), dim3((unsigned)a.nblk), dim3(NT), 0,
+  // two outputs per sweep when both tiles fit in LDS and the doubled
+  // butterflies per thread stay within the register budget of NT threads
+  constexpr bool PAIR = KOP && NT < 1024 && (2 * N * BZ + N) * sizeof(C<T>) <= 144 * 1024;
+  if constexpr (PAIR) {
+    if (a.ncomp >= 2) {
+      hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, 2>), dim3((unsigned)a.nblk), dim3(NT),
+                         0, st, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, 1>), dim3((unsigned)a.nblk), dim3(NT), 0,
                      st, a);
 }
 
@@ -566,7 +600,7 @@ static int launch_col(int n, ColArgs<T> a, hipStream_t st) {
 template <typename T, int M, bool INV>
 static void launch_z_m(const ZArgs<T>& a, hipStream_t st) {
   constexpr int NL = z_lines(M);
-  constexpr int NT = clampi(round64(NL * M / 8), 64, 1024);
+  constexpr int NT = clampi(round64(NL * M / FPM_Z_EPT), 64, 1024);
   const unsigned nblk = (unsigned)((a.nrows + NL - 1) / NL);
   if (INV)
     hipLaunchKernelGGL((k_fft_z_c2r<T, M, NL, NT>), dim3(nblk), dim3(NT), 0, st, a);

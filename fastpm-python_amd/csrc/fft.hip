@@ -204,6 +204,9 @@ template <typename T, int N> struct ColW {
   static constexpr int BZ = a < b ? a : b;
 };
 
+#ifndef FPM_FFT_PAIR
+#define FPM_FFT_PAIR 0  // transform two outputs per LDS sweep in multi-output passes
+#endif
 #ifndef FPM_Z_ELEMS
 #define FPM_Z_ELEMS 2048  // complex elements per z-pass workgroup
 #endif
@@ -287,6 +290,26 @@ __device__ __forceinline__ T kop_mult(const KopPack& K, int c, const AxisTab<T>&
   return m;
 }
 
+// The axis-separable part of component c's multiplier along one mesh axis at
+// mode index i: its gradient factors along this axis and the window
+// compensation 1/sinc(k h/2)^p of this axis.  Also returns k^2 of the axis.
+__device__ __forceinline__ double comp_axis_factor(const KopPack& K, int c, int axis, int i,
+                                                   bool half, double& kk) {
+  double g0, g1, sn;
+  axis_factors(i, K.nm[axis], K.box[axis], half, kk, g0, g1, sn);
+  double f = 1.0;
+  for (int g = 0; g < K.ngrad[c]; ++g)
+    if (K.gdir[c][g] == axis) f *= K.grad_order[c] == 0 ? g0 : g1;
+  for (int q = 0; q < K.deconv[c]; ++q) f /= sn;
+  return f;
+}
+
+template <typename T> __device__ __forceinline__ T fast_rcp(T x);
+template <> __device__ __forceinline__ float fast_rcp<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+template <> __device__ __forceinline__ double fast_rcp<double>(double x) { return 1.0 / x; }
+
 template <typename V>
 __device__ __forceinline__ V apply_mult(V v, decltype(V::x) m, int ngrad) {
   switch (ngrad & 3) {
@@ -326,27 +349,46 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   constexpr int E = N * BZ;
   constexpr int W = NCP * BZ;  // LDS lines
   constexpr int PER = (E + NT - 1) / NT;
-  // The e-axis factor table is kept in LDS when it fits beside the tiles.
-  constexpr bool TXL =
-      KOP && ((NCP * E + N) * sizeof(V) + N * sizeof(AxisTab<T>) <= 150 * 1024);
+  constexpr int MAXC = 6;
   __shared__ V tile[NCP * E];
   __shared__ V tw[N];
-  __shared__ AxisTab<T> tx[TXL ? N : 1];
-  __shared__ AxisTab<T> tz[KOP ? BZ : 1];
+  // k-space multiplier tables: per-component separable factors along the
+  // e axis (fe), the z columns (fz) and the outer index (fo, times the
+  // component's factor), and k^2 along e (ke) and z (kz)
+  __shared__ T fe[KOP ? MAXC * N : 1];
+  __shared__ T ke[KOP ? N : 1];
+  __shared__ T fz[KOP ? MAXC * BZ : 1];
+  __shared__ T kz[KOP ? BZ : 1];
+  __shared__ T fo[KOP ? MAXC : 1];
 
   const int64_t bid = xcd_remap(blockIdx.x, a.nblk);
   const int outer = (int)(bid / a.ntz);
   const int z0 = (int)(bid % a.ntz) * BZ;
 
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = a.tw[i];
-  AxisTab<T> ty{};
+  T ko = (T)0;
   if constexpr (KOP) {
-    if constexpr (TXL)
-      for (int i = threadIdx.x; i < N; i += NT)
-        tx[i] = axis_tab<T>(i, a.kop.nm[a.eaxis], a.kop.box[a.eaxis], false);
-    for (int i = threadIdx.x; i < BZ; i += NT)
-      tz[i] = axis_tab<T>(z0 + i, a.kop.nm[2], a.kop.box[2], true);
-    ty = axis_tab<T>(outer, a.kop.nm[a.oaxis], a.kop.box[a.oaxis], false);
+    const KopPack& K = a.kop;
+    for (int i = threadIdx.x; i < N * a.ncomp; i += NT) {
+      const int c = i / N, e = i % N;
+      double kk;
+      fe[c * N + e] = (T)comp_axis_factor(K, c, a.eaxis, e, false, kk);
+      if (c == 0) ke[e] = (T)kk;
+    }
+    for (int i = threadIdx.x; i < BZ * a.ncomp; i += NT) {
+      const int c = i / BZ, l = i % BZ;
+      double kk;
+      fz[c * BZ + l] = (T)comp_axis_factor(K, c, 2, z0 + l, true, kk);
+      if (c == 0) kz[l] = (T)kk;
+    }
+    if (threadIdx.x < a.ncomp) {
+      double kk;
+      const int c = threadIdx.x;
+      fo[c] = (T)(K.factor[c] * comp_axis_factor(K, c, a.oaxis, outer, false, kk));
+    }
+    double kko, g0, g1, sn;
+    axis_factors(outer, K.nm[a.oaxis], K.box[a.oaxis], false, kko, g0, g1, sn);
+    ko = (T)kko;
   }
 
   const V* src = a.in + (int64_t)outer * a.ostride + z0;
@@ -360,12 +402,22 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
       if (z0 + l < a.ncols) reg[q] = src[(int64_t)e * a.estride + l];
     }
   }
-  if constexpr (KOP) __syncthreads();  // axis tables ready
+  if constexpr (KOP) __syncthreads();  // multiplier tables ready
 
   for (int c0 = 0; c0 < a.ncomp; c0 += NCP) {
 #pragma unroll
     for (int j = 0; j < NCP; ++j) {
       const int c = c0 + j;
+      bool lap = false;
+      T r2 = (T)0;
+      int ng = 0;
+      if constexpr (KOP) {
+        if (c < a.ncomp) {
+          lap = a.kop.laplace[c] != 0;
+          r2 = (T)a.kop.r2[c];
+          ng = a.kop.ngrad[c];
+        }
+      }
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
         const int idx = threadIdx.x + q * NT;
@@ -374,11 +426,13 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
           V v = reg[q];
           if constexpr (KOP) {
             if (c < a.ncomp) {
-              const AxisTab<T> axe =
-                  TXL ? tx[e] : axis_tab<T>(e, a.kop.nm[a.eaxis], a.kop.box[a.eaxis], false);
-              const T m = a.eaxis == 0 ? kop_mult<T>(a.kop, c, axe, ty, tz[l])
-                                       : kop_mult<T>(a.kop, c, ty, axe, tz[l]);
-              v = apply_mult(v, m, a.kop.ngrad[c]);
+              T m = fo[c] * fe[c * N + e] * fz[c * BZ + l];
+              if (lap || r2 != (T)0) {
+                const T kk = ke[e] + ko + kz[l];
+                if (lap) m = kk == (T)0 ? (T)0 : m * fast_rcp<T>(kk);
+                if (r2 != (T)0) m *= exp(-kk * r2);
+              }
+              v = apply_mult(v, m, ng);
             }
           }
           tile[e * W + j * BZ + l] = v;
@@ -494,16 +548,21 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(ZArgs<T> a) {
   constexpr int PERP = (NL * NP + NT - 1) / NT;
   __shared__ V s[NL * LP];
   __shared__ V tw[2 * M];
+  __shared__ T zg[M + 1];
   const int64_t row0 = (int64_t)blockIdx.x * NL;
   for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = a.tw[i];
+  if (a.zgrad) {
+    for (int k = threadIdx.x; k <= M; k += NT) {
+      const AxisTab<T> az = axis_tab<T>(k, 2 * M, a.L2, true);
+      zg[k] = a.zorder == 0 ? az.g0 : az.g1;
+    }
+    __syncthreads();
+  }
   for (int idx = threadIdx.x; idx < NL * LP; idx += NT) {
     const int l = idx / LP, k = idx % LP;
     const int64_t row = row0 + l;
     V v = row < a.nrows ? a.cin[row * a.pitch2 + k] : cmk<V>((T)0, (T)0);
-    if (a.zgrad) {
-      const AxisTab<T> az = axis_tab<T>(k, 2 * M, a.L2, true);
-      v = apply_mult(v, a.zorder == 0 ? az.g0 : az.g1, 1);
-    }
+    if (a.zgrad) v = apply_mult(v, zg[k], 1);
     s[l * LP + k] = v;
   }
   __syncthreads();
@@ -571,9 +630,12 @@ static void launch_col_n(ColArgs<T> a, hipStream_t st) {
   a.ntz = (a.ncols + BZ - 1) / BZ;
   a.nblk = (int64_t)a.ntz * a.nouter;
   if (a.nblk <= 0) return;
-  // two outputs per sweep when both tiles fit in LDS and the doubled
+  // two outputs per sweep when enabled, both tiles fit in LDS and the doubled
   // butterflies per thread stay within the register budget of NT threads
-  constexpr bool PAIR = KOP && NT < 1024 && (2 * N * BZ + N) * sizeof(C<T>) <= 144 * 1024;
+  // (measured slower at 512^3 on gfx950: 2.98 vs 2.30 ms for three force
+  // components, so off by default)
+  constexpr bool PAIR = FPM_FFT_PAIR && KOP && NT < 1024 &&
+                        (2 * N * BZ + N) * sizeof(C<T>) <= 144 * 1024;
   if constexpr (PAIR) {
     if (a.ncomp >= 2) {
       hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, 2>), dim3((unsigned)a.nblk), dim3(NT),

@@ -184,7 +184,8 @@ def main():
     ro_calls, ro_ms = kern["readout"]
     ro_bytes = 16 * Nc + 28 * Np  # 4 meshes read once + S in + RHO/F out
     ach = ro_bytes / (ro_ms * 1e-3) / 1e9
-    out["roofline"] = {"kernel": "k_readout<float,2,4> (fused RHO+F CIC readout)", "bound": "hbm",
+    out["roofline"] = {"kernel": "k_readout_tiles<float,2,4> (fused RHO+F CIC readout)",
+                       "bound": "hbm",
                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS, "traffic": None,
                        "algorithmic_bytes": ro_bytes}

@@ -371,6 +371,10 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
   tile_coords(t, g, lo);
   const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
   if (b == e) return;  // uniform across the workgroup
+  // The first particle's index is fetched before the box so both latencies
+  // overlap.
+  const int32_t s0 = b + threadIdx.x;
+  const int64_t p0 = s0 < e ? w.list[s0] : 0;
   // Stage the box: each (component, x, y) row holds TILE aligned cells loaded
   // as TILE/4 four-wide vectors, plus H wrapped overhang cells.  All of a
   // thread's loads are issued before any LDS store.
@@ -431,8 +435,8 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
     }
   }
   __syncthreads();
-  for (int32_t s = b + threadIdx.x; s < e; s += kTileThreads) {
-    const int64_t p = w.list[s];
+  for (int32_t s = s0; s < e; s += kTileThreads) {
+    const int64_t p = s == s0 ? p0 : (int64_t)w.list[s];
     int i0[3];
     T wt[3][ORDER];
     window_of<T, ORDER>(P, p, g.nm, i0, wt);

@@ -69,6 +69,11 @@ _TYPED = {
     "fpm_positions": (_int, [_PP, _PM, _vp, _vp]),
     "fpm_power_bin": (_int, [_vp, _vp, _PM, _i64, _int, _dbl, _dbl, _vp, _i64, _vp, _vp]),
     "fpm_twiddles": (_int, [_int, _vp]),
+    "fpm_paint_weighted": (_int, [_PP, _PM, _int, _vp, _i64, _dbl, _vp, _int, _vp]),
+    "fpm_window_grad": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _dbl, _vp, _int, _vp]),
+    "fpm_kspace_accumulate": (_int, [_vp, _vp, _PK, _int, _int, _PM, _i64, _vp]),
+    "fpm_hermitian_scale": (_int, [_vp, _dbl, _dbl, _PM, _i64, _vp]),
+    "fpm_lpt2_source_vjp": (_int, [_vp, _vp, _vp, _i64, _vp]),
 }
 _PLAIN = {
     "fpm_version": (ctypes.c_char_p, []),

@@ -228,6 +228,58 @@ int fpm_power_bin_f64(const void* f1, const void* f2, const fpm_mesh* mesh, int6
                       int nbins, double kmin, double dk, double* work, int64_t work_len,
                       double* out, fpm_stream_t stream);
 
+/* ---- Reverse-mode (adjoint) kernels.  They replace the pmesh gradient
+ * operators the reference's hand-written adjoints call
+ * (fastpm/operators.py:68-98,140-193,218-253: readout_gradient,
+ * paint_gradient, c2r_gradient, r2c_gradient, conjugated kernels). */
+
+/* Paint with per-particle mass scale * weights[p * wstride] (the adjoint of
+ * readout with respect to the mesh; readout_gradient's mesh output). */
+int fpm_paint_weighted_f32(const fpm_particles* part, const fpm_mesh* mesh, int order,
+                           const void* weights, int64_t wstride, double scale, void* out,
+                           int accumulate, fpm_stream_t stream);
+int fpm_paint_weighted_f64(const fpm_particles* part, const fpm_mesh* mesh, int order,
+                           const void* weights, int64_t wstride, double scale, void* out,
+                           int accumulate, fpm_stream_t stream);
+
+/* Position derivative of the window: gx[p][d] (+)= scale * sum_c w_c[p] *
+ * sum_cells meshes[c](cell) dW(x_p - cell)/dx_d (weights == NULL or
+ * weights[c] == NULL means 1).  Readout's position gradient
+ * (readout_gradient) and paint's (paint_gradient). */
+int fpm_window_grad_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, int ncomp,
+                        const void* const* meshes, const void* const* weights,
+                        const int64_t* wstrides, double scale, void* gx, int accumulate,
+                        fpm_stream_t stream);
+int fpm_window_grad_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, int ncomp,
+                        const void* const* meshes, const void* const* weights,
+                        const int64_t* wstrides, double scale, void* gx, int accumulate,
+                        fpm_stream_t stream);
+
+/* acc (+)= M(k) in, or conj(M(k)) in when conj != 0, for the multiplier M of
+ * `op` (the adjoint of a k-space multiplier is its conjugate,
+ * fastpm/operators.py:13-22 diff_kernel(conjugate=True)). */
+int fpm_kspace_accumulate_f32(void* acc, const void* in, const fpm_kop* op, int conj,
+                              int accumulate, const fpm_mesh* mesh, int64_t pitch2,
+                              fpm_stream_t stream);
+int fpm_kspace_accumulate_f64(void* acc, const void* in, const fpm_kop* op, int conj,
+                              int accumulate, const fpm_mesh* mesh, int64_t pitch2,
+                              fpm_stream_t stream);
+
+/* Scales the half-spectrum modes with 0 < k_z index < n2/2 by `mid` and the
+ * k_z = 0 / n2/2 planes by `edge` (Hermitian bookkeeping of c2r_gradient /
+ * r2c_gradient; fastpm/operators.py:94-96). */
+int fpm_hermitian_scale_f32(void* field, double mid, double edge, const fpm_mesh* mesh,
+                            int64_t pitch2, fpm_stream_t stream);
+int fpm_hermitian_scale_f64(void* field, double mid, double edge, const fpm_mesh* mesh,
+                            int64_t pitch2, fpm_stream_t stream);
+
+/* gphi[i] = gsource * d source / d phi_i for the 2LPT source of
+ * fpm_lpt2_source (fastpm/operators.py:140-193). */
+int fpm_lpt2_source_vjp_f32(const void* gsource, const void* const* phi, void* const* gphi,
+                            int64_t ncells, fpm_stream_t stream);
+int fpm_lpt2_source_vjp_f64(const void* gsource, const void* const* phi, void* const* gphi,
+                            int64_t ncells, fpm_stream_t stream);
+
 /* Growth functions of a matter + Lambda (+ curvature) universe, host-side
  * double precision.  For each a[i] writes 9 values to out[9 i ...]:
  * D1, dD1/dlna, d2D1/dlna2, D2, dD2/dlna, d2D2/dlna2, E, dE/dlna, Omega_m(a),

@@ -1,0 +1,211 @@
+"""Adjoint (reverse-mode) kernels and the differentiable forward model (GPU).
+
+Pins: adjoint dot-product identities of each primitive, central finite
+differences of the position derivative, finite differences of the whole
+model's loss on random white-noise modes (float64, also against the FP64
+oracle's forward model), and float32 vs float64 gradients."""
+import numpy as np
+import pytest
+import torch
+
+import fastpm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _pm(nmesh, box, dtype, resampler="cic"):
+    from fastpm.pm import ParticleMesh
+
+    return ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype=dtype, resampler=resampler)
+
+
+def _parts(pm, n, rng, gpu):
+    from fastpm.pm import Particles
+
+    q = rng.uniform(0, 1, size=(n, 3)) * pm.BoxSize
+    s = rng.normal(0, 2.0, size=(n, 3))
+    t = lambda a: torch.from_numpy(a.astype(pm.dtype)).to(gpu)  # noqa: E731
+    return Particles(t(q), t(s)), q + s
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_paint_weighted_is_readout_adjoint(gpu, order):
+    from fastpm.adjoint import paint_weighted
+
+    rng = np.random.default_rng(1)
+    pm = _pm((16, 16, 16), 40.0, "f8", "cic" if order == 2 else "tsc")
+    parts, X = _parts(pm, 500, rng, gpu)
+    wts = torch.from_numpy(rng.standard_normal(500)).to(gpu)
+    m = rng.standard_normal((16, 16, 16))
+    mesh = paint_weighted(pm, parts, wts, scale=0.5)
+    lhs = float((mesh.value.cpu().numpy() * m).sum())
+    rhs = 0.5 * float((O.readout(m, X, (40.0,) * 3, order) * wts.cpu().numpy()).sum())
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+    ref = O.paint(X, (16, 16, 16), (40.0,) * 3, order, 1.0)  # weights 1 sanity
+    ones = torch.ones(500, dtype=torch.float64, device=gpu)
+    np.testing.assert_allclose(paint_weighted(pm, parts, ones).value.cpu().numpy(), ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_window_grad_finite_difference(gpu, order):
+    from fastpm.adjoint import window_grad
+    from fastpm.pm import Particles
+
+    rng = np.random.default_rng(2)
+    pm = _pm((16, 16, 16), 40.0, "f8", "cic" if order == 2 else "tsc")
+    n = 300
+    # keep particles away from cell faces so the window is smooth around them
+    cells = rng.integers(0, 16, size=(n, 3))
+    X = (cells + rng.uniform(0.2, 0.8, size=(n, 3))) * 2.5
+    if order == 3:
+        X = (cells + 0.5 + rng.uniform(-0.3, 0.3, size=(n, 3))) * 2.5
+    m = rng.standard_normal((16, 16, 16))
+    field = pm.create("real", value=torch.from_numpy(m).to(gpu))
+    w = torch.from_numpy(rng.standard_normal(n)).to(gpu)
+    gx = torch.zeros((n, 3), dtype=torch.float64, device=gpu)
+    parts = Particles(torch.from_numpy(X).to(gpu), None)
+    window_grad(pm, parts, [field], [w], gx, scale=1.0, accumulate=False)
+    h = 1e-5
+    for d in range(3):
+        e = np.zeros(3)
+        e[d] = h
+        fd = (O.readout(m, X + e, (40.0,) * 3, order) - O.readout(m, X - e, (40.0,) * 3, order)) / (2 * h)
+        np.testing.assert_allclose(gx[:, d].cpu().numpy(), fd * w.cpu().numpy(), rtol=1e-6, atol=1e-9)
+
+
+def test_fft_and_multiplier_adjoint_identities(gpu):
+    """<c2r(M X), g> = <X, conj(M) c2r^T(g)> with c2r^T(g) = N w_c r2c(g)
+    and the real inner product of stored modes."""
+    from fastpm.adjoint import _kaccum, _herm
+    from fastpm.pm import ComplexField
+    from fastpm.force.kernels import laplace, gradient, compensation
+
+    rng = np.random.default_rng(3)
+    nm = (16, 12, 8)
+    pm = _pm(nm, (50.0, 40.0, 30.0), "f8")
+    X = pm.create("real", value=torch.from_numpy(rng.standard_normal(nm)).to(gpu)).r2c()
+    g = rng.standard_normal(nm)
+    gf = pm.create("real", value=torch.from_numpy(g).to(gpu))
+    N = float(np.prod(nm))
+    for op in [laplace * gradient(0, 1), gradient(2, 0) * compensation(2), laplace * 3.0]:
+        y = X.apply(op).c2r().value.cpu().numpy()
+        lhs = float((y * g).sum())
+        G = gf.r2c()
+        _herm(pm, G, 2 * N, N)
+        acc = ComplexField(pm)
+        _kaccum(pm, acc, G, op, conj=True, accumulate=False)
+        a, b = X.value.cpu().numpy(), acc.value.cpu().numpy()
+        rhs = float((a.real * b.real + a.imag * b.imag).sum())
+        assert lhs == pytest.approx(rhs, rel=1e-10)
+
+
+def _model(dtype, nc=16, B=2, box=100.0, nsteps=3, seed_t=99, deconv=True, resampler="cic"):
+    from fastpm.core import Solver
+    from fastpm.pm import ParticleMesh
+    from fastpm.cosmology import Cosmology, LinearPower
+    from fastpm.adjoint import ForwardModel, DensityLoss
+
+    cosmo = Cosmology()
+    Pk = LinearPower(cosmo)
+    pm = ParticleMesh(Nmesh=[nc] * 3, BoxSize=box, dtype=dtype, resampler=resampler)
+    solver = Solver(pm, cosmo, B=B, compensate=deconv)
+    stages = np.linspace(0.1, 1.0, nsteps + 1)
+    model = ForwardModel(solver, Pk, stages)
+    tgt_state = model.forward(solver.whitenoise(seed_t))
+    loss = DensityLoss(solver, Pk, None)
+    loss.target = loss.density_k(tgt_state)
+    return solver, model, loss, Pk
+
+
+@pytest.mark.parametrize("resampler,eps,tol", [("tsc", 1e-3, 2e-2), ("cic", 1e-6, 2e-3)])
+def test_model_gradient_finite_differences(gpu, resampler, eps, tol):
+    """Central differences of the float64 loss on 16 random mode components
+    with 0 < k_z < n/2 against the adjoint gradient.  TSC windows are C1, so
+    eps = 1e-3 converges; CIC windows are only C0 (the loss has kinks where a
+    particle crosses a cell face), so CIC uses a small eps.  Most components
+    agree to ~1e-9; the few large-gradient modes carry the O(eps^2) truncation
+    error of the difference quotient (~1e-3 relative at eps = 1e-3)."""
+    solver, model, loss, _ = _model("f8", resampler=resampler)
+    w = solver.whitenoise(7)
+    st = model.forward(w)
+    L0, gS = loss(st)
+    G = model.backward(gS).value.cpu().numpy()
+    rng = np.random.default_rng(5)
+    nc = 16
+    fds, gs = [], []
+    for n in range(16):
+        i, j, k = rng.integers(0, nc), rng.integers(0, nc), rng.integers(1, nc // 2)
+        part = 1.0 if n % 2 == 0 else 1j
+        wp = w.copy()
+        wp.value[i, j, k] += eps * part
+        wm = w.copy()
+        wm.value[i, j, k] -= eps * part
+        Lp, _ = loss(model.forward(wp), grad=False)
+        Lm, _ = loss(model.forward(wm), grad=False)
+        fds.append((Lp - Lm) / (2 * eps))
+        gs.append(G[i, j, k].real if part == 1.0 else G[i, j, k].imag)
+    fds, gs = np.array(fds), np.array(gs)
+    print("fd", fds, "adjoint", gs)
+    assert np.linalg.norm(fds - gs) / np.linalg.norm(gs) < tol
+    assert np.median(np.abs(fds - gs) / np.abs(gs)) < 1e-6
+
+
+def test_model_gradient_vs_oracle_forward(gpu):
+    """The adjoint gradient against central differences of the FP64 oracle's
+    forward model (an independent implementation of the same loss)."""
+    solver, model, loss, Pk = _model("f8", nc=8, box=64.0, nsteps=2)
+    w = solver.whitenoise(11)
+    st = model.forward(w)
+    _, gS = loss(st)
+    G = model.backward(gS).value.cpu().numpy()
+    osol = O.Solver(8, 64.0, solver.cosmology.Om0, B=2, deconv=2)
+    q = O.lattice((8,) * 3, (64.0,) * 3)
+    tgt = loss.target.value.cpu().numpy()
+    fnm = (16, 16, 16)
+
+    def oloss(wn):
+        stt = osol.lpt(osol.linear(wn, Pk), q, 0.1)
+        osol.nbody(stt, O.leapfrog(model.stages))
+        d = O.r2c(O.paint(stt.X, fnm, (64.0,) * 3, 2) / (8 ** 3 / 16 ** 3))
+        kx, ky, kz = O.kvectors(fnm, (64.0,) * 3)
+        kk = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+        with np.errstate(divide="ignore"):
+            ip = np.where(kk > 0, 1.0 / Pk(np.where(kk > 0, kk, 1.0)), 0.0)
+        wgt = np.where((np.arange(9) == 0) | (np.arange(9) == 8), 1.0, 2.0)[None, None, :]
+        return float((wgt * np.abs(d - tgt) ** 2 * ip).sum())
+
+    w0 = w.value.cpu().numpy()
+    rng = np.random.default_rng(6)
+    eps = 1e-4
+    for _ in range(4):
+        i, j, k = rng.integers(0, 8), rng.integers(0, 8), rng.integers(1, 4)
+        wp, wm = w0.copy(), w0.copy()
+        wp[i, j, k] += eps
+        wm[i, j, k] -= eps
+        fd = (oloss(wp) - oloss(wm)) / (2 * eps)
+        assert G[i, j, k].real == pytest.approx(fd, rel=1e-4, abs=1e-8 * abs(fd) + 1e-12)
+
+
+def test_fp32_gradient_matches_fp64(gpu):
+    res = {}
+    for dt in ("f8", "f4"):
+        solver, model, loss, _ = _model(dt, nc=32, box=200.0, nsteps=5)
+        st = model.forward(solver.whitenoise(7))
+        _, gS = loss(st)
+        res[dt] = model.backward(gS).value.cpu().numpy().astype("c16")
+    a, b = res["f4"].ravel(), res["f8"].ravel()
+    corr = np.vdot(a, b).real / np.sqrt(np.vdot(a, a).real * np.vdot(b, b).real)
+    assert corr > 0.9999
+    assert np.abs(a - b).max() < 1e-2 * np.abs(b).max()
+
+
+def test_autograd_function(gpu):
+    from fastpm.adjoint import SimulateFunction
+
+    solver, model, loss, _ = _model("f8", nc=8, box=64.0, nsteps=2)
+    w = solver.whitenoise(3).value.clone().requires_grad_(True)
+    S = SimulateFunction.apply(w, model)
+    S.sum().backward()
+    # d(sum S)/dw through the same adjoint path directly
+    G = model.backward(torch.ones_like(S)).value
+    assert torch.allclose(w.grad, G)

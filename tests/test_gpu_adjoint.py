@@ -209,3 +209,40 @@ def test_autograd_function(gpu):
     # d(sum S)/dw through the same adjoint path directly
     G = model.backward(torch.ones_like(S)).value
     assert torch.allclose(w.grad, G)
+
+
+def test_config4_gradient(gpu):
+    """The differentiable configuration: L = 500 Mpc/h, 128^3 particles on a
+    256^3 mesh, 10 steps from a = 0.1, loss against the density of seed 99
+    divided by P(k); float32 gradient with respect to the white-noise modes
+    against the float64 gradient and against central differences (eps = 1e-3)
+    of the float64 loss on 32 random modes."""
+    print("config4", flush=True)
+    res = {}
+    for dt in ("f8", "f4"):
+        solver, model, loss, _ = _model(dt, nc=128, box=500.0, nsteps=10, deconv=False)
+        w = solver.whitenoise(7)
+        st = model.forward(w)
+        L0, gS = loss(st)
+        res[dt] = (model.backward(gS).value.cpu().numpy().astype("c16"), solver, model, loss, w, L0)
+    g32, g64 = res["f4"][0].ravel(), res["f8"][0].ravel()
+    corr = np.vdot(g32, g64).real / np.sqrt(np.vdot(g32, g32).real * np.vdot(g64, g64).real)
+    assert corr > 0.9999
+    _, solver, model, loss, w, L0 = res["f8"]
+    G = res["f8"][0]
+    rng = np.random.default_rng(32)
+    fds, gs = [], []
+    for n in range(32):
+        i, j, k = rng.integers(0, 128), rng.integers(0, 128), rng.integers(1, 64)
+        part = 1.0 if n % 2 == 0 else 1j
+        wp, wm = w.copy(), w.copy()
+        wp.value[i, j, k] += 1e-3 * part
+        wm.value[i, j, k] -= 1e-3 * part
+        fd = (loss(model.forward(wp), grad=False)[0] - loss(model.forward(wm), grad=False)[0]) / 2e-3
+        fds.append(fd)
+        gs.append(G[i, j, k].real if part == 1.0 else G[i, j, k].imag)
+    fds, gs = np.array(fds), np.array(gs)
+    rel = np.abs(fds - gs) / np.abs(gs)
+    print("config4 corr32/64 %.8f median rel %.3e max rel %.3e" % (corr, np.median(rel), rel.max()))
+    assert np.median(rel) < 1e-3
+    assert np.linalg.norm(fds - gs) / np.linalg.norm(gs) < 5e-2

@@ -21,6 +21,24 @@ __global__ void k_axpy(T* y, const T* x, int64_t n, T a) {
     y[i] += a * x[i];
 }
 
+// y += table[counter[0] * stride + slot] * x: the coefficient is read on the
+// device, so one captured graph can replay every step of a run.  A counter
+// outside [0, rows) leaves y unchanged.
+template <typename T>
+__global__ void k_axpy_table(T* y, const T* x, int64_t n, const double* table,
+                             const int32_t* counter, int rows, int stride, int slot) {
+  const int32_t row = counter[0];
+  if (row < 0 || row >= rows) return;
+  const T a = (T)table[(int64_t)row * stride + slot];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+__global__ void k_counter_add(int32_t* counter, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += v;
+}
+
 template <typename T>
 __global__ void k_lpt_combine(const T* dx1, const T* dx2, T* S, T* P, T* F, int64_t n, T c0, T c1,
                               T c2, T c3, T c4, T c5) {
@@ -68,6 +86,20 @@ static int axpy_impl(void* y, const void* x, int64_t n, double a, hipStream_t st
 }
 
 template <typename T>
+static int axpy_table_impl(void* y, const void* x, int64_t n, const double* table,
+                           const int32_t* counter, int rows, int stride, int slot,
+                           hipStream_t st) {
+  if (n < 0 || !table || !counter || rows < 1 || stride < 1 || slot < 0 || slot >= stride ||
+      (n > 0 && (!y || !x)))
+    return FPM_ERR_ARG;
+  if (n == 0) return FPM_OK;
+  hipLaunchKernelGGL(k_axpy_table<T>, dim3(grid_elems(n)), dim3(256), 0, st, static_cast<T*>(y),
+                     static_cast<const T*>(x), n, table, counter, rows, stride, slot);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
 static int combine_impl(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,
                         const double* c, hipStream_t st) {
   if (n < 0 || !c || (n > 0 && (!dx1 || !S || !P || !F))) return FPM_ERR_ARG;
@@ -108,6 +140,20 @@ int fpm_axpy_f32(void* y, const void* x, int64_t n, double a, fpm_stream_t s) {
 }
 int fpm_axpy_f64(void* y, const void* x, int64_t n, double a, fpm_stream_t s) {
   return axpy_impl<double>(y, x, n, a, (hipStream_t)s);
+}
+int fpm_axpy_table_f32(void* y, const void* x, int64_t n, const double* table,
+                       const int32_t* counter, int rows, int stride, int slot, fpm_stream_t s) {
+  return axpy_table_impl<float>(y, x, n, table, counter, rows, stride, slot, (hipStream_t)s);
+}
+int fpm_axpy_table_f64(void* y, const void* x, int64_t n, const double* table,
+                       const int32_t* counter, int rows, int stride, int slot, fpm_stream_t s) {
+  return axpy_table_impl<double>(y, x, n, table, counter, rows, stride, slot, (hipStream_t)s);
+}
+int fpm_counter_add(int32_t* counter, int32_t v, fpm_stream_t s) {
+  if (!counter) return FPM_ERR_ARG;
+  hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, (hipStream_t)s, counter, v);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
 }
 int fpm_lpt_combine_f32(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,
                         const double* c, fpm_stream_t s) {

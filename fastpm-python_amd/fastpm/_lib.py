@@ -74,6 +74,7 @@ _TYPED = {
     "fpm_kspace_accumulate": (_int, [_vp, _vp, _PK, _int, _int, _PM, _i64, _vp]),
     "fpm_hermitian_scale": (_int, [_vp, _dbl, _dbl, _PM, _i64, _vp]),
     "fpm_lpt2_source_vjp": (_int, [_vp, _vp, _vp, _i64, _vp]),
+    "fpm_axpy_table": (_int, [_vp, _vp, _i64, _vp, _vp, _int, _int, _int, _vp]),
 }
 _PLAIN = {
     "fpm_version": (ctypes.c_char_p, []),
@@ -87,6 +88,7 @@ _PLAIN = {
     "fpm_bin_particles_f32": (_int, [_PP, _PM, _int, _vp, _i64, _vp]),
     "fpm_paint_tiled_f32": (_int, [_PP, _PM, _int, _dbl, _vp, _vp, _i64, _vp]),
     "fpm_readout_tiled_f32": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "fpm_counter_add": (_int, [_vp, ctypes.c_int32, _vp]),
 }
 
 

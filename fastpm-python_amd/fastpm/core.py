@@ -283,6 +283,13 @@ class FastPMStep:
         transform, three fused multiplier + inverse transforms, and one fused
         four-component readout."""
         assert ac == state.a["S"]
+        delta_k = self.force_kernels(state)
+        state.a["F"] = af
+        return dict(delta_k=delta_k)
+
+    def force_kernels(self, state):
+        """Enqueues the force computation (no host-side bookkeeping, so it can
+        be captured in a graph); returns the delta_k field."""
         ws = self.workspace()
         layout, X1, rho = self.prepare_force(state, smoothing=None)
         with self._mark("r2c"):
@@ -303,8 +310,7 @@ class FastPMStep:
                     c2r_many(delta_k, [ops[d]], outs=ws["force"][d:d + 1], work=ws["work"][d:d + 1])
                 with self._mark("readout"):
                     self.pm.readout_many([ws["force"][d]], X1, [F[:, d]])
-        state.a["F"] = af
-        return dict(delta_k=delta_k)
+        return delta_k
 
 
 class _NullRegion:

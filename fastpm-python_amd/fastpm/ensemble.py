@@ -1,0 +1,139 @@
+"""Many independent boxes advanced together, one captured HIP graph per step.
+
+Each box has its own particle mesh, force mesh (Nmesh * B), cosmology
+(sigma8 may differ) and white-noise seed; all share the stage list.  The
+kick/drift coefficients of every step are precomputed in FP64 into a device
+table per box and read through a device step counter (`fpm_axpy_table`), so
+the whole K, D, F, K sequence of all boxes is captured once as a graph and
+replayed for every step.  Paint, transforms and readout are the same kernels
+the single-box Solver runs (the tiled, deterministic paint on meshes that are
+multiples of 16), so a replayed ensemble reproduces the per-box eager runs
+bit for bit.
+
+The reference has no ensemble driver; this is the batched configuration of
+the benchmark (64 boxes of {32, 48, 64, 96, 128}^3 particles, 20 steps).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .core import Solver, leapfrog
+from .pm import ParticleMesh, _stream
+from .cosmology import LinearPower
+
+
+class Box:
+    def __init__(self, nc, box, cosmo, seed, dtype="f4", B=2, resampler="cic", compensate=True):
+        self.nc = int(nc)
+        self.seed = int(seed)
+        self.cosmo = cosmo
+        self.pm = ParticleMesh(Nmesh=[nc] * 3, BoxSize=box, dtype=dtype, resampler=resampler)
+        self.solver = Solver(self.pm, cosmo, B=B, compensate=compensate)
+        self.step = self.solver.nbodystep
+        self.state = None
+        self.table = None
+
+
+class Ensemble:
+    def __init__(self, specs, stages, dtype="f4", B=2, resampler="cic", compensate=True):
+        """specs: iterable of dict(nc=, box=, cosmo=, seed=)."""
+        self.stages = np.asarray(stages, dtype="f8")
+        self.boxes = [Box(s["nc"], s["box"], s["cosmo"], s["seed"], dtype, B, resampler,
+                          compensate) for s in specs]
+        self.nsteps = len(self.stages) - 1
+        dev = self.boxes[0].pm.device
+        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.graph = None
+
+    @property
+    def npart(self):
+        return sum(b.nc ** 3 for b in self.boxes)
+
+    def initialize(self, Pk_cache=None):
+        """White noise, linear field, 2LPT at stages[0] and the first force of
+        every box; builds the per-box coefficient tables."""
+        Pk_cache = {} if Pk_cache is None else Pk_cache
+        a0 = self.stages[0]
+        for b in self.boxes:
+            key = (b.cosmo.sigma8, b.cosmo.Om0, b.cosmo.h, b.cosmo.n_s)
+            if key not in Pk_cache:
+                Pk_cache[key] = LinearPower(b.cosmo)
+            dlin = b.solver.linear(b.solver.whitenoise(b.seed), Pk_cache[key])
+            b.state = b.solver.lpt(dlin, b.pm.generate_uniform_particle_grid(), a=a0)
+            b.step.run("F", a0, a0, a0, b.state, None)
+            pt = b.step._pt
+            rows = []
+            for i in range(self.nsteps):
+                s0, s1 = self.stages[i], self.stages[i + 1]
+                ah = (s0 * s1) ** 0.5
+                rows.append([pt.kick_factor(s0, s0, ah), pt.drift_factor(s0, ah, s1),
+                             pt.kick_factor(ah, s1, s1)])
+            b.table = torch.tensor(np.array(rows, dtype="f8"), device=self.counter.device)
+        self.counter.zero_()
+
+    def _enqueue_step(self):
+        """K, D, F, K of every box for the step held in the device counter."""
+        st = _stream()
+        cnt = self.counter.data_ptr()
+        for b in self.boxes:
+            s = b.state
+            dt = b.pm.dtype
+            n = s.P.numel()
+            _lib.call("fpm_axpy_table", dt, s.P.data_ptr(), s.F.data_ptr(), n,
+                      b.table.data_ptr(), cnt, self.nsteps, 3, 0, st)
+            _lib.call("fpm_axpy_table", dt, s.S.data_ptr(), s.P.data_ptr(), n,
+                      b.table.data_ptr(), cnt, self.nsteps, 3, 1, st)
+            b.step.force_kernels(s)
+            _lib.call("fpm_axpy_table", dt, s.P.data_ptr(), s.F.data_ptr(), n,
+                      b.table.data_ptr(), cnt, self.nsteps, 3, 2, st)
+        _lib.call_plain("fpm_counter_add", cnt, 1, st)
+
+    def step_eager(self):
+        self._enqueue_step()
+
+    def capture(self):
+        """Captures one step of all boxes as a graph (run one eager step first
+        so every buffer exists)."""
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._enqueue_step()
+        self.graph = g
+        return g
+
+    def run(self, use_graph=True):
+        """Runs all steps: the first eagerly, the rest by graph replay."""
+        done = 0
+        if self.nsteps > 0:
+            self.step_eager()
+            done = 1
+        if use_graph and self.nsteps > 1:
+            torch.cuda.synchronize()
+            if self.graph is None:
+                self.capture()
+            for _ in range(self.nsteps - done):
+                self.graph.replay()
+        else:
+            for _ in range(self.nsteps - done):
+                self.step_eager()
+        self._finish_tags()
+
+    def _finish_tags(self):
+        a = self.stages[-1]
+        for b in self.boxes:
+            b.state.a = dict(S=a, P=a, F=a)
+
+
+def config_specs(n=64, seed=0, grids=(32, 48, 64, 96, 128), box_range=(100.0, 800.0),
+                 sigma8_range=(0.7, 0.9), base=None):
+    """Box specifications of the batched benchmark configuration: particle
+    grids drawn from `grids`, sides and sigma8 uniform in their ranges,
+    white-noise seeds 0..n-1."""
+    from .cosmology import Cosmology
+
+    base = Cosmology(h=0.7, Om0=0.3, n_s=0.96, sigma8=0.8) if base is None else base
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        specs.append(dict(nc=int(rng.choice(grids)), box=float(rng.uniform(*box_range)),
+                          cosmo=base.clone(sigma8=float(rng.uniform(*sigma8_range))), seed=i))
+    return specs

@@ -199,6 +199,20 @@ int fpm_lpt2_source_f64(const void* const* phi, void* out, int64_t ncells, fpm_s
 int fpm_axpy_f32(void* y, const void* x, int64_t n, double a, fpm_stream_t stream);
 int fpm_axpy_f64(void* y, const void* x, int64_t n, double a, fpm_stream_t stream);
 
+/* y += table[counter[0] * stride + slot] * x, with the FP64 coefficient table
+ * ([rows][stride]) and the step counter in device memory, so a captured graph
+ * can replay the kick/drift updates of every step (fastpm/core.py:168-180
+ * with the coefficients precomputed per step); a counter outside [0, rows)
+ * leaves y unchanged.  fpm_counter_add adds v to counter[0] (a kernel, safe
+ * inside graph capture). */
+int fpm_axpy_table_f32(void* y, const void* x, int64_t n, const double* table,
+                       const int32_t* counter, int rows, int stride, int slot,
+                       fpm_stream_t stream);
+int fpm_axpy_table_f64(void* y, const void* x, int64_t n, const double* table,
+                       const int32_t* counter, int rows, int stride, int slot,
+                       fpm_stream_t stream);
+int fpm_counter_add(int32_t* counter, int32_t v, fpm_stream_t stream);
+
 /* S = c0 dx1 + c1 dx2; P = c2 dx1 + c3 dx2; F = c4 dx1 + c5 dx2 over n
  * elements (dx2 may be NULL for first order).  fastpm/core.py:128-141. */
 int fpm_lpt_combine_f32(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,

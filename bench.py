@@ -39,12 +39,16 @@ def parse():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-fp64", action="store_true", help="skip the float64 plain run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-box batched ensemble (graph replay) measurement")
+    ap.add_argument("--large", action="store_true",
+                    help="also run the large box: 512^3 particles on 1024^3, TSC, FP32 vs FP64")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for HBM counters)")
     return ap.parse_args()
 
 
-def run_sim(args, dtype, fused, timed):
+def run_sim(args, dtype, fused, timed, resampler="cic", a0=0.1):
     """Runs the whole simulation; times `timed` steps after `args.warmup`."""
     import numpy as np
     import torch
@@ -54,17 +58,17 @@ def run_sim(args, dtype, fused, timed):
 
     cosmo = Cosmology(Om0=0.3, h=0.7, n_s=0.96, sigma8=0.8, Ob0=0.05, Tcmb0=2.7255)
     Pk = LinearPower(cosmo)
-    pm = ParticleMesh(Nmesh=[args.nc] * 3, BoxSize=args.box, dtype=dtype, resampler="cic")
+    pm = ParticleMesh(Nmesh=[args.nc] * 3, BoxSize=args.box, dtype=dtype, resampler=resampler)
     solver = Solver(pm, cosmo, B=args.B, compensate=True, fused=fused)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     dlin = solver.linear(solver.whitenoise(args.seed), Pk)
     Q = pm.generate_uniform_particle_grid(shift=0.0)
-    state = solver.lpt(dlin, Q, a=0.1, order=2)
+    state = solver.lpt(dlin, Q, a=a0, order=2)
     del dlin
     torch.cuda.synchronize()
     t_ic = time.perf_counter() - t0
-    stages = np.linspace(0.1, 1.0, args.nsteps + 1)
+    stages = np.linspace(a0, 1.0, args.nsteps + 1)
     actions = list(leapfrog(stages))
     step = solver.nbodystep
     step.run(*actions[0], state, None)  # initial force
@@ -100,8 +104,8 @@ def gates(r32, r64, args):
 
     s64, s32 = r64["state"], r32["state"]
     fpm64 = r64["solver"].fpm
-    m64 = fpm64.paint(s64.particles, mass=1.0)
-    m32 = r32["solver"].fpm.paint(s32.particles, mass=1.0)
+    m64 = fpm64.paint(s64.particles, mass=1.0, resampler="cic")
+    m32 = r32["solver"].fpm.paint(s32.particles, mass=1.0, resampler="cic")
     m32d = RealField(fpm64, m32.value.double().contiguous())
     del m32
     c64, c32 = m64.r2c(), m32d.r2c()
@@ -114,6 +118,64 @@ def gates(r32, r64, args):
     return dict(pk_rel_err_max=float(np.max(np.abs(p32[sel] / p64[sel] - 1))),
                 rk_min=float(np.min(r[sel])), max_disp_err_mpch=float(dx.abs().max()),
                 max_disp_err_cells=float(dx.abs().max()) * r64["nmesh"] / args.box)
+
+
+def run_ensemble():
+    """64 boxes of {32,48,64,96,128}^3 particles on twice-finer meshes, sides
+    U[100, 800] Mpc/h, sigma8 U[0.7, 0.9], seeds 0..63, 2LPT at a = 0.1 and
+    20 steps in FP32; each step of all boxes is one captured graph replay."""
+    import numpy as np
+    import torch
+    from fastpm.ensemble import Ensemble, config_specs
+
+    stages = np.linspace(0.1, 1.0, 21)
+    ens = Ensemble(config_specs(64), stages)
+    t0 = time.perf_counter()
+    ens.initialize()
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    ens.step_eager()
+    torch.cuda.synchronize()
+    ens.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(ens.nsteps - 1):
+        ens.graph.replay()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    n = ens.nsteps - 1
+    out = dict(workload="64 boxes, particle grids {32,48,64,96,128}^3, meshes 2x, 20 steps, "
+                        "FP32, graph replay per step",
+               particles=int(ens.npart),
+               cells=int(sum(int(b.solver.fpm.Nmesh.prod()) for b in ens.boxes)),
+               value=ens.npart * n / el, unit="particle-steps/s", ms_per_step=el / n * 1e3,
+               timed_steps=n, init_seconds=t_init)
+    del ens
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_large(args):
+    """512^3 particles on a 1024^3 mesh, L = 2000 Mpc/h, TSC with window
+    compensation, 2LPT at a = 0.05, 40 steps to a = 1: FP32 against FP64
+    (plain) P(k) and displacements."""
+    import copy
+
+    la = copy.copy(args)
+    la.nc, la.box, la.B, la.nsteps, la.warmup, la.steps = 512, 2000.0, 2, 40, 1, 4
+    res = {}
+    for dt, fused in (("f4", True), ("f8", False)):
+        r = run_sim(la, dt, fused, la.steps, resampler="tsc", a0=0.05)
+        res[dt] = r
+    g = gates(res["f4"], res["f8"], la)
+    out = dict(workload="L=2000 Mpc/h, 512^3 particles, 1024^3 mesh, TSC with compensation, "
+                        "2LPT at a=0.05, 40 steps",
+               fp32_ms_per_step=res["f4"]["elapsed"] / la.steps * 1e3,
+               fp64_plain_ms_per_step=res["f8"]["elapsed"] / la.steps * 1e3,
+               fp32_kernels_ms={k: {"calls": c, "mean_ms": m}
+                                for k, (c, m) in res["f4"]["kern"].items()},
+               accuracy=g)
+    return out
 
 
 def cpu_baseline(args, r64):
@@ -209,6 +271,14 @@ def main():
         if not args.no_cpu:
             print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
             out["cpu_baseline"] = cpu_baseline(args, r64)
+        del r64
+        torch.cuda.empty_cache()
+    if not args.no_ensemble:
+        print("bench: batched ensemble", file=sys.stderr, flush=True)
+        out["ensemble"] = run_ensemble()
+    if args.large:
+        print("bench: large box", file=sys.stderr, flush=True)
+        out["large_box"] = run_large(args)
     print(json.dumps(out))
 
 

@@ -35,8 +35,12 @@ class Box:
 
 
 class Ensemble:
-    def __init__(self, specs, stages, dtype="f4", B=2, resampler="cic", compensate=True):
-        """specs: iterable of dict(nc=, box=, cosmo=, seed=)."""
+    def __init__(self, specs, stages, dtype="f4", B=2, resampler="cic", compensate=True,
+                 nstreams=4):
+        """specs: iterable of dict(nc=, box=, cosmo=, seed=).  Boxes are spread
+        over `nstreams` streams (largest first, each to the least loaded
+        stream) so small boxes run concurrently; inside a captured graph the
+        streams become parallel branches."""
         self.stages = np.asarray(stages, dtype="f8")
         self.boxes = [Box(s["nc"], s["box"], s["cosmo"], s["seed"], dtype, B, resampler,
                           compensate) for s in specs]
@@ -44,6 +48,14 @@ class Ensemble:
         dev = self.boxes[0].pm.device
         self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
         self.graph = None
+        self.nstreams = max(1, int(nstreams))
+        self.streams = [torch.cuda.Stream(device=dev) for _ in range(self.nstreams)]
+        load = [0] * self.nstreams
+        self.lanes = [[] for _ in range(self.nstreams)]
+        for b in sorted(self.boxes, key=lambda b: -b.nc ** 3):
+            i = int(np.argmin(load))
+            self.lanes[i].append(b)
+            load[i] += b.solver.fpm.Nmesh.prod()
 
     @property
     def npart(self):
@@ -71,22 +83,32 @@ class Ensemble:
             b.table = torch.tensor(np.array(rows, dtype="f8"), device=self.counter.device)
         self.counter.zero_()
 
-    def _enqueue_step(self):
-        """K, D, F, K of every box for the step held in the device counter."""
+    def _enqueue_box(self, b):
         st = _stream()
         cnt = self.counter.data_ptr()
-        for b in self.boxes:
-            s = b.state
-            dt = b.pm.dtype
-            n = s.P.numel()
-            _lib.call("fpm_axpy_table", dt, s.P.data_ptr(), s.F.data_ptr(), n,
-                      b.table.data_ptr(), cnt, self.nsteps, 3, 0, st)
-            _lib.call("fpm_axpy_table", dt, s.S.data_ptr(), s.P.data_ptr(), n,
-                      b.table.data_ptr(), cnt, self.nsteps, 3, 1, st)
-            b.step.force_kernels(s)
-            _lib.call("fpm_axpy_table", dt, s.P.data_ptr(), s.F.data_ptr(), n,
-                      b.table.data_ptr(), cnt, self.nsteps, 3, 2, st)
-        _lib.call_plain("fpm_counter_add", cnt, 1, st)
+        s = b.state
+        dt = b.pm.dtype
+        n = s.P.numel()
+        _lib.call("fpm_axpy_table", dt, s.P.data_ptr(), s.F.data_ptr(), n,
+                  b.table.data_ptr(), cnt, self.nsteps, 3, 0, st)
+        _lib.call("fpm_axpy_table", dt, s.S.data_ptr(), s.P.data_ptr(), n,
+                  b.table.data_ptr(), cnt, self.nsteps, 3, 1, st)
+        b.step.force_kernels(s)
+        _lib.call("fpm_axpy_table", dt, s.P.data_ptr(), s.F.data_ptr(), n,
+                  b.table.data_ptr(), cnt, self.nsteps, 3, 2, st)
+
+    def _enqueue_step(self):
+        """K, D, F, K of every box for the step held in the device counter:
+        fork to the box streams, join, then advance the counter."""
+        main = torch.cuda.current_stream()
+        for lane, stream in zip(self.lanes, self.streams):
+            stream.wait_stream(main)
+            with torch.cuda.stream(stream):
+                for b in lane:
+                    self._enqueue_box(b)
+        for stream in self.streams:
+            main.wait_stream(stream)
+        _lib.call_plain("fpm_counter_add", self.counter.data_ptr(), 1, _stream())
 
     def step_eager(self):
         self._enqueue_step()

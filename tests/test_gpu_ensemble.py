@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 def _specs():
     from fastpm.ensemble import config_specs
 
-    return config_specs(n=5, seed=3, grids=(8, 12, 16), box_range=(50.0, 120.0))
+    # force meshes 16, 32, 48: all on the tiled (deterministic) paint
+    return config_specs(n=5, seed=3, grids=(8, 16, 24), box_range=(50.0, 120.0))
 
 
 def test_ensemble_graph_matches_eager(gpu):

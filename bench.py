@@ -120,6 +120,18 @@ def gates(r32, r64, args):
                 max_disp_err_cells=float(dx.abs().max()) * r64["nmesh"] / args.box)
 
 
+def _profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` at this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def run_ensemble():
     """64 boxes of {32,48,64,96,128}^3 particles on twice-finer meshes, sides
     U[100, 800] Mpc/h, sigma8 U[0.7, 0.9], seeds 0..63, 2LPT at a = 0.1 and
@@ -249,7 +261,7 @@ def main():
     out["roofline"] = {"kernel": "k_readout_tiles<float,2,4> (fused RHO+F CIC readout)",
                        "bound": "hbm",
                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                       "frac": ach / HBM_PEAK_GBS, "traffic": _profiled_traffic("k_readout_tiles<float, 2, 4>"),
                        "algorithmic_bytes": ro_bytes}
     step_bytes = 36 * Nc + 32 * (r32["nmesh"] ** 2 * (r32["nmesh"] // 2 + 1)) + 148 * Np
     out["step_roofline"] = {"algorithmic_bytes": step_bytes,

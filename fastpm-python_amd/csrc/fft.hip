@@ -118,12 +118,20 @@ template <bool INV, typename V> struct Dft<3, INV, V> {
 };
 
 // ---------------------------------------------------------------- one stage
-// NL lines of length N in LDS.  Element e of line l lives at l*LS + e*ES.
-// LFAST selects which index varies fastest across threads (line index for the
-// column tiles, whose lines are interleaved; element index for row tiles).
-// TS is the stride into the twiddle table (2 when the table is for length 2N).
+// NL lines of length N in LDS.  Element e of line l lives at
+// l*LS + pad(e)*ES with pad(e) = e + (e >> PAD) when PAD > 0: one spare slot
+// every 2^PAD elements spreads the stride-R stores of the early stages over
+// the LDS banks.  LFAST selects which index varies fastest across threads
+// (line index for the column tiles, whose lines are interleaved; element
+// index for row tiles).  TS is the stride into the twiddle table (2 when the
+// table is for length 2N).
+template <int PAD> __device__ __forceinline__ constexpr int padx(int e) {
+  if constexpr (PAD > 0) return e + (e >> PAD);
+  else return e;
+}
+
 template <typename T, int N, int R, int NS, int NL, int NT, bool LFAST, int ES, int LS, int TS,
-          bool INV>
+          bool INV, int PAD = 0>
 __device__ __forceinline__ void stage(C<T>* s, const C<T>* tw) {
   using V = C<T>;
   constexpr int NB = NL * (N / R);
@@ -136,7 +144,7 @@ __device__ __forceinline__ void stage(C<T>* s, const C<T>* tw) {
       int l, j;
       if (LFAST) { l = b % NL; j = b / NL; } else { j = b % (N / R); l = b / (N / R); }
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = s[l * LS + (j + r * (N / R)) * ES];
+      for (int r = 0; r < R; ++r) v[q][r] = s[l * LS + padx<PAD>(j + r * (N / R)) * ES];
       if (NS > 1) {
         const int k = j % NS;
 #pragma unroll
@@ -158,20 +166,21 @@ __device__ __forceinline__ void stage(C<T>* s, const C<T>* tw) {
       if (LFAST) { l = b % NL; j = b / NL; } else { j = b % (N / R); l = b / (N / R); }
       const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[l * LS + (base + r * NS) * ES] = v[q][r];
+      for (int r = 0; r < R; ++r) s[l * LS + padx<PAD>(base + r * NS) * ES] = v[q][r];
     }
   }
   __syncthreads();
 }
 
-template <typename T, int N, int S, int NS, int NL, int NT, bool LFAST, int ES, int LS, int TS,
-          bool INV>
-struct Stages {
+// Stages S..SEND-1 of the plan (LDS to LDS).
+template <typename T, int N, int S, int SEND, int NS, int NL, int NT, bool LFAST, int ES, int LS,
+          int TS, bool INV, int PAD = 0>
+struct StagesTo {
   __device__ __forceinline__ static void run(C<T>* s, const C<T>* tw) {
-    if constexpr (S < Plan<N>::ns) {
+    if constexpr (S < SEND) {
       constexpr int R = Plan<N>::r[S];
-      stage<T, N, R, NS, NL, NT, LFAST, ES, LS, TS, INV>(s, tw);
-      Stages<T, N, S + 1, NS * R, NL, NT, LFAST, ES, LS, TS, INV>::run(s, tw);
+      stage<T, N, R, NS, NL, NT, LFAST, ES, LS, TS, INV, PAD>(s, tw);
+      StagesTo<T, N, S + 1, SEND, NS * R, NL, NT, LFAST, ES, LS, TS, INV, PAD>::run(s, tw);
     }
   }
 };
@@ -186,7 +195,7 @@ constexpr int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi 
 #define FPM_EPT_F32 8  // complex elements per thread in a column tile (float)
 #endif
 #ifndef FPM_EPT_F64
-#define FPM_EPT_F64 4  // (double)
+#define FPM_EPT_F64 8  // (double)
 #endif
 // Tile shapes chosen by a sweep of row-segment width (64/128/256 bytes) and
 // elements per thread (4/8/16) at 512^3 on gfx950.
@@ -196,17 +205,20 @@ template <typename T> constexpr int threads_for(int E) {
   return clampi(round64(E / (sizeof(T) == 4 ? FPM_EPT_F32 : FPM_EPT_F64)), 64, 1024);
 }
 
+constexpr int pow2floor(int x) {
+  int p = 1;
+  while (2 * p <= x) p *= 2;
+  return p;
+}
+
 // Column tile width (z columns per tile), capped so the tile stays within
-// 128 KiB of LDS.
+// 96 KiB of LDS (the rest holds twiddles and multiplier tables).
 template <typename T, int N> struct ColW {
   static constexpr int a = FPM_COL_BYTES / (int)sizeof(C<T>);
-  static constexpr int b = (128 * 1024 / (int)sizeof(C<T>)) / N;
+  static constexpr int b = pow2floor((96 * 1024 / (int)sizeof(C<T>)) / N);
   static constexpr int BZ = a < b ? a : b;
 };
 
-#ifndef FPM_FFT_PAIR
-#define FPM_FFT_PAIR 0  // transform two outputs per LDS sweep in multi-output passes
-#endif
 #ifndef FPM_Z_ELEMS
 #define FPM_Z_ELEMS 2048  // complex elements per z-pass workgroup
 #endif
@@ -304,6 +316,39 @@ __device__ __forceinline__ double comp_axis_factor(const KopPack& K, int c, int 
   return f;
 }
 
+// The same axis factors in T precision, read off the axis' twiddle table
+// tw[m] = exp(-2 pi i m / n) (m < n): sin w and sin 2w of w = 2 pi nn / n are
+// table entries, and sin(w / 2) follows from the half-angle formulas
+// sin(t/2) = sin t / sqrt(2 (1 + cos t)) for cos t >= 0, sqrt((1 - cos t)/2)
+// otherwise (no cancellation on either branch).
+template <typename T>
+__device__ __forceinline__ void axis_factors_tw(int i, int n, T two_pi_l, T inv6h, bool half,
+                                                const C<T>* tw, T& kk, T& g0, T& g1, T& sn) {
+  const int nn = half ? i : (i < (n + 1) / 2 ? i : i - n);
+  const bool nyq = (n % 2 == 0) && (i == n / 2);
+  const T k = two_pi_l * (T)nn;
+  kk = k * k;
+  g0 = nyq ? (T)0 : k;
+  const T s1 = -tw[i % n].y, s2 = -tw[(2 * i) % n].y;
+  g1 = nyq ? (T)0 : ((T)8 * s1 - s2) * inv6h;
+  const int a = nn < 0 ? -nn : nn;
+  const C<T> t = tw[a % n];
+  const T c = t.x, s = -t.y;
+  const T sh = c >= (T)0 ? s / sqrt((T)2 * ((T)1 + c)) : sqrt(((T)1 - c) * (T)0.5);
+  sn = a == 0 ? (T)1
+              : sh * ((T)n / ((T)3.14159265358979323846264338327950 * (T)a));
+}
+
+// Component c's separable factor along `axis` from that axis' factors.
+template <typename T>
+__device__ __forceinline__ T comp_factor_t(const KopPack& K, int c, int axis, T g0, T g1, T sn) {
+  T f = (T)1;
+  for (int g = 0; g < K.ngrad[c]; ++g)
+    if (K.gdir[c][g] == axis) f *= K.grad_order[c] == 0 ? g0 : g1;
+  for (int q = 0; q < K.deconv[c]; ++q) f /= sn;
+  return f;
+}
+
 template <typename T> __device__ __forceinline__ T fast_rcp(T x);
 template <> __device__ __forceinline__ float fast_rcp<float>(float x) {
   return __builtin_amdgcn_rcpf(x);
@@ -340,21 +385,30 @@ struct ColArgs {
   int eaxis, oaxis;
 };
 
-// NCP outputs are transformed together: their tiles sit side by side in LDS
-// ([N][NCP * BZ]) and every Stockham stage runs over all NCP * BZ lines, so a
-// multi-output pass costs one set of stage barriers per NCP outputs.
-template <typename T, int N, int BZ, int NT, bool INV, bool KOP, int NCP>
+
+// The first Stockham stage takes its radix-R0 inputs straight from the global
+// loads (element j + r N/R0 of column l for butterfly (l, j)) and the last
+// stage stores its outputs (element j + r N/RL) straight to global memory, so
+// an n-point column costs ns - 1 LDS exchanges.  With several outputs (MC)
+// the loaded tile stays in registers and each output re-applies its
+// multiplier to a copy.  Measured at 512^3 on
+// gfx950 this beats one workgroup per (tile, output), whose repeated tile
+// reads mostly miss L2.
+template <typename T, int N, int BZ, int NT, bool INV, bool KOP, bool MC>
 __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   using V = C<T>;
   constexpr int E = N * BZ;
-  constexpr int W = NCP * BZ;  // LDS lines
-  constexpr int PER = (E + NT - 1) / NT;
-  constexpr int MAXC = 6;
-  __shared__ V tile[NCP * E];
+  constexpr int MAXC = MC ? 6 : 1;
+  constexpr int NSTG = Plan<N>::ns;
+  constexpr int R0 = Plan<N>::r[0];
+  constexpr int RL = Plan<N>::r[NSTG - 1];
+  constexpr int NB0 = BZ * (N / R0), PER0 = (NB0 + NT - 1) / NT;
+  constexpr int NBL = BZ * (N / RL), PERL = (NBL + NT - 1) / NT;
+  __shared__ V tile[E];
   __shared__ V tw[N];
-  // k-space multiplier tables: per-component separable factors along the
-  // e axis (fe), the z columns (fz) and the outer index (fo, times the
-  // component's factor), and k^2 along e (ke) and z (kz)
+  // k-space multiplier tables: per-output separable factors along the e axis
+  // (fe) and the z columns (fz) and the outer-axis factor times the output's
+  // factor (fo); k^2 along e (ke) and z (kz)
   __shared__ T fe[KOP ? MAXC * N : 1];
   __shared__ T ke[KOP ? N : 1];
   __shared__ T fz[KOP ? MAXC * BZ : 1];
@@ -369,94 +423,133 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   T ko = (T)0;
   if constexpr (KOP) {
     const KopPack& K = a.kop;
-    for (int i = threadIdx.x; i < N * a.ncomp; i += NT) {
-      const int c = i / N, e = i % N;
-      double kk;
-      fe[c * N + e] = (T)comp_axis_factor(K, c, a.eaxis, e, false, kk);
-      if (c == 0) ke[e] = (T)kk;
+    const int nc = MC ? a.ncomp : 1;
+    // the transform axis: factors once per mode from the twiddle table, then
+    // each output's product
+    const T two_pi_l = (T)(6.283185307179586476925286766559 / K.box[a.eaxis]);
+    const T inv6h = (T)((double)N / (6.0 * K.box[a.eaxis]));
+    for (int e = threadIdx.x; e < N; e += NT) {
+      T kk, g0, g1, sn;
+      axis_factors_tw<T>(e, N, two_pi_l, inv6h, false, a.tw, kk, g0, g1, sn);
+      ke[e] = kk;
+      for (int c = 0; c < nc; ++c) fe[c * N + e] = comp_factor_t<T>(K, c, a.eaxis, g0, g1, sn);
     }
-    for (int i = threadIdx.x; i < BZ * a.ncomp; i += NT) {
+    for (int i = threadIdx.x; i < BZ * nc; i += NT) {
       const int c = i / BZ, l = i % BZ;
       double kk;
       fz[c * BZ + l] = (T)comp_axis_factor(K, c, 2, z0 + l, true, kk);
       if (c == 0) kz[l] = (T)kk;
     }
-    if (threadIdx.x < a.ncomp) {
+    if (threadIdx.x < nc) {
       double kk;
       const int c = threadIdx.x;
       fo[c] = (T)(K.factor[c] * comp_axis_factor(K, c, a.oaxis, outer, false, kk));
     }
-    double kko, g0, g1, sn;
-    axis_factors(outer, K.nm[a.oaxis], K.box[a.oaxis], false, kko, g0, g1, sn);
-    ko = (T)kko;
+    const int no = K.nm[a.oaxis];
+    const int nno = outer < (no + 1) / 2 ? outer : outer - no;
+    const double kao = 6.283185307179586476925286766559 * (double)nno / K.box[a.oaxis];
+    ko = (T)(kao * kao);
   }
 
+  // butterfly b of the first stage: column l = b % BZ, j = b / BZ
   const V* src = a.in + (int64_t)outer * a.ostride + z0;
-  V reg[PER];
+  V v0[PER0][R0];
 #pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int idx = threadIdx.x + q * NT;
-    reg[q] = cmk<V>((T)0, (T)0);
-    if ((E % NT == 0) || idx < E) {
-      const int l = idx % BZ, e = idx / BZ;
-      if (z0 + l < a.ncols) reg[q] = src[(int64_t)e * a.estride + l];
-    }
+  for (int q = 0; q < PER0; ++q) {
+    const int b = threadIdx.x + q * NT;
+    const int l = b % BZ, j = b / BZ;
+    const bool ok = ((NB0 % NT == 0) || b < NB0) && z0 + l < a.ncols;
+#pragma unroll
+    for (int r = 0; r < R0; ++r)
+      v0[q][r] = ok ? src[(int64_t)(j + r * (N / R0)) * a.estride + l] : cmk<V>((T)0, (T)0);
   }
   if constexpr (KOP) __syncthreads();  // multiplier tables ready
 
-  for (int c0 = 0; c0 < a.ncomp; c0 += NCP) {
+  // output c from the first-stage inputs v (overwritten)
+  auto one_output = [&](int c, V (&v)[PER0][R0]) {
+    V* dst = a.out[c] + (int64_t)outer * a.ostride + z0;
+    bool lap = false;
+    T r2 = (T)0;
+    int ng = 0;
+    if constexpr (KOP) {
+      lap = a.kop.laplace[c] != 0;
+      r2 = (T)a.kop.r2[c];
+      ng = a.kop.ngrad[c];
+    }
 #pragma unroll
-    for (int j = 0; j < NCP; ++j) {
-      const int c = c0 + j;
-      bool lap = false;
-      T r2 = (T)0;
-      int ng = 0;
-      if constexpr (KOP) {
-        if (c < a.ncomp) {
-          lap = a.kop.laplace[c] != 0;
-          r2 = (T)a.kop.r2[c];
-          ng = a.kop.ngrad[c];
-        }
-      }
+    for (int q = 0; q < PER0; ++q) {
+      const int b = threadIdx.x + q * NT;
+      if ((NB0 % NT == 0) || b < NB0) {
+        const int l = b % BZ, j = b / BZ;
+        if constexpr (KOP) {
+          const T mz = fo[c] * fz[c * BZ + l];
+          const T kzo = ko + kz[l];
 #pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        const int idx = threadIdx.x + q * NT;
-        if ((E % NT == 0) || idx < E) {
-          const int l = idx % BZ, e = idx / BZ;
-          V v = reg[q];
-          if constexpr (KOP) {
-            if (c < a.ncomp) {
-              T m = fo[c] * fe[c * N + e] * fz[c * BZ + l];
-              if (lap || r2 != (T)0) {
-                const T kk = ke[e] + ko + kz[l];
-                if (lap) m = kk == (T)0 ? (T)0 : m * fast_rcp<T>(kk);
-                if (r2 != (T)0) m *= exp(-kk * r2);
-              }
-              v = apply_mult(v, m, ng);
+          for (int r = 0; r < R0; ++r) {
+            const int e = j + r * (N / R0);
+            T m = mz * fe[c * N + e];
+            if (lap || r2 != (T)0) {
+              const T kk = ke[e] + kzo;
+              if (lap) m = kk == (T)0 ? (T)0 : m * fast_rcp<T>(kk);
+              if (r2 != (T)0) m *= exp(-kk * r2);
             }
+            v[q][r] = apply_mult(v[q][r], m, ng);
           }
-          tile[e * W + j * BZ + l] = v;
+        }
+        Dft<R0, INV, V>::run(v[q]);
+        if constexpr (NSTG == 1) {
+          if (z0 + l < a.ncols) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r)
+              dst[(int64_t)(j * R0 + r) * a.estride + l] = cscale(v[q][r], a.scale);
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < R0; ++r) tile[(j * R0 + r) * BZ + l] = v[q][r];
         }
       }
     }
-    __syncthreads();
-    Stages<T, N, 0, 1, W, NT, true, W, 1, 1, INV>::run(tile, tw);
+    if constexpr (NSTG > 1) {
+      __syncthreads();
+      StagesTo<T, N, 1, NSTG - 1, R0, BZ, NT, true, BZ, 1, 1, INV>::run(tile, tw);
+      // last stage: Ns = N / RL, outputs j + r N/RL straight to global memory
 #pragma unroll
-    for (int j = 0; j < NCP; ++j) {
-      const int c = c0 + j;
-      if (c >= a.ncomp) break;
-      V* dst = a.out[c] + (int64_t)outer * a.ostride + z0;
+      for (int q = 0; q < PERL; ++q) {
+        const int b = threadIdx.x + q * NT;
+        if ((NBL % NT == 0) || b < NBL) {
+          const int l = b % BZ, j = b / BZ;
+          V w8[RL];
 #pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        const int idx = threadIdx.x + q * NT;
-        if ((E % NT == 0) || idx < E) {
-          const int l = idx % BZ, e = idx / BZ;
-          if (z0 + l < a.ncols)
-            dst[(int64_t)e * a.estride + l] = cscale(tile[e * W + j * BZ + l], a.scale);
+          for (int r = 0; r < RL; ++r) w8[r] = tile[(j + r * (N / RL)) * BZ + l];
+#pragma unroll
+          for (int r = 1; r < RL; ++r) {
+            V w = tw[r * j];
+            if (INV) w.y = -w.y;
+            w8[r] = cmul(w8[r], w);
+          }
+          Dft<RL, INV, V>::run(w8);
+          if (z0 + l < a.ncols) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r)
+              dst[(int64_t)(j + r * (N / RL)) * a.estride + l] = cscale(w8[r], a.scale);
+          }
         }
       }
     }
-    __syncthreads();
+  };
+
+  if constexpr (MC) {
+    for (int c = 0; c < a.ncomp; ++c) {
+      if (c > 0) __syncthreads();  // the previous output has read the tile
+      V v[PER0][R0];
+#pragma unroll
+      for (int q = 0; q < PER0; ++q)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[q][r] = v0[q][r];
+      one_output(c, v);
+    }
+  } else {
+    one_output(0, v0);
   }
 }
 
@@ -481,10 +574,27 @@ struct ZArgs {
   double L2;
 };
 
+#ifndef FPM_Z_PAD
+#define FPM_Z_PAD 4  // LDS padding shift of the z-pass lines
+#endif
+
+// Line pitch in LDS of a padded z-pass line of M elements.
+// (odd, so adjacent lines start on different banks).
+template <int M> constexpr int z_pitch() {
+  const int p = padx<FPM_Z_PAD>(M - 1) + 1;
+  return p % 2 ? p : p + 1;
+}
+
+// Forward: the first Stockham stage reads its inputs straight from the real
+// rows; the unpacked half spectrum is stored straight from registers.
 template <typename T, int M, int NL, int NT>
 __global__ __launch_bounds__(NT) void k_fft_z_r2c(ZArgs<T> a) {
   using V = C<T>;
-  constexpr int LP = M + 1;
+  constexpr int PAD = FPM_Z_PAD;
+  constexpr int LP = z_pitch<M>();
+  constexpr int NSTG = Plan<M>::ns;
+  constexpr int R0 = Plan<M>::r[0];
+  constexpr int NB0 = NL * (M / R0), PER0 = (NB0 + NT - 1) / NT;
   constexpr int NP = M / 2 + 1;                 // pairs (k, M-k), k = 0..M/2
   constexpr int PERP = (NL * NP + NT - 1) / NT;
   __shared__ V s[NL * LP];
@@ -492,119 +602,131 @@ __global__ __launch_bounds__(NT) void k_fft_z_r2c(ZArgs<T> a) {
   const int64_t row0 = (int64_t)blockIdx.x * NL;
   for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = a.tw[i];
   const V* in = reinterpret_cast<const V*>(a.rin);
-  for (int idx = threadIdx.x; idx < NL * M; idx += NT) {
-    const int l = idx / M, m = idx % M;
-    const int64_t row = row0 + l;
-    s[l * LP + m] = row < a.nrows ? in[row * M + m] : cmk<V>((T)0, (T)0);
+#pragma unroll
+  for (int q = 0; q < PER0; ++q) {
+    const int b = threadIdx.x + q * NT;
+    if ((NB0 % NT == 0) || b < NB0) {
+      const int j = b % (M / R0), l = b / (M / R0);
+      const int64_t row = row0 + l;
+      V v[R0];
+#pragma unroll
+      for (int r = 0; r < R0; ++r)
+        v[r] = row < a.nrows ? in[row * M + j + r * (M / R0)] : cmk<V>((T)0, (T)0);
+      Dft<R0, false, V>::run(v);
+#pragma unroll
+      for (int r = 0; r < R0; ++r) s[l * LP + padx<PAD>(j * R0 + r)] = v[r];
+    }
   }
   __syncthreads();
-  Stages<T, M, 0, 1, NL, NT, false, 1, LP, 2, false>::run(s, tw);
-  V xa[PERP], xb[PERP];
+  StagesTo<T, M, 1, NSTG, R0, NL, NT, false, 1, LP, 2, false, PAD>::run(s, tw);
 #pragma unroll
   for (int q = 0; q < PERP; ++q) {
     const int idx = threadIdx.x + q * NT;
     if (idx < NL * NP) {
       const int l = idx / NP, k = idx % NP;
-      const V za = s[l * LP + (k % M)];
-      const V zb = s[l * LP + ((M - k) % M)];
+      const V za = s[l * LP + padx<PAD>(k % M)];
+      const V zb = s[l * LP + padx<PAD>((M - k) % M)];
       const V cb = cconj(zb), ca = cconj(za);
-      // X_k
-      {
+      V xa, xb;
+      {  // X_k
         const V sum = cadd(za, cb), dif = csub(za, cb);
         const V t = cmul(tw[k], dif);  // w^k (Z_k - conj Z_{M-k})
-        xa[q] = cmk<V>((T)0.5 * (sum.x + t.y), (T)0.5 * (sum.y - t.x));
+        xa = cmk<V>((T)0.5 * (sum.x + t.y), (T)0.5 * (sum.y - t.x));
       }
-      // X_{M-k}
-      {
+      {  // X_{M-k}
         const V sum = cadd(zb, ca), dif = csub(zb, ca);
         const V t = cmul(tw[M - k], dif);
-        xb[q] = cmk<V>((T)0.5 * (sum.x + t.y), (T)0.5 * (sum.y - t.x));
+        xb = cmk<V>((T)0.5 * (sum.x + t.y), (T)0.5 * (sum.y - t.x));
+      }
+      const int64_t row = row0 + l;
+      if (row < a.nrows) {
+        a.cout[row * a.pitch2 + k] = xa;
+        if (2 * k != M) a.cout[row * a.pitch2 + (M - k)] = xb;
       }
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PERP; ++q) {
-    const int idx = threadIdx.x + q * NT;
-    if (idx < NL * NP) {
-      const int l = idx / NP, k = idx % NP;
-      s[l * LP + k] = xa[q];
-      s[l * LP + (M - k)] = xb[q];
-    }
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < NL * LP; idx += NT) {
-    const int l = idx / LP, k = idx % LP;
-    const int64_t row = row0 + l;
-    if (row < a.nrows) a.cout[row * a.pitch2 + k] = s[l * LP + k];
   }
 }
 
+// Inverse: the packed rows are built straight from the global loads (with the
+// optional z-gradient factor) and the last stage stores the real rows.
 template <typename T, int M, int NL, int NT>
 __global__ __launch_bounds__(NT) void k_fft_z_c2r(ZArgs<T> a) {
   using V = C<T>;
-  constexpr int LP = M + 1;
+  constexpr int PAD = FPM_Z_PAD;
+  constexpr int LP = z_pitch<M>();
+  constexpr int NSTG = Plan<M>::ns;
+  constexpr int RL = Plan<M>::r[NSTG - 1];
+  constexpr int NBL = NL * (M / RL), PERL = (NBL + NT - 1) / NT;
   constexpr int NP = M / 2 + 1;
   constexpr int PERP = (NL * NP + NT - 1) / NT;
   __shared__ V s[NL * LP];
   __shared__ V tw[2 * M];
-  __shared__ T zg[M + 1];
   const int64_t row0 = (int64_t)blockIdx.x * NL;
   for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = a.tw[i];
+  T two_pi_l = (T)0, inv6h = (T)0;
   if (a.zgrad) {
-    for (int k = threadIdx.x; k <= M; k += NT) {
-      const AxisTab<T> az = axis_tab<T>(k, 2 * M, a.L2, true);
-      zg[k] = a.zorder == 0 ? az.g0 : az.g1;
-    }
-    __syncthreads();
+    two_pi_l = (T)(6.283185307179586476925286766559 / a.L2);
+    inv6h = (T)((double)(2 * M) / (6.0 * a.L2));
   }
-  for (int idx = threadIdx.x; idx < NL * LP; idx += NT) {
-    const int l = idx / LP, k = idx % LP;
-    const int64_t row = row0 + l;
-    V v = row < a.nrows ? a.cin[row * a.pitch2 + k] : cmk<V>((T)0, (T)0);
-    if (a.zgrad) v = apply_mult(v, zg[k], 1);
-    s[l * LP + k] = v;
-  }
-  __syncthreads();
-  V za[PERP], zb[PERP];
+  __syncthreads();  // twiddles staged
 #pragma unroll
   for (int q = 0; q < PERP; ++q) {
     const int idx = threadIdx.x + q * NT;
     if (idx < NL * NP) {
       const int l = idx / NP, k = idx % NP;
-      V xa = s[l * LP + k];
-      V xb = s[l * LP + (M - k)];
+      const int64_t row = row0 + l;
+      V xa = cmk<V>((T)0, (T)0), xb = xa;
+      if (row < a.nrows) {
+        xa = a.cin[row * a.pitch2 + k];
+        xb = a.cin[row * a.pitch2 + (M - k)];
+      }
+      if (a.zgrad) {
+        T kk, g0, g1, sn;
+        axis_factors_tw<T>(k, 2 * M, two_pi_l, inv6h, true, a.tw, kk, g0, g1, sn);
+        xa = apply_mult(xa, a.zorder == 0 ? g0 : g1, 1);
+        axis_factors_tw<T>(M - k, 2 * M, two_pi_l, inv6h, true, a.tw, kk, g0, g1, sn);
+        xb = apply_mult(xb, a.zorder == 0 ? g0 : g1, 1);
+      }
       if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
       const V ca = cconj(xa), cb = cconj(xb);
+      V za, zb;
       {
         const V sum = cadd(xa, cb), dif = csub(xa, cb);
         const V t = cmul(cconj(tw[k]), dif);  // w^-k (X_k - conj X_{M-k})
-        za[q] = cmk<V>(sum.x - t.y, sum.y + t.x);
+        za = cmk<V>(sum.x - t.y, sum.y + t.x);
       }
       {
         const V sum = cadd(xb, ca), dif = csub(xb, ca);
         const V t = cmul(cconj(tw[M - k]), dif);
-        zb[q] = cmk<V>(sum.x - t.y, sum.y + t.x);
+        zb = cmk<V>(sum.x - t.y, sum.y + t.x);
+      }
+      s[l * LP + padx<PAD>(k)] = za;
+      if (k != 0) s[l * LP + padx<PAD>(M - k)] = zb;
+    }
+  }
+  __syncthreads();
+  StagesTo<T, M, 0, NSTG - 1, 1, NL, NT, false, 1, LP, 2, true, PAD>::run(s, tw);
+  // last stage: Ns = M / RL, outputs j + r M/RL straight to the real rows
+  V* out = reinterpret_cast<V*>(a.rout);
+#pragma unroll
+  for (int q = 0; q < PERL; ++q) {
+    const int b = threadIdx.x + q * NT;
+    if ((NBL % NT == 0) || b < NBL) {
+      const int j = b % (M / RL), l = b / (M / RL);
+      V v[RL];
+#pragma unroll
+      for (int r = 0; r < RL; ++r) v[r] = s[l * LP + padx<PAD>(j + r * (M / RL))];
+      if constexpr (NSTG > 1) {
+#pragma unroll
+        for (int r = 1; r < RL; ++r) v[r] = cmul(v[r], cconj(tw[2 * r * j]));
+      }
+      Dft<RL, true, V>::run(v);
+      const int64_t row = row0 + l;
+      if (row < a.nrows) {
+#pragma unroll
+        for (int r = 0; r < RL; ++r) out[row * M + j + r * (M / RL)] = v[r];
       }
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PERP; ++q) {
-    const int idx = threadIdx.x + q * NT;
-    if (idx < NL * NP) {
-      const int l = idx / NP, k = idx % NP;
-      s[l * LP + k] = za[q];
-      if (k != 0) s[l * LP + (M - k)] = zb[q];
-    }
-  }
-  __syncthreads();
-  Stages<T, M, 0, 1, NL, NT, false, 1, LP, 2, true>::run(s, tw);
-  V* out = reinterpret_cast<V*>(a.rout);
-  for (int idx = threadIdx.x; idx < NL * M; idx += NT) {
-    const int l = idx / M, m = idx % M;
-    const int64_t row = row0 + l;
-    if (row < a.nrows) out[row * M + m] = s[l * LP + m];
   }
 }
 
@@ -630,25 +752,19 @@ static void launch_col_n(ColArgs<T> a, hipStream_t st) {
   a.ntz = (a.ncols + BZ - 1) / BZ;
   a.nblk = (int64_t)a.ntz * a.nouter;
   if (a.nblk <= 0) return;
-  // two outputs per sweep when enabled, both tiles fit in LDS and the doubled
-  // butterflies per thread stay within the register budget of NT threads
-  // (measured slower at 512^3 on gfx950: 2.98 vs 2.30 ms for three force
-  // components, so off by default)
-  constexpr bool PAIR = FPM_FFT_PAIR && KOP && NT < 1024 &&
-                        (2 * N * BZ + N) * sizeof(C<T>) <= 144 * 1024;
-  if constexpr (PAIR) {
-    if (a.ncomp >= 2) {
-      hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, 2>), dim3((unsigned)a.nblk), dim3(NT),
-                         0, st, a);
-      return;
-    }
-  }
-  hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, 1>), dim3((unsigned)a.nblk), dim3(NT), 0,
-                     st, a);
+  if (KOP && a.ncomp > 1)
+    hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, KOP>), dim3((unsigned)a.nblk), dim3(NT),
+                       0, st, a);
+  else
+    hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, false>), dim3((unsigned)a.nblk),
+                       dim3(NT), 0, st, a);
 }
 
 template <typename T, bool INV, bool KOP>
 static int launch_col(int n, ColArgs<T> a, hipStream_t st) {
+  if (a.ncomp < 1 || a.ncomp > 6 || (!KOP && a.ncomp != 1)) return FPM_ERR_ARG;
+  for (int c = 0; c < a.ncomp; ++c)
+    if (!a.out[c]) return FPM_ERR_ARG;
   switch (n) {
 #define FPM_CASE(N_) \
   case N_: launch_col_n<T, N_, INV, KOP>(a, st); break;
@@ -776,8 +892,11 @@ static int c2r_impl(const void* in, int ncomp, const fpm_kop* ops, void* const* 
   int rc = check_mesh(m, pitch2);
   if (rc) return rc;
   if (ncomp < 1 || ncomp > 6 || !in || !outs || !work || !tw0 || !tw1 || !tw2) return FPM_ERR_ARG;
-  for (int c = 0; c < ncomp; ++c)
+  for (int c = 0; c < ncomp; ++c) {
     if (!outs[c] || !work[c] || work[c] == in) return FPM_ERR_ARG;
+    for (int d = 0; d < c; ++d)
+      if (work[c] == work[d]) return FPM_ERR_ARG;
+  }
   const int n0 = m->nmesh[0], n1 = m->nmesh[1], n2 = m->nmesh[2];
   const int nh = n2 / 2 + 1;
   ColArgs<T> x{};

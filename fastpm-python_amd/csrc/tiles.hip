@@ -58,25 +58,33 @@ struct Workspace {
   int32_t* offs;       // nbins + 1
   int32_t* cursor;
   int32_t* misplaced;  // particles found outside their binned tile
+  int32_t* btot;       // per-workgroup totals of the sub-bin count scan
   int64_t bytes;
 };
 
 static int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 8;
+constexpr int kScanBlock = kScanThreads * kScanPer;  // sub-bins per scan workgroup
+
+static int64_t scan_blocks(const TileGeom& g) { return (g.nbins + kScanBlock - 1) / kScanBlock; }
+
 // Byte offsets of the workspace sections: key, list, count, offs, cursor,
-// misplaced; off[6] is the total size.
-static void ws_offsets(const TileGeom& g, int64_t n, int64_t off[7]) {
-  const int64_t sizes[6] = {n * 4, n * 4, g.nbins * 4, (g.nbins + 1) * 4, g.nbins * 4, 4};
+// misplaced, btot; off[7] is the total size.
+static void ws_offsets(const TileGeom& g, int64_t n, int64_t off[8]) {
+  const int64_t sizes[7] = {n * 4, n * 4, g.nbins * 4, (g.nbins + 1) * 4, g.nbins * 4, 4,
+                            scan_blocks(g) * 4};
   int64_t o = 0;
-  for (int i = 0; i < 6; ++i) {
+  for (int i = 0; i < 7; ++i) {
     off[i] = o;
     o += align256(sizes[i]);
   }
-  off[6] = o;
+  off[7] = o;
 }
 
 static int64_t layout_ws(const TileGeom& g, int64_t n, char* base, Workspace* w) {
-  int64_t off[7];
+  int64_t off[8];
   ws_offsets(g, n, off);
   if (w) {
     w->key = reinterpret_cast<int32_t*>(base + off[0]);
@@ -85,9 +93,10 @@ static int64_t layout_ws(const TileGeom& g, int64_t n, char* base, Workspace* w)
     w->offs = reinterpret_cast<int32_t*>(base + off[3]);
     w->cursor = reinterpret_cast<int32_t*>(base + off[4]);
     w->misplaced = reinterpret_cast<int32_t*>(base + off[5]);
-    w->bytes = off[6];
+    w->btot = reinterpret_cast<int32_t*>(base + off[6]);
+    w->bytes = off[7];
   }
-  return off[6];
+  return off[7];
 }
 
 // Window origin (wrapped) and weights, identical to the plain paint/readout.
@@ -169,50 +178,65 @@ __global__ void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w) {
   }
 }
 
-// Exclusive scan of the sub-bin counts in one workgroup, kScanChunk counts at
-// a time staged in LDS with coalesced loads and stores.
-constexpr int kScanThreads = 1024;
-constexpr int kScanChunk = 16384;
-
-__global__ __launch_bounds__(kScanThreads) void k_bin_scan(Workspace w, int64_t nbins) {
-  __shared__ int32_t buf[kScanChunk];
-  __shared__ int32_t part[kScanThreads];
-  constexpr int PER = kScanChunk / kScanThreads;
-  const int tid = threadIdx.x;
-  int32_t carry = 0;
-  for (int64_t c0 = 0; c0 < nbins; c0 += kScanChunk) {
-    for (int i = tid; i < kScanChunk; i += kScanThreads)
-      buf[i] = c0 + i < nbins ? w.count[c0 + i] : 0;
-    __syncthreads();
-    int32_t s = 0;
+// Exclusive scan of the sub-bin counts in two passes over kScanBlock-bin
+// chunks.  Pass one scans each chunk locally (into offs) and records the
+// chunk total; pass two adds the sum of the earlier chunk totals, which every
+// workgroup reduces for itself (a few hundred values), and fills cursor.
+__device__ __forceinline__ int32_t wave_incl_scan(int32_t v, int lane) {
 #pragma unroll
-    for (int i = 0; i < PER; ++i) s += buf[tid * PER + i];
-    part[tid] = s;
-    __syncthreads();
-    for (int o = 1; o < kScanThreads; o <<= 1) {
-      const int32_t v = tid >= o ? part[tid - o] : 0;
-      __syncthreads();
-      part[tid] += v;
-      __syncthreads();
-    }
-    int32_t run = carry + (tid ? part[tid - 1] : 0);
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int32_t c = buf[tid * PER + i];
-      buf[tid * PER + i] = run;
-      run += c;
-    }
-    const int32_t total = part[kScanThreads - 1];
-    __syncthreads();
-    for (int i = tid; i < kScanChunk; i += kScanThreads)
-      if (c0 + i < nbins) {
-        w.offs[c0 + i] = buf[i];
-        w.cursor[c0 + i] = buf[i];
-      }
-    carry += total;
-    __syncthreads();
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t u = __shfl_up(v, o);
+    if (lane >= o) v += u;
   }
-  if (tid == 0) w.offs[nbins] = carry;
+  return v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_bin_scan_local(Workspace w, int64_t nbins) {
+  __shared__ int32_t wsum[kScanThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t i0 = (int64_t)blockIdx.x * kScanBlock + (int64_t)tid * kScanPer;
+  int32_t v[kScanPer];
+  int32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    v[i] = i0 + i < nbins ? w.count[i0 + i] : 0;
+    s += v[i];
+  }
+  const int32_t incl = wave_incl_scan(s, lane);
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  int32_t run = incl - s;
+  for (int k = 0; k < wid; ++k) run += wsum[k];
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    if (i0 + i < nbins) w.offs[i0 + i] = run;
+    run += v[i];
+  }
+  if (tid == kScanThreads - 1) w.btot[blockIdx.x] = run;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int64_t nbins) {
+  __shared__ int32_t wsum[kScanThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int32_t s = 0;
+  for (int b = tid; b < (int)blockIdx.x; b += kScanThreads) s += w.btot[b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) wsum[wid] = s;
+  __syncthreads();
+  int32_t base = 0;
+#pragma unroll
+  for (int k = 0; k < kScanThreads / 64; ++k) base += wsum[k];
+  const int64_t c0 = (int64_t)blockIdx.x * kScanBlock;
+  for (int i = tid; i < kScanBlock; i += kScanThreads) {
+    const int64_t b = c0 + i;
+    if (b < nbins) {
+      const int32_t o = w.offs[b] + base;
+      w.offs[b] = o;
+      w.cursor[b] = o;
+    }
+  }
+  if (blockIdx.x == gridDim.x - 1 && tid == 0) w.offs[nbins] = base + w.btot[blockIdx.x];
 }
 
 // Each lane is in exactly one same-sub-bin group, so only a group's leader
@@ -560,7 +584,9 @@ static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, voi
     else
       hipLaunchKernelGGL((k_bin_count<T, 3>), dim3(grid_parts(P.n)), dim3(256), 0, st, P, g, w);
   }
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanThreads), 0, st, w, g.nbins);
+  const unsigned nsb = (unsigned)scan_blocks(g);
+  hipLaunchKernelGGL(k_bin_scan_local, dim3(nsb), dim3(kScanThreads), 0, st, w, g.nbins);
+  hipLaunchKernelGGL(k_bin_scan_add, dim3(nsb), dim3(kScanThreads), 0, st, w, g.nbins);
   if (P.n > 0)
     hipLaunchKernelGGL(k_bin_scatter, dim3(grid_parts(P.n)), dim3(256), 0, st, P.n, w);
   FPM_LAUNCH_CHECK();
@@ -654,7 +680,7 @@ int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n) {
 
 int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n) {
   if (!tiles_ok(mesh) || (order != 2 && order != 3) || n < 0) return -1;
-  int64_t off[7];
+  int64_t off[8];
   ws_offsets(tile_geom(mesh), n, off);
   return off[5];
 }

@@ -64,12 +64,21 @@ __device__ __forceinline__ void particle_cell(const PartDesc<T>& P, int64_t p, i
   // readout) must round its position identically.
 #pragma clang fp contract(off)
   if (P.q == nullptr) {
-    const int32_t nz = P.lat[2], ny = P.lat[1];
-    const int64_t iz = p % nz;
-    const int64_t t = p / nz;
-    const int64_t iy = t % ny;
-    const int64_t ix = t / ny;
-    const int64_t li[3] = {ix, iy, iz};
+    int64_t li[3];
+    if (p <= 0x7fffffffLL) {  // 32-bit index arithmetic (far cheaper than 64-bit division)
+      const uint32_t nz = (uint32_t)P.lat[2], ny = (uint32_t)P.lat[1];
+      const uint32_t pu = (uint32_t)p;
+      const uint32_t t = pu / nz;
+      li[2] = pu - t * nz;
+      li[0] = t / ny;
+      li[1] = t - (uint32_t)li[0] * ny;
+    } else {
+      const int64_t nz = P.lat[2], ny = P.lat[1];
+      const int64_t t = p / nz;
+      li[2] = p - t * nz;
+      li[0] = t / ny;
+      li[1] = t - li[0] * ny;
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double base = ((double)li[d] + P.lat_shift) * P.ratio[d];

@@ -77,7 +77,9 @@ class StateVector:
     def save(self, filename, attrs={}):
         """Snapshot directory: ``Header.json`` with the attributes written by
         core.py:65-89 and ``1/Position.npy``, ``1/Velocity.npy`` (km/s),
-        ``1/Density.npy``."""
+        ``1/Density.npy``.  For restarts (``StateVector.load``) it also keeps
+        the solver state: ``1/Displacement.npy``, ``1/Momentum.npy``,
+        ``1/Force.npy`` and the time tags ``a_S``, ``a_P``, ``a_F``."""
         a = self.a["S"]
         os.makedirs(os.path.join(filename, "1"), exist_ok=True)
         header = {}
@@ -86,6 +88,8 @@ class StateVector:
             if v is not None:
                 header[key] = v if np.isscalar(v) else list(np.ravel(v))
         header["Time"] = a
+        for key in ("S", "P", "F"):
+            header["a_" + key] = self.a[key]
         header["h"] = self.cosmology.H0 / self.H0
         header["RSDFactor"] = 1.0 / (self.H0 * a * self.cosmology.efunc(1.0 / a - 1))
         for key in attrs:
@@ -99,6 +103,26 @@ class StateVector:
         np.save(os.path.join(filename, "1", "Position.npy"), self.X.cpu().numpy())
         np.save(os.path.join(filename, "1", "Velocity.npy"), self.V.cpu().numpy())
         np.save(os.path.join(filename, "1", "Density.npy"), self.RHO.cpu().numpy())
+        np.save(os.path.join(filename, "1", "Displacement.npy"), self.S.cpu().numpy())
+        np.save(os.path.join(filename, "1", "Momentum.npy"), self.P.cpu().numpy())
+        np.save(os.path.join(filename, "1", "Force.npy"), self.F.cpu().numpy())
+
+    @classmethod
+    def load(cls, solver, filename, Q):
+        """Restores a state written by ``save`` (the reference has no loader):
+        S, P, F, RHO and the time tags, on the Lagrangian positions Q."""
+        with open(os.path.join(filename, "Header.json")) as f:
+            header = json.load(f)
+        state = cls(solver, Q)
+        for attr, name in (("S", "Displacement"), ("P", "Momentum"), ("F", "Force"),
+                           ("RHO", "Density")):
+            arr = np.load(os.path.join(filename, "1", name + ".npy"))
+            t = getattr(state, attr)
+            if tuple(arr.shape) != tuple(t.shape):
+                raise ValueError("%s has shape %s, expected %s" % (name, arr.shape, tuple(t.shape)))
+            t.copy_(torch.from_numpy(arr).to(dtype=t.dtype))
+        state.a = {key: header["a_" + key] for key in ("S", "P", "F")}
+        return state
 
 
 class Solver:

@@ -265,7 +265,8 @@ def test_errors(gpu):
 
 @pytest.mark.parametrize("order", [2, 3])
 @pytest.mark.parametrize("nmesh,kind", [((16, 16, 16), "lattice"), ((32, 48, 64), "random"),
-                                        ((64, 64, 64), "lattice"), ((96, 32, 16), "cluster")])
+                                        ((64, 64, 64), "lattice"), ((96, 32, 16), "cluster"),
+                                        ((32, 48, 48), "lattice4")])
 def test_tiled_paint_readout(gpu, order, nmesh, kind):
     """Binned paint/readout against the oracle; bitwise reproducible; no
     particle found outside its binned tile."""
@@ -275,8 +276,9 @@ def test_tiled_paint_readout(gpu, order, nmesh, kind):
     box = (64.0, 96.0, 128.0)
     rng = np.random.default_rng(21)
     pm = _pm(nmesh, box, "f4", "cic" if order == 2 else "tsc")
-    if kind == "lattice":
-        nlat = tuple(n // 2 for n in nmesh)
+    if kind.startswith("lattice"):
+        # lattice4: sides 8 x 12 x 12, not multiples of 8 (index-order binning)
+        nlat = tuple(n // (4 if kind == "lattice4" else 2) for n in nmesh)
         pm_p = _pm(nlat, box, "f4")
         Q = pm_p.generate_uniform_particle_grid(shift=0.0)
         s = rng.normal(0, 3.0, size=(len(Q), 3))

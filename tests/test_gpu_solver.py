@@ -3,8 +3,8 @@ against the FP64 oracle Solver (GPU only).
 
 Float64 runs agree with the oracle to the tolerance of the growth ODE: the
 oracle restates the reference's odeint solve (relative tolerance ~1.5e-8)
-while the product integrates with RK4, so end-to-end agreement is checked at
-rtol 1e-6 of each array's scale.  Float32 runs are checked against the
+while the product integrates with Dormand-Prince 5(4), so end-to-end
+agreement is checked at rtol 1e-6 of each array's scale.  Float32 runs are checked against the
 north-star gates at small sizes: P(k) relative error <= 1e-3 for
 k < 0.5 k_Nyq and r(k) >= 0.999."""
 import numpy as np
@@ -156,3 +156,62 @@ def test_state_api(gpu, tmp_path):
     st.save(str(tmp_path / "snap"))
     X = np.load(tmp_path / "snap" / "1" / "Position.npy")
     assert X.shape == (512, 3)
+
+
+def test_pipeline_monitor_outputs(gpu, tmp_path):
+    """The CLI outputs (main.py:81-113): power-linear.txt, one power-<a>.txt per
+    Force with columns k, P, N, P/D^2 equal to the oracle estimator on the same
+    delta_k, and a snapshot at the output epoch."""
+    from fastpm.core import leapfrog
+    from fastpm.utils import PipelineMonitor
+
+    solver, _, Pk, pm = _setup(8, 64.0, 2, "f8")
+    dlin = solver.linear(solver.whitenoise(11), Pk)
+    st = solver.lpt(dlin, pm.generate_uniform_particle_grid(), 0.5)
+    mon = PipelineMonitor(solver, str(tmp_path), aout=[1.0], verbose=False)
+    mon.write_linear(dlin)
+    seen = []
+
+    def monitor(action, ai, ac, af, state, event):
+        mon(action, ai, ac, af, state, event)
+        if action == "F":
+            seen.append((state.a["F"], event["delta_k"].value.cpu().numpy().copy()))
+
+    solver.nbody(st, leapfrog([0.5, 0.75, 1.0]), monitor=monitor)
+    assert len(seen) == 3
+    nm = (16, 16, 16)
+    growth = O.Growth(0.3, a=[0.5, 0.75, 1.0])
+    for a, dk in seen:
+        got = np.loadtxt(tmp_path / ("power-%06.4f.txt" % a))
+        assert got.shape == (8, 4)
+        k, p, n, _ = O.power_spectrum(dk, nm, (64.0,) * 3, nbins=8)
+        ok = n > 0
+        assert np.array_equal(got[:, 2], n)
+        assert np.allclose(got[ok, 0], k[ok], rtol=1e-12)
+        assert np.allclose(got[ok, 1], p[ok], rtol=1e-10)
+        assert np.allclose(got[ok, 3], p[ok] / growth.D1(a) ** 2, rtol=1e-6)
+    lin = np.loadtxt(tmp_path / "power-linear.txt")
+    assert lin.shape == (4, 4)
+    snap = tmp_path / "fpm-1.0000"
+    assert (snap / "Header.json").exists() and (snap / "1" / "Position.npy").exists()
+
+
+def test_state_restart_bitwise(gpu, tmp_path):
+    """save -> load -> continue reproduces an uninterrupted float32 run bit for
+    bit (tiled deterministic paint, same kernels)."""
+    from fastpm.core import StateVector, leapfrog
+
+    solver, _, Pk, pm = _setup(8, 64.0, 2, "f4")
+    dlin = solver.linear(solver.whitenoise(3), Pk)
+    Q = pm.generate_uniform_particle_grid()
+    ref = solver.lpt(dlin, Q, 0.3)
+    solver.nbody(ref, leapfrog([0.3, 0.5, 0.7, 1.0]))
+    st = solver.lpt(dlin, Q, 0.3)
+    solver.nbody(st, leapfrog([0.3, 0.5]))
+    st.save(str(tmp_path / "ck"))
+    back = StateVector.load(solver, str(tmp_path / "ck"), Q)
+    assert back.a == st.a
+    steps = list(leapfrog([0.5, 0.7, 1.0]))[1:]  # the first Force is in the checkpoint
+    solver.nbody(back, steps)
+    for key in "SPF":
+        assert torch.equal(getattr(back, key), getattr(ref, key)), key

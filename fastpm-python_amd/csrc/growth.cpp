@@ -151,7 +151,7 @@ extern "C" const char* fpm_strerror(int status) {
   switch (status) {
     case FPM_OK: return "ok";
     case FPM_ERR_ARG: return "invalid argument (shape, size, order or pointer)";
-    case FPM_ERR_UNSUPPORTED: return "unsupported FFT length";
+    case FPM_ERR_UNSUPPORTED: return "unsupported size (FFT length or mesh side)";
     case FPM_ERR_HIP: return "HIP launch failed";
     case FPM_ERR_WORKSPACE: return "workspace too small";
     default: return "unknown status";

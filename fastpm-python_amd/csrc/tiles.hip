@@ -564,6 +564,186 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
   }
 }
 
+// ---------------------------------------------------------------- force readout
+// The four-point gradient of the force, i a(k) with
+// a = (8 sin kh - sin 2kh) / (6h), is the Fourier symbol of the real-space
+// stencil D f(x) = [8 (f(x+h) - f(x-h)) - (f(x+2h) - f(x-2h))] / (12 h), so
+// c2r(i a_d pot_k) = D_d c2r(pot_k) exactly: one inverse transform of the
+// potential replaces three, and the stencil is applied while interpolating.
+// A workgroup per tile stages the density box (lo .. lo+TILE+ORDER-2) and the
+// potential box with two more cells on each side, then reads RHO and
+// F_d = sum_window w * D_d phi for its particles.
+template <typename T>
+struct GradRead {
+  const T* rho;      // density mesh (nullptr: no RHO output)
+  const T* phi;      // potential mesh
+  T* rho_out;
+  T* f_out;          // f_out[p * f_stride + d]
+  int64_t rho_stride, f_stride;
+  T rho_scale;
+  T f_scale[3];      // force factor / (12 h_d)
+};
+
+// Scalar box of (TILE + L + U)^3 cells starting at lo - L, staged with the
+// aligned TILE-cell middle of every row as four-wide vectors and L + U wrapped
+// edge cells.
+template <typename T, int L, int U>
+struct ScalarBox {
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  static constexpr int TB = TILE + L + U;
+  static constexpr int NQ = TILE / 4;
+  static constexpr int NV = TB * TB * NQ;
+  static constexpr int PV = (NV + kTileThreads - 1) / kTileThreads;
+  static constexpr int NE = TB * TB * (L + U);
+  static constexpr int PE = (NE + kTileThreads - 1) / kTileThreads;
+  V4 vr[PV];
+  T er[PE];
+
+  __device__ __forceinline__ static int wrap(int i, int n) {
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+  }
+
+  __device__ __forceinline__ void load(const TileGeom& g, const T* mesh, const int lo[3]) {
+#pragma unroll
+    for (int k = 0; k < PV; ++k) {
+      const int i = threadIdx.x + k * kTileThreads;
+      if (i < NV) {
+        const int q = i % NQ, r = i / NQ;
+        const int gy = wrap(lo[1] - L + r % TB, g.nm[1]);
+        const int gx = wrap(lo[0] - L + r / TB, g.nm[0]);
+        vr[k] = *reinterpret_cast<const V4*>(mesh + ((int64_t)gx * g.nm[1] + gy) * g.nm[2] +
+                                             lo[2] + 4 * q);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PE; ++k) {
+      const int i = threadIdx.x + k * kTileThreads;
+      if (i < NE) {
+        const int h = i % (L + U), r = i / (L + U);
+        const int gy = wrap(lo[1] - L + r % TB, g.nm[1]);
+        const int gx = wrap(lo[0] - L + r / TB, g.nm[0]);
+        const int gz = wrap(h < L ? lo[2] - L + h : lo[2] + TILE + (h - L), g.nm[2]);
+        er[k] = mesh[((int64_t)gx * g.nm[1] + gy) * g.nm[2] + gz];
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(T* box) const {
+#pragma unroll
+    for (int k = 0; k < PV; ++k) {
+      const int i = threadIdx.x + k * kTileThreads;
+      if (i < NV) {
+        const int q = i % NQ, r = i / NQ;
+        T* dst = &box[r * TB + L + 4 * q];
+        dst[0] = vr[k].x;
+        dst[1] = vr[k].y;
+        dst[2] = vr[k].z;
+        dst[3] = vr[k].w;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PE; ++k) {
+      const int i = threadIdx.x + k * kTileThreads;
+      if (i < NE) {
+        const int h = i % (L + U), r = i / (L + U);
+        box[r * TB + (h < L ? h : TILE + h)] = er[k];
+      }
+    }
+  }
+};
+
+template <typename T, int ORDER>
+__device__ __forceinline__ void grad_one(const T* rbox, const T* pbox, const Workspace& w,
+                                         const GradRead<T>& G, int64_t p, const int i0[3],
+                                         const T wt[3][ORDER], const int lo[3]) {
+  constexpr int TR = TILE + ORDER - 1;   // density box side
+  constexpr int TP = TILE + ORDER + 3;   // potential box side (2 more cells each way)
+  constexpr int NLN = ORDER + 4;         // stencil line: window points -2 .. +2
+  int lx = i0[0] - lo[0], ly = i0[1] - lo[1], lz = i0[2] - lo[2];
+  if ((unsigned)lx >= (unsigned)TILE || (unsigned)ly >= (unsigned)TILE ||
+      (unsigned)lz >= (unsigned)TILE) {
+    atomicAdd(w.misplaced, 1);
+    lx = lx < 0 ? 0 : (lx >= TILE ? TILE - 1 : lx);
+    ly = ly < 0 ? 0 : (ly >= TILE ? TILE - 1 : ly);
+    lz = lz < 0 ? 0 : (lz >= TILE ? TILE - 1 : lz);
+  }
+  if (G.rho_out) {
+    T acc = (T)0;
+#pragma unroll
+    for (int a = 0; a < ORDER; ++a)
+#pragma unroll
+      for (int b = 0; b < ORDER; ++b) {
+        const T wab = wt[0][a] * wt[1][b];
+        const int row = ((lx + a) * TR + (ly + b)) * TR + lz;
+#pragma unroll
+        for (int c = 0; c < ORDER; ++c) acc += wab * wt[2][c] * rbox[row + c];
+      }
+    G.rho_out[p * G.rho_stride] = acc * G.rho_scale;
+  }
+  // potential box index of window point (a, b, c): (lx+2+a, ly+2+b, lz+2+c)
+  auto at = [&](int x, int y, int z) { return pbox[(x * TP + y) * TP + z]; };
+  T f[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+#pragma unroll
+    for (int b = 0; b < ORDER; ++b)
+#pragma unroll
+      for (int c = 0; c < ORDER; ++c) {
+        T line[NLN];
+#pragma unroll
+        for (int u = 0; u < NLN; ++u) {
+          int ix[3];
+          ix[d] = (d == 0 ? lx : (d == 1 ? ly : lz)) + u;
+          ix[e1] = (e1 == 0 ? lx : (e1 == 1 ? ly : lz)) + 2 + b;
+          ix[e2] = (e2 == 0 ? lx : (e2 == 1 ? ly : lz)) + 2 + c;
+          line[u] = at(ix[0], ix[1], ix[2]);
+        }
+        T s = (T)0;
+#pragma unroll
+        for (int a = 0; a < ORDER; ++a)
+          s += wt[d][a] * ((T)8 * (line[a + 3] - line[a + 1]) - (line[a + 4] - line[a]));
+        f[d] += wt[e1][b] * wt[e2][c] * s;
+      }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) G.f_out[p * G.f_stride + d] = f[d] * G.f_scale[d];
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kTileThreads) void k_readout_grad_tiles(PartDesc<T> P, TileGeom g,
+                                                                      Workspace w, GradRead<T> G) {
+  constexpr int TR = TILE + ORDER - 1;
+  constexpr int TP = TILE + ORDER + 3;
+  __shared__ T rbox[TR * TR * TR];
+  __shared__ T pbox[TP * TP * TP];
+  const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
+  int lo[3];
+  tile_coords(t, g, lo);
+  const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
+  if (b == e) return;  // uniform across the workgroup
+  const int32_t s0 = b + threadIdx.x;
+  const int64_t p0 = s0 < e ? w.list[s0] : 0;
+  ScalarBox<T, 2, ORDER + 1> pb;
+  pb.load(g, G.phi, lo);
+  ScalarBox<T, 0, ORDER - 1> rb;
+  if (G.rho_out) rb.load(g, G.rho, lo);
+  int f0[3];
+  T wt0[3][ORDER];
+  if (s0 < e) window_of<T, ORDER>(P, p0, g.nm, f0, wt0);
+  pb.store(pbox);
+  if (G.rho_out) rb.store(rbox);
+  __syncthreads();
+  if (s0 < e) grad_one<T, ORDER>(rbox, pbox, w, G, p0, f0, wt0, lo);
+  for (int32_t s = s0 + kTileThreads; s < e; s += kTileThreads) {
+    const int64_t p = w.list[s];
+    int i0[3];
+    T wt[3][ORDER];
+    window_of<T, ORDER>(P, p, g.nm, i0, wt);
+    grad_one<T, ORDER>(rbox, pbox, w, G, p, i0, wt, lo);
+  }
+}
+
 // Readout without staging: the tile's particles gather straight from the
 // meshes (through L1/L2), in tile order so that a wave's gathers share lines.
 template <typename T, int ORDER, int NC>
@@ -747,6 +927,40 @@ static int readout_tiles_impl(const fpm_particles* part, const fpm_mesh* m, int 
   return FPM_OK;
 }
 
+template <typename T>
+static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int order,
+                             const void* rho, const void* phi, void* rho_out, int64_t rho_stride,
+                             void* f_out, int64_t f_stride, double rho_scale, double f_scale,
+                             void* ws, int64_t ws_bytes, hipStream_t st) {
+  PartDesc<T> P;
+  TileGeom g;
+  Workspace w;
+  int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
+  if (rc) return rc;
+  if (!phi || !f_out || f_stride < 3 || (rho_out && (!rho || rho_stride < 1))) return FPM_ERR_ARG;
+  // the potential box reaches two cells past the density box on each side
+  for (int d = 0; d < 3; ++d)
+    if (m->nmesh[d] < TILE + order + 3) return FPM_ERR_UNSUPPORTED;
+  GradRead<T> G{};
+  G.rho = static_cast<const T*>(rho);
+  G.phi = static_cast<const T*>(phi);
+  G.rho_out = static_cast<T*>(rho_out);
+  G.f_out = static_cast<T*>(f_out);
+  G.rho_stride = rho_stride;
+  G.f_stride = f_stride;
+  G.rho_scale = (T)rho_scale;
+  for (int d = 0; d < 3; ++d)
+    G.f_scale[d] = (T)(f_scale * (double)m->nmesh[d] / (12.0 * m->box[d]));
+  if (P.n == 0) return FPM_OK;
+  const dim3 grid((unsigned)g.ntiles), blk(kTileThreads);
+  if (order == 2)
+    hipLaunchKernelGGL((k_readout_grad_tiles<T, 2>), grid, blk, 0, st, P, g, w, G);
+  else
+    hipLaunchKernelGGL((k_readout_grad_tiles<T, 3>), grid, blk, 0, st, P, g, w, G);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
 }  // namespace fpm
 
 using namespace fpm;
@@ -780,6 +994,16 @@ int fpm_readout_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int o
                           const double* scales, void* ws, int64_t ws_bytes, fpm_stream_t s) {
   return readout_tiles_impl<float>(part, mesh, order, ncomp, meshes, outs, strides, scales, ws,
                                    ws_bytes, (hipStream_t)s);
+}
+
+int fpm_readout_tiled_grad4_f32(const fpm_particles* part, const fpm_mesh* mesh, int order,
+                                const void* rho, const void* phi, void* rho_out,
+                                int64_t rho_stride, void* force_out, int64_t force_stride,
+                                double rho_scale, double force_scale, void* ws, int64_t ws_bytes,
+                                fpm_stream_t s) {
+  return readout_grad_impl<float>(part, mesh, order, rho, phi, rho_out, rho_stride, force_out,
+                                  force_stride, rho_scale, force_scale, ws, ws_bytes,
+                                  (hipStream_t)s);
 }
 
 }  // extern "C"

@@ -88,6 +88,8 @@ _PLAIN = {
     "fpm_bin_particles_f32": (_int, [_PP, _PM, _int, _vp, _i64, _vp]),
     "fpm_paint_tiled_f32": (_int, [_PP, _PM, _int, _dbl, _vp, _vp, _i64, _vp]),
     "fpm_readout_tiled_f32": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "fpm_readout_tiled_grad4_f32": (_int, [_PP, _PM, _int, _vp, _vp, _vp, _i64, _vp, _i64, _dbl,
+                                           _dbl, _vp, _i64, _vp]),
     "fpm_counter_add": (_int, [_vp, ctypes.c_int32, _vp]),
 }
 

@@ -229,8 +229,11 @@ class FastPMStep:
         self._pt = MatterDominated(self.cosmology.Om0, a_normalize=solver.a_linear)
         self.timer = None  # optional KernelTimer recording each library call
         # fused float32 path on tile-divisible meshes: binned deterministic
-        # paint and LDS-staged readout
+        # paint and LDS-staged readout; the force's four-point gradient is
+        # applied as a real-space stencil inside the readout (one inverse
+        # transform of the potential instead of three)
         self.tiled = solver.fused and self.pm.tiles_supported()
+        self.stencil = self.tiled and self.pm.grad4_supported()
 
     def _mark(self, name):
         return self.timer.region(name) if self.timer is not None else _NullRegion()
@@ -304,7 +307,9 @@ class FastPMStep:
 
     def Force(self, state, ai, ac, af):
         """RHO and F at the particles (core.py:195-210): paint, forward
-        transform, three fused multiplier + inverse transforms, and one fused
+        transform, then either (float32 tiled) one inverse transform of the
+        potential and a readout applying the four-point gradient stencil, or
+        three fused multiplier + inverse transforms and one fused
         four-component readout."""
         assert ac == state.a["S"]
         delta_k = self.force_kernels(state)
@@ -320,7 +325,13 @@ class FastPMStep:
             delta_k = rho.r2c(out=ws["delta_k"])
         F = state.F
         ops = self.force_ops()
-        if self.solver.fused:
+        if self.stencil:
+            phi = ws["force"][0]
+            with self._mark("c2r"):
+                c2r_many(delta_k, [self.force_common()], outs=[phi], work=ws["work"][:1])
+            with self._mark("readout"):
+                self.pm.readout_tiled_grad4(phi, X1, F, rho=rho, rho_out=state.RHO)
+        elif self.solver.fused:
             with self._mark("c2r"):
                 c2r_grad3(delta_k, self.force_common(), outs=ws["force"], work=ws["work"])
             with self._mark("readout"):

@@ -303,6 +303,30 @@ class ParticleMesh:
         _lib.call_plain("fpm_readout_tiled_f32", parts.desc(self.dtype), self._desc, order, n,
                         meshes, optrs, strides, sc, ws.data_ptr(), ws.numel(), _stream())
 
+    def grad4_supported(self, resampler=None):
+        """Whether readout_tiled_grad4 runs on this mesh (tiled, and at least
+        16 + order + 3 cells per side for the stencil box)."""
+        order = resampler_order(resampler or self.resampler)
+        return self.tiles_supported() and int(self.Nmesh.min()) >= 16 + order + 3
+
+    def readout_tiled_grad4(self, phi, pos, force, rho=None, rho_out=None, resampler=None,
+                            force_scale=1.0, rho_scale=1.0):
+        """force[:, d] = force_scale * readout(D_d phi) with D_d the four-point
+        gradient stencil (equal to c2r of the order-1 gradient kernel times
+        phi's spectrum), and rho_out = rho_scale * readout(rho) when given;
+        `pos` must have been binned with bin_particles."""
+        parts = self._particles(pos)
+        order = resampler_order(resampler or self.resampler)
+        ws = self.tile_workspace(len(parts), order)
+        if force.dim() != 2 or force.shape[1] != 3 or force.stride(1) != 1:
+            raise ValueError("force must be an [N, 3] array with unit stride along the last axis")
+        rp = rho.value.data_ptr() if rho is not None else None
+        ro = rho_out.data_ptr() if rho_out is not None else None
+        rs = int(rho_out.stride(0)) if rho_out is not None else 1
+        _lib.call_plain("fpm_readout_tiled_grad4_f32", parts.desc(self.dtype), self._desc, order,
+                        rp, phi.value.data_ptr(), ro, rs, force.data_ptr(), int(force.stride(0)),
+                        float(rho_scale), float(force_scale), ws.data_ptr(), ws.numel(), _stream())
+
     def generate_whitenoise(self, seed, type="complex", unitary=False, mean=0):
         """Counter-based complex white noise (replaces pmesh generate_whitenoise,
         fastpm/core.py:114)."""

@@ -128,6 +128,23 @@ int fpm_readout_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int o
                           const void* const* meshes, void* const* outs, const int64_t* out_strides,
                           const double* scales, void* ws, int64_t ws_bytes, fpm_stream_t stream);
 
+/* Force readout with the four-point gradient applied in real space
+ * (replaces fastpm/force/gravity.py:17-24 -- c2r(i a_d(k) pot_k) and a readout
+ * for each d -- and the RHO readout of fastpm/core.py:203).  The four-point
+ * gradient a_d = (8 sin k_d h - sin 2 k_d h) / (6 h) is the Fourier symbol of
+ * the stencil [8 (f(x+h) - f(x-h)) - (f(x+2h) - f(x-2h))] / (12 h), so with
+ * phi = c2r(pot_k) this writes
+ *   force_out[p * force_stride + d] = force_scale * readout(D_d phi)(x_p)
+ * and, when rho_out is not NULL, rho_out[p * rho_stride] = rho_scale *
+ * readout(rho)(x_p).  Same binning workspace as fpm_readout_tiled_f32; float32
+ * meshes that are multiples of 16 and at least 19 (CIC) / 20 (TSC) cells per
+ * side. */
+int fpm_readout_tiled_grad4_f32(const fpm_particles* part, const fpm_mesh* mesh, int order,
+                                const void* rho, const void* phi, void* rho_out,
+                                int64_t rho_stride, void* force_out, int64_t force_stride,
+                                double rho_scale, double force_scale, void* ws, int64_t ws_bytes,
+                                fpm_stream_t stream);
+
 /* Real-to-half-complex 3D transform, normalised by 1/(n0 n1 n2).  `in` is left
  * intact.  tw0/tw1/tw2: device twiddle tables for n0, n1, n2.
  * Replaces RealField.r2c (fastpm/core.py:205, fastpm/force/lpt.py:64). */

@@ -360,3 +360,62 @@ def test_c2r_grad3(gpu, dtype, nmesh, order):
         assert _rel(outs[d].value.cpu().numpy(), ref) < _tol(dtype) * 8, d
     # input untouched
     assert _rel(ck.value.cpu().numpy(), xk) < _tol(dtype) * 4
+
+
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("nmesh", [(32, 32, 32), (32, 48, 64), (64, 32, 48)])
+def test_readout_grad4_matches_kspace(gpu, order, nmesh):
+    """readout(D_d phi) with the four-point stencil equals the readout of
+    c2r(i a_d(k) phi_k) (order-1 gradient kernel): against the oracle in FP64
+    and against the library's own k-space path; RHO rides along."""
+    from fastpm.pm import Particles, c2r_grad3
+    from fastpm.force.kernels import KernelOp
+
+    print("grad4", order, nmesh, flush=True)
+    box = (80.0, 96.0, 112.0)
+    rng = np.random.default_rng(5)
+    pm = _pm(nmesh, box, "f4", "cic" if order == 2 else "tsc")
+    nlat = tuple(n // 2 for n in nmesh)
+    pm_p = _pm(nlat, box, "f4")
+    Q = pm_p.generate_uniform_particle_grid(shift=0.0)
+    s = rng.normal(0, 2.0, size=(len(Q), 3)).astype("f4")
+    parts = Particles(Q, torch.from_numpy(s).to(gpu))
+    X = pm.positions(parts).cpu().numpy().astype("f8")
+    phi_h = rng.standard_normal(nmesh).astype("f4")
+    rho_h = rng.standard_normal(nmesh).astype("f4")
+    phi = pm.create("real", value=torch.from_numpy(phi_h).to(gpu))
+    rho = pm.create("real", value=torch.from_numpy(rho_h).to(gpu))
+    pm.bin_particles(parts)
+    F = torch.empty((len(Q), 3), dtype=torch.float32, device=gpu)
+    R = torch.empty((len(Q),), dtype=torch.float32, device=gpu)
+    pm.readout_tiled_grad4(phi, parts, F, rho=rho, rho_out=R, force_scale=1.5, rho_scale=0.5)
+    got = F.cpu().numpy().astype("f8")
+    xk = O.r2c(phi_h.astype("f8"))
+    for d in range(3):
+        ref = O.readout(O.c2r(xk * O.k_gradient(d, nmesh, box, 1), nmesh), X, box, order) * 1.5
+        assert np.abs(got[:, d] - ref).max() < 2e-5 * np.abs(ref).max(), d
+    rref = O.readout(rho_h.astype("f8"), X, box, order) * 0.5
+    assert np.abs(R.cpu().numpy() - rref).max() < 2e-5 * np.abs(rref).max()
+    # the library's k-space route: c2r_grad3 with an identity common factor
+    outs = c2r_grad3(phi.r2c(), KernelOp(), grad_order=1)
+    F2 = torch.empty_like(F)
+    pm.readout_tiled(outs, parts, [F2[:, 0], F2[:, 1], F2[:, 2]], scales=[1.5] * 3)
+    assert _rel(got, F2.cpu().numpy().astype("f8")) < 2e-5
+    assert pm.tile_misplaced(len(Q), order) == 0
+
+
+def test_readout_grad4_small_mesh_rejected(gpu):
+    """Meshes below 16 + order + 3 cells per side are refused (the stencil
+    box would wrap onto itself); the Force then keeps the k-space route."""
+    from fastpm.pm import Particles
+    from fastpm import _lib
+
+    pm = _pm((16, 16, 16), (50.0,) * 3, "f4")
+    assert pm.tiles_supported() and not pm.grad4_supported()
+    Q = _pm((8, 8, 8), (50.0,) * 3, "f4").generate_uniform_particle_grid()
+    parts = Particles(Q, None)
+    pm.bin_particles(parts)
+    phi = pm.create("real")
+    F = torch.empty((len(Q), 3), dtype=torch.float32, device=gpu)
+    with pytest.raises(RuntimeError, match="unsupported"):
+        pm.readout_tiled_grad4(phi, parts, F)

@@ -339,6 +339,26 @@ __device__ __forceinline__ void axis_factors_tw(int i, int n, T two_pi_l, T inv6
               : sh * ((T)n / ((T)3.14159265358979323846264338327950 * (T)a));
 }
 
+// The same from the twiddle value t = exp(-2 pi i m / n) of this mode's table
+// entry m = i (mod n): sin 2w = 2 sin w cos w, and |sin w| with cos w give
+// sin(|w| / 2) by the half-angle formulas.
+template <typename T>
+__device__ __forceinline__ void axis_factors_tv(int i, int n, C<T> t, T two_pi_l, T inv6h,
+                                                bool half, T& kk, T& g0, T& g1, T& sn) {
+  const int nn = half ? i : (i < (n + 1) / 2 ? i : i - n);
+  const bool nyq = (n % 2 == 0) && (i == n / 2);
+  const T k = two_pi_l * (T)nn;
+  kk = k * k;
+  g0 = nyq ? (T)0 : k;
+  const T c = t.x, s1 = -t.y;
+  g1 = nyq ? (T)0 : ((T)8 * s1 - (T)2 * s1 * c) * inv6h;
+  const int a = nn < 0 ? -nn : nn;
+  const T sa = s1 < (T)0 ? -s1 : s1;
+  const T sh = c >= (T)0 ? sa / sqrt((T)2 * ((T)1 + c)) : sqrt(((T)1 - c) * (T)0.5);
+  sn = a == 0 ? (T)1
+              : sh * ((T)n / ((T)3.14159265358979323846264338327950 * (T)a));
+}
+
 // Component c's separable factor along `axis` from that axis' factors.
 template <typename T>
 __device__ __forceinline__ T comp_factor_t(const KopPack& K, int c, int axis, T g0, T g1, T sn) {
@@ -383,6 +403,10 @@ struct ColArgs {
   // eaxis, outer along oaxis (x and y passes), the column along z.
   KopPack kop;
   int eaxis, oaxis;
+  // fused forward + inverse pass (FWD): the forward transform of the loaded
+  // tile, times kscale, is stored to kout before the inverse outputs
+  C<T>* kout;
+  T kscale;
 };
 
 
@@ -394,7 +418,7 @@ struct ColArgs {
 // multiplier to a copy.  Measured at 512^3 on
 // gfx950 this beats one workgroup per (tile, output), whose repeated tile
 // reads mostly miss L2.
-template <typename T, int N, int BZ, int NT, bool INV, bool KOP, bool MC>
+template <typename T, int N, int BZ, int NT, bool INV, bool KOP, bool MC, bool FWD = false>
 __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   using V = C<T>;
   constexpr int E = N * BZ;
@@ -408,9 +432,9 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   __shared__ V tw[N];
   // k-space multiplier tables: per-output separable factors along the e axis
   // (fe) and the z columns (fz) and the outer-axis factor times the output's
-  // factor (fo); k^2 along e (ke) and z (kz)
+  // factor (fo); k^2 along z (kz).  k^2 along e is recomputed from the mode
+  // index, which keeps the single-output tile within a quarter of the LDS.
   __shared__ T fe[KOP ? MAXC * N : 1];
-  __shared__ T ke[KOP ? N : 1];
   __shared__ T fz[KOP ? MAXC * BZ : 1];
   __shared__ T kz[KOP ? BZ : 1];
   __shared__ T fo[KOP ? MAXC : 1];
@@ -419,8 +443,33 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   const int outer = (int)(bid / a.ntz);
   const int z0 = (int)(bid % a.ntz) * BZ;
 
-  for (int i = threadIdx.x; i < N; i += NT) tw[i] = a.tw[i];
-  T ko = (T)0;
+  // the twiddle loads go out first and the tile loads second, so the tables
+  // (built from each thread's own twiddles) wait only for the former
+  constexpr int PTW = (N + NT - 1) / NT;
+  V twr[PTW];
+#pragma unroll
+  for (int q = 0; q < PTW; ++q) {
+    const int i = threadIdx.x + q * NT;
+    twr[q] = ((N % NT == 0) || i < N) ? a.tw[i] : cmk<V>((T)0, (T)0);
+  }
+  // butterfly b of the first stage: column l = b % BZ, j = b / BZ
+  const V* src = a.in + (int64_t)outer * a.ostride + z0;
+  V v0[PER0][R0];
+#pragma unroll
+  for (int q = 0; q < PER0; ++q) {
+    const int b = threadIdx.x + q * NT;
+    const int l = b % BZ, j = b / BZ;
+    const bool ok = ((NB0 % NT == 0) || b < NB0) && z0 + l < a.ncols;
+#pragma unroll
+    for (int r = 0; r < R0; ++r)
+      v0[q][r] = ok ? src[(int64_t)(j + r * (N / R0)) * a.estride + l] : cmk<V>((T)0, (T)0);
+  }
+#pragma unroll
+  for (int q = 0; q < PTW; ++q) {
+    const int i = threadIdx.x + q * NT;
+    if ((N % NT == 0) || i < N) tw[i] = twr[q];
+  }
+  T ko = (T)0, kstep = (T)0;
   if constexpr (KOP) {
     const KopPack& K = a.kop;
     const int nc = MC ? a.ncomp : 1;
@@ -428,11 +477,14 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
     // each output's product
     const T two_pi_l = (T)(6.283185307179586476925286766559 / K.box[a.eaxis]);
     const T inv6h = (T)((double)N / (6.0 * K.box[a.eaxis]));
-    for (int e = threadIdx.x; e < N; e += NT) {
-      T kk, g0, g1, sn;
-      axis_factors_tw<T>(e, N, two_pi_l, inv6h, false, a.tw, kk, g0, g1, sn);
-      ke[e] = kk;
-      for (int c = 0; c < nc; ++c) fe[c * N + e] = comp_factor_t<T>(K, c, a.eaxis, g0, g1, sn);
+#pragma unroll
+    for (int q = 0; q < PTW; ++q) {
+      const int e = threadIdx.x + q * NT;
+      if ((N % NT == 0) || e < N) {
+        T kk, g0, g1, sn;
+        axis_factors_tv<T>(e, N, twr[q], two_pi_l, inv6h, false, kk, g0, g1, sn);
+        for (int c = 0; c < nc; ++c) fe[c * N + e] = comp_factor_t<T>(K, c, a.eaxis, g0, g1, sn);
+      }
     }
     for (int i = threadIdx.x; i < BZ * nc; i += NT) {
       const int c = i / BZ, l = i % BZ;
@@ -449,21 +501,102 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
     const int nno = outer < (no + 1) / 2 ? outer : outer - no;
     const double kao = 6.283185307179586476925286766559 * (double)nno / K.box[a.oaxis];
     ko = (T)(kao * kao);
+    kstep = (T)(6.283185307179586476925286766559 / K.box[a.eaxis]);
   }
 
-  // butterfly b of the first stage: column l = b % BZ, j = b / BZ
-  const V* src = a.in + (int64_t)outer * a.ostride + z0;
-  V v0[PER0][R0];
-#pragma unroll
-  for (int q = 0; q < PER0; ++q) {
-    const int b = threadIdx.x + q * NT;
-    const int l = b % BZ, j = b / BZ;
-    const bool ok = ((NB0 % NT == 0) || b < NB0) && z0 + l < a.ncols;
-#pragma unroll
-    for (int r = 0; r < R0; ++r)
-      v0[q][r] = ok ? src[(int64_t)(j + r * (N / R0)) * a.estride + l] : cmk<V>((T)0, (T)0);
-  }
   if constexpr (KOP) __syncthreads();  // multiplier tables ready
+
+  if constexpr (FWD) {
+    // forward transform of the tile: first stage in registers, the rest in
+    // LDS, the last stage's outputs (element j + r N/RL) back in registers
+    constexpr bool SAME = (R0 == RL) && (PER0 == PERL);
+    V vf[PERL][RL];
+    if constexpr (NSTG == 1) {
+#pragma unroll
+      for (int q = 0; q < PER0; ++q) {
+        Dft<R0, false, V>::run(v0[q]);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) vf[q][r] = v0[q][r];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < PER0; ++q) {
+        const int b = threadIdx.x + q * NT;
+        if ((NB0 % NT == 0) || b < NB0) {
+          const int l = b % BZ, j = b / BZ;
+          V v[R0];
+#pragma unroll
+          for (int r = 0; r < R0; ++r) v[r] = v0[q][r];
+          Dft<R0, false, V>::run(v);
+#pragma unroll
+          for (int r = 0; r < R0; ++r) tile[(j * R0 + r) * BZ + l] = v[r];
+        }
+      }
+      __syncthreads();
+      StagesTo<T, N, 1, NSTG - 1, R0, BZ, NT, true, BZ, 1, 1, false>::run(tile, tw);
+#pragma unroll
+      for (int q = 0; q < PERL; ++q) {
+        const int b = threadIdx.x + q * NT;
+        if ((NBL % NT == 0) || b < NBL) {
+          const int l = b % BZ, j = b / BZ;
+#pragma unroll
+          for (int r = 0; r < RL; ++r) vf[q][r] = tile[(j + r * (N / RL)) * BZ + l];
+#pragma unroll
+          for (int r = 1; r < RL; ++r) vf[q][r] = cmul(vf[q][r], tw[r * j]);
+          Dft<RL, false, V>::run(vf[q]);
+        }
+      }
+    }
+    // scale, store the forward result, and hand it to the inverse
+    constexpr int RS = NSTG == 1 ? R0 : RL;
+    constexpr int PS = NSTG == 1 ? PER0 : PERL;
+    constexpr int NBS = NSTG == 1 ? NB0 : NBL;
+    V* kdst = a.kout + (int64_t)outer * a.ostride + z0;
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+      const int b = threadIdx.x + q * NT;
+      if ((NBS % NT == 0) || b < NBS) {
+        const int l = b % BZ, j = b / BZ;
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+          vf[q][r] = cscale(vf[q][r], a.kscale);
+          const int e = NSTG == 1 ? j * R0 + r : j + r * (N / RL);
+          if (z0 + l < a.ncols) kdst[(int64_t)e * a.estride + l] = vf[q][r];
+        }
+      }
+    }
+    if constexpr (SAME || NSTG == 1) {
+      // the forward outputs of butterfly (l, j) are the inverse's first-stage
+      // inputs of the same butterfly (single stage: element j R0 + r, j = 0)
+#pragma unroll
+      for (int q = 0; q < PER0; ++q)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v0[q][r] = vf[q][r];
+      __syncthreads();  // the last forward stage has read the tile
+    } else {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PERL; ++q) {
+        const int b = threadIdx.x + q * NT;
+        if ((NBL % NT == 0) || b < NBL) {
+          const int l = b % BZ, j = b / BZ;
+#pragma unroll
+          for (int r = 0; r < RL; ++r) tile[(j + r * (N / RL)) * BZ + l] = vf[q][r];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PER0; ++q) {
+        const int b = threadIdx.x + q * NT;
+        if ((NB0 % NT == 0) || b < NB0) {
+          const int l = b % BZ, j = b / BZ;
+#pragma unroll
+          for (int r = 0; r < R0; ++r) v0[q][r] = tile[(j + r * (N / R0)) * BZ + l];
+        }
+      }
+      __syncthreads();
+    }
+  }
 
   // output c from the first-stage inputs v (overwritten)
   auto one_output = [&](int c, V (&v)[PER0][R0]) {
@@ -489,7 +622,8 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
             const int e = j + r * (N / R0);
             T m = mz * fe[c * N + e];
             if (lap || r2 != (T)0) {
-              const T kk = ke[e] + kzo;
+              const T ka = kstep * (T)(e < (N + 1) / 2 ? e : e - N);
+              const T kk = ka * ka + kzo;
               if (lap) m = kk == (T)0 ? (T)0 : m * fast_rcp<T>(kk);
               if (r2 != (T)0) m *= exp(-kk * r2);
             }
@@ -682,9 +816,9 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(ZArgs<T> a) {
       }
       if (a.zgrad) {
         T kk, g0, g1, sn;
-        axis_factors_tw<T>(k, 2 * M, two_pi_l, inv6h, true, a.tw, kk, g0, g1, sn);
+        axis_factors_tw<T>(k, 2 * M, two_pi_l, inv6h, true, tw, kk, g0, g1, sn);
         xa = apply_mult(xa, a.zorder == 0 ? g0 : g1, 1);
-        axis_factors_tw<T>(M - k, 2 * M, two_pi_l, inv6h, true, a.tw, kk, g0, g1, sn);
+        axis_factors_tw<T>(M - k, 2 * M, two_pi_l, inv6h, true, tw, kk, g0, g1, sn);
         xb = apply_mult(xb, a.zorder == 0 ? g0 : g1, 1);
       }
       if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
@@ -758,6 +892,38 @@ static void launch_col_n(ColArgs<T> a, hipStream_t st) {
   else
     hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, false>), dim3((unsigned)a.nblk),
                        dim3(NT), 0, st, a);
+}
+
+template <typename T, int N>
+static void launch_col_fused_n(ColArgs<T> a, hipStream_t st) {
+  constexpr int BZ = ColW<T, N>::BZ;
+  constexpr int NT = threads_for<T>(N * BZ);
+  a.ntz = (a.ncols + BZ - 1) / BZ;
+  a.nblk = (int64_t)a.ntz * a.nouter;
+  if (a.nblk <= 0) return;
+  if (a.ncomp > 1)
+    hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, true, true, true, true>), dim3((unsigned)a.nblk),
+                       dim3(NT), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, true, true, false, true>), dim3((unsigned)a.nblk),
+                       dim3(NT), 0, st, a);
+}
+
+// Forward transform along the e axis (stored to a.kout, times a.kscale) fused
+// with the multiplier and inverse transform of every output.
+template <typename T>
+static int launch_col_fused(int n, ColArgs<T> a, hipStream_t st) {
+  if (a.ncomp < 1 || a.ncomp > 6 || !a.kout) return FPM_ERR_ARG;
+  for (int c = 0; c < a.ncomp; ++c)
+    if (!a.out[c]) return FPM_ERR_ARG;
+  switch (n) {
+#define FPM_CASE(N_) \
+  case N_: launch_col_fused_n<T, N_>(a, st); break;
+    FPM_LENGTHS(FPM_CASE)
+#undef FPM_CASE
+    default: return FPM_ERR_UNSUPPORTED;
+  }
+  return FPM_OK;
 }
 
 template <typename T, bool INV, bool KOP>
@@ -940,6 +1106,82 @@ static int c2r_impl(const void* in, int ncomp, const fpm_kop* ops, void* const* 
   return FPM_OK;
 }
 
+// Forward transform of `rin` into `kout` and inverse transforms of ncomp
+// multiplier outputs, sharing the x pass: the forward x pass of r2c and the
+// first inverse pass of c2r work on the same x columns, so one kernel does
+// both (the forward result is still stored) and the mesh is read once less.
+template <typename T>
+static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* ops,
+                        void* const* outs, void* const* work, const fpm_mesh* m, int64_t pitch2,
+                        const void* tw0, const void* tw1, const void* tw2, hipStream_t st) {
+  int rc = check_mesh(m, pitch2);
+  if (rc) return rc;
+  if (ncomp < 1 || ncomp > 6 || !rin || !kout || !outs || !work || !tw0 || !tw1 || !tw2)
+    return FPM_ERR_ARG;
+  for (int c = 0; c < ncomp; ++c) {
+    if (!outs[c] || !work[c] || work[c] == kout) return FPM_ERR_ARG;
+    for (int d = 0; d < c; ++d)
+      if (work[c] == work[d]) return FPM_ERR_ARG;
+  }
+  const int n0 = m->nmesh[0], n1 = m->nmesh[1], n2 = m->nmesh[2];
+  const int nh = n2 / 2 + 1;
+  C<T>* ck = static_cast<C<T>*>(kout);
+  ZArgs<T> z{};
+  z.rin = static_cast<const T*>(rin);
+  z.cout = ck;
+  z.nrows = (int64_t)n0 * n1;
+  z.pitch2 = pitch2;
+  z.tw = static_cast<const C<T>*>(tw2);
+  if ((rc = launch_z<T, false>(n2, z, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  ColArgs<T> y{};
+  y.in = ck;
+  y.out[0] = ck;
+  y.ncomp = 1;
+  y.estride = pitch2;
+  y.ostride = (int64_t)n1 * pitch2;
+  y.nouter = n0;
+  y.ncols = nh;
+  y.scale = (T)1;
+  y.tw = static_cast<const C<T>*>(tw1);
+  if ((rc = launch_col<T, false, false>(n1, y, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  ColArgs<T> x{};
+  x.in = ck;
+  x.kout = ck;
+  x.kscale = (T)(1.0 / ((double)n0 * n1 * n2));
+  for (int c = 0; c < ncomp; ++c) x.out[c] = static_cast<C<T>*>(work[c]);
+  x.ncomp = ncomp;
+  x.estride = (int64_t)n1 * pitch2;
+  x.ostride = pitch2;
+  x.nouter = n1;
+  x.ncols = nh;
+  x.scale = (T)1;
+  x.tw = static_cast<const C<T>*>(tw0);
+  x.eaxis = 0;
+  x.oaxis = 1;
+  if ((rc = fill_kop(x.kop, ncomp, ops, m))) return rc;
+  if ((rc = launch_col_fused<T>(n0, x, st))) return rc;
+  FPM_LAUNCH_CHECK();
+  for (int c = 0; c < ncomp; ++c) {
+    C<T>* wc = static_cast<C<T>*>(work[c]);
+    ColArgs<T> yi = y;
+    yi.in = wc;
+    yi.out[0] = wc;
+    if ((rc = launch_col<T, true, false>(n1, yi, st))) return rc;
+    FPM_LAUNCH_CHECK();
+    ZArgs<T> zi{};
+    zi.cin = wc;
+    zi.rout = static_cast<T*>(outs[c]);
+    zi.nrows = (int64_t)n0 * n1;
+    zi.pitch2 = pitch2;
+    zi.tw = static_cast<const C<T>*>(tw2);
+    if ((rc = launch_z<T, true>(n2, zi, st))) return rc;
+    FPM_LAUNCH_CHECK();
+  }
+  return FPM_OK;
+}
+
 // outs[d] = c2r(i a_d(k_d) * common(k) * in), d = 0, 1, 2, using that a_d
 // depends on k_d alone: the x pass reads `in` once and writes the common
 // product before and after the x gradient (A, B); the y pass turns A into
@@ -1098,6 +1340,18 @@ int fpm_fft_r2c_f32(const void* in, void* out, const fpm_mesh* mesh, int64_t pit
 int fpm_fft_r2c_f64(const void* in, void* out, const fpm_mesh* mesh, int64_t pitch2,
                     const void* tw0, const void* tw1, const void* tw2, fpm_stream_t s) {
   return r2c_impl<double>(in, out, mesh, pitch2, tw0, tw1, tw2, (hipStream_t)s);
+}
+int fpm_fft_r2c_c2r_f32(const void* in, void* kout, int ncomp, const fpm_kop* ops,
+                        void* const* outs, void* const* work, const fpm_mesh* mesh, int64_t pitch2,
+                        const void* tw0, const void* tw1, const void* tw2, fpm_stream_t s) {
+  return r2c_c2r_impl<float>(in, kout, ncomp, ops, outs, work, mesh, pitch2, tw0, tw1, tw2,
+                             (hipStream_t)s);
+}
+int fpm_fft_r2c_c2r_f64(const void* in, void* kout, int ncomp, const fpm_kop* ops,
+                        void* const* outs, void* const* work, const fpm_mesh* mesh, int64_t pitch2,
+                        const void* tw0, const void* tw1, const void* tw2, fpm_stream_t s) {
+  return r2c_c2r_impl<double>(in, kout, ncomp, ops, outs, work, mesh, pitch2, tw0, tw1, tw2,
+                              (hipStream_t)s);
 }
 int fpm_fft_c2r_f32(const void* in, int ncomp, const fpm_kop* ops, void* const* outs,
                     void* const* work, const fpm_mesh* mesh, int64_t pitch2, const void* tw0,

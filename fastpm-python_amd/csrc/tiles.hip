@@ -529,8 +529,9 @@ __device__ __forceinline__ void gather_one(const VC* box, const Workspace& w,
 
 // One workgroup per tile: the (TILE + ORDER - 1)^3 box of every component is
 // staged in LDS, then the tile's particles are interpolated from it.  The
-// first particle's index is fetched before the box and its position loads
-// issued before the LDS stores, so those latencies overlap the box loads.
+// indices of each thread's first two particles are fetched before the box and
+// their position loads issued before the LDS stores, so those latencies
+// overlap the box loads (a tile holds 512 particles on average at B = 2).
 // (A persistent variant that prefetched the next tile's box while
 // interpolating measured slower at 512^3 on gfx950: 1.27 vs 0.98 ms.)
 template <typename T, int ORDER, int NC>
@@ -545,17 +546,20 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
   tile_coords(t, g, lo);
   const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
   if (b == e) return;  // uniform across the workgroup
-  const int32_t s0 = b + threadIdx.x;
+  const int32_t s0 = b + threadIdx.x, s1 = s0 + kTileThreads;
   const int64_t p0 = s0 < e ? w.list[s0] : 0;
+  const int64_t p1 = s1 < e ? w.list[s1] : 0;
   BoxRegs<T, ORDER, NC> br;
   br.load(g, R, lo);
-  int f0[3];
-  T wt0[3][ORDER];
+  int f0[3], f1[3];
+  T wt0[3][ORDER], wt1[3][ORDER];
   if (s0 < e) window_of<T, ORDER>(P, p0, g.nm, f0, wt0);
+  if (s1 < e) window_of<T, ORDER>(P, p1, g.nm, f1, wt1);
   br.store(box);
   __syncthreads();
   if (s0 < e) gather_one<T, ORDER, NC>(box, w, R, p0, f0, wt0, lo);
-  for (int32_t s = s0 + kTileThreads; s < e; s += kTileThreads) {
+  if (s1 < e) gather_one<T, ORDER, NC>(box, w, R, p1, f1, wt1, lo);
+  for (int32_t s = s1 + kTileThreads; s < e; s += kTileThreads) {
     const int64_t p = w.list[s];
     int i0[3];
     T wt[3][ORDER];
@@ -722,20 +726,23 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_grad_tiles(PartDesc<T>
   tile_coords(t, g, lo);
   const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
   if (b == e) return;  // uniform across the workgroup
-  const int32_t s0 = b + threadIdx.x;
+  const int32_t s0 = b + threadIdx.x, s1 = s0 + kTileThreads;
   const int64_t p0 = s0 < e ? w.list[s0] : 0;
+  const int64_t p1 = s1 < e ? w.list[s1] : 0;
   ScalarBox<T, 2, ORDER + 1> pb;
   pb.load(g, G.phi, lo);
   ScalarBox<T, 0, ORDER - 1> rb;
   if (G.rho_out) rb.load(g, G.rho, lo);
-  int f0[3];
-  T wt0[3][ORDER];
+  int f0[3], f1[3];
+  T wt0[3][ORDER], wt1[3][ORDER];
   if (s0 < e) window_of<T, ORDER>(P, p0, g.nm, f0, wt0);
+  if (s1 < e) window_of<T, ORDER>(P, p1, g.nm, f1, wt1);
   pb.store(pbox);
   if (G.rho_out) rb.store(rbox);
   __syncthreads();
   if (s0 < e) grad_one<T, ORDER>(rbox, pbox, w, G, p0, f0, wt0, lo);
-  for (int32_t s = s0 + kTileThreads; s < e; s += kTileThreads) {
+  if (s1 < e) grad_one<T, ORDER>(rbox, pbox, w, G, p1, f1, wt1, lo);
+  for (int32_t s = s1 + kTileThreads; s < e; s += kTileThreads) {
     const int64_t p = w.list[s];
     int i0[3];
     T wt[3][ORDER];

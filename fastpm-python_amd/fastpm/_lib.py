@@ -59,6 +59,7 @@ _TYPED = {
     "fpm_readout": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _vp, _vp]),
     "fpm_fft_r2c": (_int, [_vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
     "fpm_fft_c2r": (_int, [_vp, _int, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
+    "fpm_fft_r2c_c2r": (_int, [_vp, _vp, _int, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
     "fpm_fft_c2r_grad3": (_int, [_vp, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
     "fpm_kspace_apply": (_int, [_vp, _PK, _PM, _i64, _vp]),
     "fpm_whitenoise": (_int, [_vp, _PM, _i64, ctypes.c_uint64, _int, _vp]),

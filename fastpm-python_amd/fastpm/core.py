@@ -13,7 +13,8 @@ import torch
 
 from . import _lib
 from .background import MatterDominated
-from .pm import ParticleMesh, Particles, RealField, ComplexField, c2r_many, c2r_grad3, _stream
+from .pm import (ParticleMesh, Particles, RealField, ComplexField, c2r_many, c2r_grad3,
+                 r2c_c2r_many, _stream)
 from .force.kernels import laplace, gradient, compensation
 
 
@@ -321,17 +322,20 @@ class FastPMStep:
         be captured in a graph); returns the delta_k field."""
         ws = self.workspace()
         layout, X1, rho = self.prepare_force(state, smoothing=None)
-        with self._mark("r2c"):
-            delta_k = rho.r2c(out=ws["delta_k"])
         F = state.F
-        ops = self.force_ops()
         if self.stencil:
+            # delta_k = r2c(rho) and phi = c2r(common * delta_k), sharing the x pass
             phi = ws["force"][0]
-            with self._mark("c2r"):
-                c2r_many(delta_k, [self.force_common()], outs=[phi], work=ws["work"][:1])
+            with self._mark("fft"):
+                delta_k, _ = r2c_c2r_many(rho, [self.force_common()], kout=ws["delta_k"],
+                                          outs=[phi], work=ws["work"][:1])
             with self._mark("readout"):
                 self.pm.readout_tiled_grad4(phi, X1, F, rho=rho, rho_out=state.RHO)
-        elif self.solver.fused:
+            return delta_k
+        with self._mark("r2c"):
+            delta_k = rho.r2c(out=ws["delta_k"])
+        ops = self.force_ops()
+        if self.solver.fused:
             with self._mark("c2r"):
                 c2r_grad3(delta_k, self.force_common(), outs=ws["force"], work=ws["work"])
             with self._mark("readout"):

@@ -465,6 +465,31 @@ def c2r_grad3(cfield, common, outs=None, work=None, grad_order=1):
     return outs
 
 
+def r2c_c2r_many(rfield, ops, kout=None, outs=None, work=None):
+    """kout = rfield.r2c() and outs[c] = c2r(ops[c] * kout) in one call whose
+    last forward and first inverse pass share a kernel; returns (kout, outs)."""
+    pm = rfield.pm
+    n = len(ops)
+    if kout is None:
+        kout = ComplexField(pm)
+    if outs is None:
+        outs = [RealField(pm) for _ in range(n)]
+    if work is None:
+        work = [pm._complex_tensor() for _ in range(n)]
+    kops = (_lib.FpmKop * n)()
+    for c, o in enumerate(ops):
+        if o is None:
+            kops[c].factor = 1.0
+        else:
+            kops[c] = o.fpm_kop() if not isinstance(o, _lib.FpmKop) else o
+    tw = pm.twiddles()
+    _lib.call("fpm_fft_r2c_c2r", pm.dtype, rfield.value.data_ptr(), kout._t.data_ptr(), n, kops,
+              _lib.ptr_array([o.value.data_ptr() for o in outs]),
+              _lib.ptr_array([w.data_ptr() for w in work]), pm._desc, pm.pitch2,
+              tw[0].data_ptr(), tw[1].data_ptr(), tw[2].data_ptr(), _stream())
+    return kout, outs
+
+
 def c2r_many(cfield, ops, outs=None, work=None):
     """outs[c] = c2r(ops[c] * cfield) for up to six operators; the input is read
     once and left intact."""

@@ -166,6 +166,24 @@ int fpm_fft_c2r_f64(const void* in, int ncomp, const fpm_kop* ops, void* const* 
                     void* const* work, const fpm_mesh* mesh, int64_t pitch2,
                     const void* tw0, const void* tw1, const void* tw2, fpm_stream_t stream);
 
+/* Forward transform followed by multiplier-then-inverse transforms:
+ * kout = r2c(in) (normalised by 1/(n0 n1 n2), as fpm_fft_r2c) and
+ * outs[c] = c2r(ops[c](k) * kout) for ncomp (<= 6) outputs, with work[c]
+ * complex scratch meshes as in fpm_fft_c2r.  The last forward pass and the
+ * first inverse pass run along the same axis, so one kernel does both and
+ * the spectrum is read once less than with separate calls.  `in` is left
+ * intact.  Replaces the Force's rho.r2c (fastpm/core.py:205) followed by
+ * pot_k = delta_k.apply(laplace).apply(longrange) and its c2r
+ * (fastpm/force/gravity.py:14-22). */
+int fpm_fft_r2c_c2r_f32(const void* in, void* kout, int ncomp, const fpm_kop* ops,
+                        void* const* outs, void* const* work, const fpm_mesh* mesh,
+                        int64_t pitch2, const void* tw0, const void* tw1, const void* tw2,
+                        fpm_stream_t stream);
+int fpm_fft_r2c_c2r_f64(const void* in, void* kout, int ncomp, const fpm_kop* ops,
+                        void* const* outs, void* const* work, const fpm_mesh* mesh,
+                        int64_t pitch2, const void* tw0, const void* tw1, const void* tw2,
+                        fpm_stream_t stream);
+
 /* The three gradient components of one field in one call:
  * outs[d] = c2r(i a_d(k) * common(k) * in), d = 0, 1, 2, where `common` has
  * ngrad == 0 and its grad_order selects a_d.  Uses that a_d depends on k_d

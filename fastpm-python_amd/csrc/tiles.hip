@@ -52,10 +52,19 @@ static TileGeom tile_geom(const fpm_mesh* m) {
   return g;
 }
 
+// Binned particle record: the window's offsets within its first cell (CIC:
+// the fraction f, TSC: f or f - 1 around the nearest point) and the window
+// origin local to the particle's tile (4 bits per axis).
+struct PRec {
+  float t[3];
+  uint32_t cells;  // lx | ly << 4 | lz << 8
+};
+
 struct Workspace {
   int32_t* key;        // sub-bin of each particle
   int32_t* rank;       // its slot within the sub-bin
-  int32_t* list;
+  int32_t* list;       // particle index per slot
+  PRec* rec;           // particle record per slot
   int32_t* count;
   int32_t* offs;       // nbins + 1
   int32_t* misplaced;  // particles found outside their binned tile
@@ -71,39 +80,41 @@ constexpr int kScanBlock = kScanThreads * kScanPer;  // sub-bins per scan workgr
 
 static int64_t scan_blocks(const TileGeom& g) { return (g.nbins + kScanBlock - 1) / kScanBlock; }
 
-// Byte offsets of the workspace sections: key, rank, list, count, offs,
-// misplaced, btot; off[7] is the total size.
-static void ws_offsets(const TileGeom& g, int64_t n, int64_t off[8]) {
-  const int64_t sizes[7] = {n * 4, n * 4, n * 4, g.nbins * 4, (g.nbins + 1) * 4, 4,
-                            scan_blocks(g) * 4};
+// Byte offsets of the workspace sections: key, rank, list, rec, count, offs,
+// misplaced, btot; off[8] is the total size.
+static void ws_offsets(const TileGeom& g, int64_t n, int64_t off[9]) {
+  const int64_t sizes[8] = {n * 4,       n * 4, n * 4, n * (int64_t)sizeof(PRec),
+                            g.nbins * 4, (g.nbins + 1) * 4, 4, scan_blocks(g) * 4};
   int64_t o = 0;
-  for (int i = 0; i < 7; ++i) {
+  for (int i = 0; i < 8; ++i) {
     off[i] = o;
     o += align256(sizes[i]);
   }
-  off[7] = o;
+  off[8] = o;
 }
 
 static int64_t layout_ws(const TileGeom& g, int64_t n, char* base, Workspace* w) {
-  int64_t off[8];
+  int64_t off[9];
   ws_offsets(g, n, off);
   if (w) {
     w->key = reinterpret_cast<int32_t*>(base + off[0]);
     w->rank = reinterpret_cast<int32_t*>(base + off[1]);
     w->list = reinterpret_cast<int32_t*>(base + off[2]);
-    w->count = reinterpret_cast<int32_t*>(base + off[3]);
-    w->offs = reinterpret_cast<int32_t*>(base + off[4]);
-    w->misplaced = reinterpret_cast<int32_t*>(base + off[5]);
-    w->btot = reinterpret_cast<int32_t*>(base + off[6]);
-    w->bytes = off[7];
+    w->rec = reinterpret_cast<PRec*>(base + off[3]);
+    w->count = reinterpret_cast<int32_t*>(base + off[4]);
+    w->offs = reinterpret_cast<int32_t*>(base + off[5]);
+    w->misplaced = reinterpret_cast<int32_t*>(base + off[6]);
+    w->btot = reinterpret_cast<int32_t*>(base + off[7]);
+    w->bytes = off[8];
   }
-  return off[7];
+  return off[8];
 }
 
-// Window origin (wrapped) and weights, identical to the plain paint/readout.
+// Window origin (wrapped) and the offset t it is weighted by (CIC: the
+// fraction f; TSC: f, or f - 1 when the nearest point is the next cell).
 template <typename T, int ORDER>
-__device__ __forceinline__ void window_of(const PartDesc<T>& P, int64_t p, const int nm[3],
-                                          int i0[3], T w[3][ORDER]) {
+__device__ __forceinline__ void window_t(const PartDesc<T>& P, int64_t p, const int nm[3],
+                                         int i0[3], T t[3]) {
   int64_t cell[3];
   T f[3];
   particle_cell(P, p, cell, f);
@@ -112,19 +123,50 @@ __device__ __forceinline__ void window_of(const PartDesc<T>& P, int64_t p, const
     int64_t o;
     if constexpr (ORDER == 2) {
       o = cell[d];
-      w[d][0] = (T)1 - f[d];
-      w[d][1] = f[d];
+      t[d] = f[d];
     } else {
       const bool up = f[d] >= (T)0.5;
-      const T dd = up ? f[d] - (T)1 : f[d];
+      t[d] = up ? f[d] - (T)1 : f[d];
       o = cell[d] + (up ? 1 : 0) - 1;
+    }
+    i0[d] = (int)wrapi(o, nm[d]);
+  }
+}
+
+// Window weights from t, identical to the plain paint/readout.
+template <typename T, int ORDER>
+__device__ __forceinline__ void weights_t(const T t[3], T w[3][ORDER]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if constexpr (ORDER == 2) {
+      w[d][0] = (T)1 - t[d];
+      w[d][1] = t[d];
+    } else {
+      const T dd = t[d];
       const T a = (T)0.5 - dd, b = (T)0.5 + dd;
       w[d][0] = (T)0.5 * a * a;
       w[d][1] = (T)0.75 - dd * dd;
       w[d][2] = (T)0.5 * b * b;
     }
-    i0[d] = (int)wrapi(o, nm[d]);
   }
+}
+
+template <typename T, int ORDER>
+__device__ __forceinline__ void window_of(const PartDesc<T>& P, int64_t p, const int nm[3],
+                                          int i0[3], T w[3][ORDER]) {
+  T t[3];
+  window_t<T, ORDER>(P, p, nm, i0, t);
+  weights_t<T, ORDER>(t, w);
+}
+
+// A record's window: origin local to its tile and weights.
+template <typename T, int ORDER>
+__device__ __forceinline__ void window_rec(const PRec& r, int l[3], T w[3][ORDER]) {
+  l[0] = (int)(r.cells & 15u);
+  l[1] = (int)((r.cells >> 4) & 15u);
+  l[2] = (int)((r.cells >> 8) & 15u);
+  const T t[3] = {(T)r.t[0], (T)r.t[1], (T)r.t[2]};
+  weights_t<T, ORDER>(t, w);
 }
 
 template <int ORDER>
@@ -269,49 +311,64 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int6
   if (blockIdx.x == gridDim.x - 1 && tid == 0) w.offs[nbins] = base + w.btot[blockIdx.x];
 }
 
-// Each particle goes to the start of its sub-bin plus its slot from the count.
-__global__ void k_bin_scatter(int64_t n, Workspace w, BinOrder o) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    w.list[w.offs[w.key[i]] + w.rank[i]] = (int32_t)bin_particle(o, i);
+// Each particle goes to the start of its sub-bin plus its slot from the count,
+// with its record (window offsets and tile-local origin), so paint and
+// readout read their tile's particles contiguously and need no positions.
+template <typename T, int ORDER>
+__global__ void k_bin_scatter(PartDesc<T> P, TileGeom g, Workspace w, BinOrder o) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = bin_particle(o, i);
+    const int32_t key = w.key[i];
+    const int32_t slot = w.offs[key] + w.rank[i];
+    int i0[3];
+    T t[3];
+    window_t<T, ORDER>(P, p, g.nm, i0, t);
+    PRec r;
+    r.t[0] = (float)t[0];
+    r.t[1] = (float)t[1];
+    r.t[2] = (float)t[2];
+    r.cells = (uint32_t)(i0[0] % TILE) | (uint32_t)(i0[1] % TILE) << 4 |
+              (uint32_t)(i0[2] % TILE) << 8;
+    w.list[slot] = (int32_t)p;
+    w.rec[slot] = r;
+  }
 }
 
 // ---------------------------------------------------------------- paint
-// Deposits of the particle at list slot s that land in the target tile at lo.
-// Every window point is wrapped and tested against the tile, so a deposit
-// never leaves the tile's LDS accumulator.
+// Deposits of one record into the target tile.  The record's origin is local
+// to its source tile, which lies o_d tiles below the target along axis d
+// (o_d in {0, 1}); window points outside the target are skipped, and on an
+// axis with a single tile they wrap onto it.
 template <typename T, int ORDER>
-__device__ __forceinline__ void deposit_one(const PartDesc<T>& P, const TileGeom& g,
-                                            const Workspace& w, int32_t s, const int lo[3],
+__device__ __forceinline__ void deposit_rec(const PRec& rec, int o, const TileGeom& g,
                                             unsigned long long* acc, double mass_fx) {
-  const int64_t p = w.list[s];
-  int i0[3];
+  int l[3];
   T wt[3][ORDER];
-  window_of<T, ORDER>(P, p, g.nm, i0, wt);
-  int lx[ORDER], ly[ORDER], lz[ORDER];
+  window_rec<T, ORDER>(rec, l, wt);
+  const int od[3] = {(o >> 2) & 1, (o >> 1) & 1, o & 1};
+  int u[3][ORDER];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int a = 0; a < ORDER; ++a) {
+      int v = l[d] + a - TILE * od[d];
+      if (g.nt[d] == 1 && v >= TILE) v -= TILE;
+      u[d][a] = v;
+    }
 #pragma unroll
   for (int a = 0; a < ORDER; ++a) {
-    int cx = i0[0] + a, cy = i0[1] + a, cz = i0[2] + a;
-    if (cx >= g.nm[0]) cx -= g.nm[0];
-    if (cy >= g.nm[1]) cy -= g.nm[1];
-    if (cz >= g.nm[2]) cz -= g.nm[2];
-    lx[a] = cx - lo[0];
-    ly[a] = cy - lo[1];
-    lz[a] = cz - lo[2];
-  }
-#pragma unroll
-  for (int a = 0; a < ORDER; ++a) {
-    if ((unsigned)lx[a] >= (unsigned)TILE) continue;
+    if ((unsigned)u[0][a] >= (unsigned)TILE) continue;
 #pragma unroll
     for (int c = 0; c < ORDER; ++c) {
-      if ((unsigned)ly[c] >= (unsigned)TILE) continue;
+      if ((unsigned)u[1][c] >= (unsigned)TILE) continue;
       const T wac = wt[0][a] * wt[1][c];
 #pragma unroll
       for (int d = 0; d < ORDER; ++d) {
-        if ((unsigned)lz[d] >= (unsigned)TILE) continue;
+        if ((unsigned)u[2][d] >= (unsigned)TILE) continue;
         const double v = (double)(wac * wt[2][d]) * mass_fx;
         const long long q = __double2ll_rn(v);
-        atomicAdd(&acc[(lx[a] * TILE + ly[c]) * TILE + lz[d]], (unsigned long long)q);
+        atomicAdd(&acc[(u[0][a] * TILE + u[1][c]) * TILE + u[2][d]], (unsigned long long)q);
       }
     }
   }
@@ -328,6 +385,7 @@ __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, Til
   __shared__ unsigned long long acc[NB];
   __shared__ int32_t rstart[64];
   __shared__ int32_t rpre[65];
+  __shared__ int32_t rdir[64];
   const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
   int lo[3];
   tile_coords(t, g, lo);
@@ -360,6 +418,7 @@ __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, Til
       if (lane >= d) inc += v;
     }
     rstart[lane] = b;
+    rdir[lane] = o;
     rpre[lane + 1] = inc;
     if (lane == 0) rpre[0] = 0;
   }
@@ -371,7 +430,7 @@ __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, Til
 #pragma unroll
     for (int step = 32; step >= 1; step >>= 1)
       if (rpre[r + step] <= f) r += step;
-    deposit_one<T, ORDER>(P, g, w, rstart[r] + (f - rpre[r]), lo, acc, mass_fx);
+    deposit_rec<T, ORDER>(w.rec[rstart[r] + (f - rpre[r])], rdir[r], g, acc, mass_fx);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < NB; i += kTileThreads) {
@@ -491,20 +550,10 @@ struct BoxRegs {
 
 // Interpolation of one particle from the staged box.
 template <typename T, int ORDER, int NC, typename VC>
-__device__ __forceinline__ void gather_one(const VC* box, const Workspace& w,
-                                           const TileRead<T>& R, int64_t p, const int i0[3],
-                                           const T wt[3][ORDER], const int lo[3]) {
+__device__ __forceinline__ void gather_one(const VC* box, const TileRead<T>& R, int64_t p,
+                                           const int l[3], const T wt[3][ORDER]) {
   constexpr int TB = TILE + ORDER - 1;
-  int lx = i0[0] - lo[0], ly = i0[1] - lo[1], lz = i0[2] - lo[2];
-  if ((unsigned)lx >= (unsigned)TILE || (unsigned)ly >= (unsigned)TILE ||
-      (unsigned)lz >= (unsigned)TILE) {
-    // placed in another tile when binned: keep the LDS access in bounds and
-    // report it (never expected: binning uses the same arithmetic)
-    atomicAdd(w.misplaced, 1);
-    lx = lx < 0 ? 0 : (lx >= TILE ? TILE - 1 : lx);
-    ly = ly < 0 ? 0 : (ly >= TILE ? TILE - 1 : ly);
-    lz = lz < 0 ? 0 : (lz >= TILE ? TILE - 1 : lz);
-  }
+  const int lx = l[0], ly = l[1], lz = l[2];
   T acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = (T)0;
@@ -549,22 +598,25 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
   const int32_t s0 = b + threadIdx.x, s1 = s0 + kTileThreads;
   const int64_t p0 = s0 < e ? w.list[s0] : 0;
   const int64_t p1 = s1 < e ? w.list[s1] : 0;
+  const PRec r0 = s0 < e ? w.rec[s0] : PRec{};
+  const PRec r1 = s1 < e ? w.rec[s1] : PRec{};
   BoxRegs<T, ORDER, NC> br;
   br.load(g, R, lo);
-  int f0[3], f1[3];
-  T wt0[3][ORDER], wt1[3][ORDER];
-  if (s0 < e) window_of<T, ORDER>(P, p0, g.nm, f0, wt0);
-  if (s1 < e) window_of<T, ORDER>(P, p1, g.nm, f1, wt1);
   br.store(box);
   __syncthreads();
-  if (s0 < e) gather_one<T, ORDER, NC>(box, w, R, p0, f0, wt0, lo);
-  if (s1 < e) gather_one<T, ORDER, NC>(box, w, R, p1, f1, wt1, lo);
+  int l[3];
+  T wt[3][ORDER];
+  if (s0 < e) {
+    window_rec<T, ORDER>(r0, l, wt);
+    gather_one<T, ORDER, NC>(box, R, p0, l, wt);
+  }
+  if (s1 < e) {
+    window_rec<T, ORDER>(r1, l, wt);
+    gather_one<T, ORDER, NC>(box, R, p1, l, wt);
+  }
   for (int32_t s = s1 + kTileThreads; s < e; s += kTileThreads) {
-    const int64_t p = w.list[s];
-    int i0[3];
-    T wt[3][ORDER];
-    window_of<T, ORDER>(P, p, g.nm, i0, wt);
-    gather_one<T, ORDER, NC>(box, w, R, p, i0, wt, lo);
+    window_rec<T, ORDER>(w.rec[s], l, wt);
+    gather_one<T, ORDER, NC>(box, R, (int64_t)w.list[s], l, wt);
   }
 }
 
@@ -657,20 +709,12 @@ struct ScalarBox {
 };
 
 template <typename T, int ORDER>
-__device__ __forceinline__ void grad_one(const T* rbox, const T* pbox, const Workspace& w,
-                                         const GradRead<T>& G, int64_t p, const int i0[3],
-                                         const T wt[3][ORDER], const int lo[3]) {
+__device__ __forceinline__ void grad_one(const T* rbox, const T* pbox, const GradRead<T>& G,
+                                         int64_t p, const int l[3], const T wt[3][ORDER]) {
   constexpr int TR = TILE + ORDER - 1;   // density box side
   constexpr int TP = TILE + ORDER + 3;   // potential box side (2 more cells each way)
   constexpr int NLN = ORDER + 4;         // stencil line: window points -2 .. +2
-  int lx = i0[0] - lo[0], ly = i0[1] - lo[1], lz = i0[2] - lo[2];
-  if ((unsigned)lx >= (unsigned)TILE || (unsigned)ly >= (unsigned)TILE ||
-      (unsigned)lz >= (unsigned)TILE) {
-    atomicAdd(w.misplaced, 1);
-    lx = lx < 0 ? 0 : (lx >= TILE ? TILE - 1 : lx);
-    ly = ly < 0 ? 0 : (ly >= TILE ? TILE - 1 : ly);
-    lz = lz < 0 ? 0 : (lz >= TILE ? TILE - 1 : lz);
-  }
+  const int lx = l[0], ly = l[1], lz = l[2];
   if (G.rho_out) {
     T acc = (T)0;
 #pragma unroll
@@ -729,25 +773,28 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_grad_tiles(PartDesc<T>
   const int32_t s0 = b + threadIdx.x, s1 = s0 + kTileThreads;
   const int64_t p0 = s0 < e ? w.list[s0] : 0;
   const int64_t p1 = s1 < e ? w.list[s1] : 0;
+  const PRec r0 = s0 < e ? w.rec[s0] : PRec{};
+  const PRec r1 = s1 < e ? w.rec[s1] : PRec{};
   ScalarBox<T, 2, ORDER + 1> pb;
   pb.load(g, G.phi, lo);
   ScalarBox<T, 0, ORDER - 1> rb;
   if (G.rho_out) rb.load(g, G.rho, lo);
-  int f0[3], f1[3];
-  T wt0[3][ORDER], wt1[3][ORDER];
-  if (s0 < e) window_of<T, ORDER>(P, p0, g.nm, f0, wt0);
-  if (s1 < e) window_of<T, ORDER>(P, p1, g.nm, f1, wt1);
   pb.store(pbox);
   if (G.rho_out) rb.store(rbox);
   __syncthreads();
-  if (s0 < e) grad_one<T, ORDER>(rbox, pbox, w, G, p0, f0, wt0, lo);
-  if (s1 < e) grad_one<T, ORDER>(rbox, pbox, w, G, p1, f1, wt1, lo);
+  int l[3];
+  T wt[3][ORDER];
+  if (s0 < e) {
+    window_rec<T, ORDER>(r0, l, wt);
+    grad_one<T, ORDER>(rbox, pbox, G, p0, l, wt);
+  }
+  if (s1 < e) {
+    window_rec<T, ORDER>(r1, l, wt);
+    grad_one<T, ORDER>(rbox, pbox, G, p1, l, wt);
+  }
   for (int32_t s = s1 + kTileThreads; s < e; s += kTileThreads) {
-    const int64_t p = w.list[s];
-    int i0[3];
-    T wt[3][ORDER];
-    window_of<T, ORDER>(P, p, g.nm, i0, wt);
-    grad_one<T, ORDER>(rbox, pbox, w, G, p, i0, wt, lo);
+    window_rec<T, ORDER>(w.rec[s], l, wt);
+    grad_one<T, ORDER>(rbox, pbox, G, (int64_t)w.list[s], l, wt);
   }
 }
 
@@ -850,8 +897,12 @@ static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, voi
   const unsigned nsb = (unsigned)scan_blocks(g);
   hipLaunchKernelGGL(k_bin_scan_local, dim3(nsb), dim3(kScanThreads), 0, st, w, g.nbins);
   hipLaunchKernelGGL(k_bin_scan_add, dim3(nsb), dim3(kScanThreads), 0, st, w, g.nbins);
-  if (P.n > 0)
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_parts(P.n)), dim3(256), 0, st, P.n, w, o);
+  if (P.n > 0) {
+    if (order == 2)
+      hipLaunchKernelGGL((k_bin_scatter<T, 2>), dim3(grid_parts(P.n)), dim3(256), 0, st, P, g, w, o);
+    else
+      hipLaunchKernelGGL((k_bin_scatter<T, 3>), dim3(grid_parts(P.n)), dim3(256), 0, st, P, g, w, o);
+  }
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }
@@ -983,9 +1034,9 @@ int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n) {
 
 int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n) {
   if (!tiles_ok(mesh) || (order != 2 && order != 3) || n < 0) return -1;
-  int64_t off[8];
+  int64_t off[9];
   ws_offsets(tile_geom(mesh), n, off);
-  return off[5];
+  return off[6];
 }
 
 int fpm_bin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,

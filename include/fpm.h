@@ -108,15 +108,18 @@ int fpm_readout_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, 
 
 /* Tile-binned paint and readout (float32 storage, meshes whose sides are
  * multiples of 16).  The mesh is cut into 16^3-cell tiles; fpm_bin_particles
- * sorts particle indices into per-tile lists held in the caller's workspace
- * (fpm_tile_workspace_size bytes).  fpm_paint_tiled and fpm_readout_tiled use
- * those lists, so they must follow a bin call with the same particles, mesh
- * and order.  The tiled paint accumulates in 64-bit fixed point, is bitwise
- * reproducible, overwrites `out` (hold=False) and needs no clearing pass.
- * Same contracts as fpm_paint / fpm_readout (fastpm/core.py:191,203;
- * fastpm/force/gravity.py:24).  fpm_tile_misplaced_offset gives the byte
- * offset in the workspace of an int32 counter of particles that paint or
- * readout found outside their binned tile (always expected to stay 0). */
+ * sorts the particles into per-tile lists held in the caller's workspace
+ * (fpm_tile_workspace_size bytes): each slot holds the particle index and a
+ * record of its window (offsets within the first cell, tile-local origin).
+ * fpm_paint_tiled and fpm_readout_tiled read the lists only -- the positions
+ * as of the bin call are used -- so they must follow a bin call with the same
+ * particles, mesh and order.  The tiled paint accumulates in 64-bit fixed
+ * point, is bitwise reproducible, overwrites `out` (hold=False) and needs no
+ * clearing pass.  Same contracts as fpm_paint / fpm_readout
+ * (fastpm/core.py:191,203; fastpm/force/gravity.py:24).
+ * fpm_tile_misplaced_offset gives the byte offset in the workspace of an int32
+ * counter of particles found outside their binned tile (zeroed by each bin
+ * call; the record-based kernels cannot misplace a particle, so it stays 0). */
 int fpm_tiles_supported(const fpm_mesh* mesh);
 int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n);
 int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n);

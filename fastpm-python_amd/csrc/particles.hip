@@ -35,6 +35,27 @@ __global__ void k_axpy_table(T* y, const T* x, int64_t n, const double* table,
     y[i] += a * x[i];
 }
 
+// y = a y + b x (the damped kick of the glass generator).
+template <typename T>
+__global__ void k_axpby(T* y, const T* x, int64_t n, T a, T b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * y[i] + b * x[i];
+}
+
+// s = (s + a p) mod L_d on [N, 3] arrays, the result in [0, L_d).
+template <typename T>
+__global__ void k_drift_wrap(T* s, const T* p, int64_t n3, T a, T L0, T L1, T L2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int d = (int)(i % 3);
+    const T L = d == 0 ? L0 : (d == 1 ? L1 : L2);
+    T v = s[i] + a * p[i];
+    v = v - L * floor(v / L);
+    s[i] = v < L ? v : v - L;  // v / L rounding up to 1 leaves v == L
+  }
+}
+
 __global__ void k_counter_add(int32_t* counter, int32_t v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += v;
 }
@@ -100,6 +121,30 @@ static int axpy_table_impl(void* y, const void* x, int64_t n, const double* tabl
 }
 
 template <typename T>
+static int axpby_impl(void* y, const void* x, int64_t n, double a, double b, hipStream_t st) {
+  if (n < 0 || (n > 0 && (!y || !x))) return FPM_ERR_ARG;
+  if (n == 0) return FPM_OK;
+  hipLaunchKernelGGL(k_axpby<T>, dim3(grid_elems(n)), dim3(256), 0, st, static_cast<T*>(y),
+                     static_cast<const T*>(x), n, (T)a, (T)b);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
+static int drift_wrap_impl(void* s, const void* p, int64_t n, double a, const double* box,
+                           hipStream_t st) {
+  if (n < 0 || !box || (n > 0 && (!s || !p))) return FPM_ERR_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (!(box[d] > 0)) return FPM_ERR_ARG;
+  if (n == 0) return FPM_OK;
+  hipLaunchKernelGGL(k_drift_wrap<T>, dim3(grid_elems(3 * n)), dim3(256), 0, st,
+                     static_cast<T*>(s), static_cast<const T*>(p), 3 * n, (T)a, (T)box[0],
+                     (T)box[1], (T)box[2]);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
 static int combine_impl(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,
                         const double* c, hipStream_t st) {
   if (n < 0 || !c || (n > 0 && (!dx1 || !S || !P || !F))) return FPM_ERR_ARG;
@@ -148,6 +193,20 @@ int fpm_axpy_table_f32(void* y, const void* x, int64_t n, const double* table,
 int fpm_axpy_table_f64(void* y, const void* x, int64_t n, const double* table,
                        const int32_t* counter, int rows, int stride, int slot, fpm_stream_t s) {
   return axpy_table_impl<double>(y, x, n, table, counter, rows, stride, slot, (hipStream_t)s);
+}
+int fpm_axpby_f32(void* y, const void* x, int64_t n, double a, double b, fpm_stream_t s) {
+  return axpby_impl<float>(y, x, n, a, b, (hipStream_t)s);
+}
+int fpm_axpby_f64(void* y, const void* x, int64_t n, double a, double b, fpm_stream_t s) {
+  return axpby_impl<double>(y, x, n, a, b, (hipStream_t)s);
+}
+int fpm_drift_wrap_f32(void* S, const void* P, int64_t n, double a, const double* box,
+                       fpm_stream_t s) {
+  return drift_wrap_impl<float>(S, P, n, a, box, (hipStream_t)s);
+}
+int fpm_drift_wrap_f64(void* S, const void* P, int64_t n, double a, const double* box,
+                       fpm_stream_t s) {
+  return drift_wrap_impl<double>(S, P, n, a, box, (hipStream_t)s);
 }
 int fpm_counter_add(int32_t* counter, int32_t v, fpm_stream_t s) {
   if (!counter) return FPM_ERR_ARG;

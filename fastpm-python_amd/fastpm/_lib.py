@@ -58,6 +58,8 @@ _TYPED = {
     "fpm_paint": (_int, [_PP, _PM, _int, _dbl, _vp, _int, _vp]),
     "fpm_readout": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _vp, _vp]),
     "fpm_fft_r2c": (_int, [_vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
+    "fpm_axpby": (_int, [_vp, _vp, _i64, _dbl, _dbl, _vp]),
+    "fpm_drift_wrap": (_int, [_vp, _vp, _i64, _dbl, _vp, _vp]),
     "fpm_fft_c2r": (_int, [_vp, _int, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
     "fpm_fft_r2c_c2r": (_int, [_vp, _vp, _int, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
     "fpm_fft_c2r_grad3": (_int, [_vp, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),

@@ -237,6 +237,18 @@ int fpm_lpt2_source_f64(const void* const* phi, void* out, int64_t ncells, fpm_s
 int fpm_axpy_f32(void* y, const void* x, int64_t n, double a, fpm_stream_t stream);
 int fpm_axpy_f64(void* y, const void* x, int64_t n, double a, fpm_stream_t stream);
 
+/* y = a y + b x over n elements: the damped kick of the glass generator,
+ * P = P + (af - ai) (F - P) (fastpm/glass.py:89-93). */
+int fpm_axpby_f32(void* y, const void* x, int64_t n, double a, double b, fpm_stream_t stream);
+int fpm_axpby_f64(void* y, const void* x, int64_t n, double a, double b, fpm_stream_t stream);
+
+/* S = (S + a P) mod box_d on [n][3] arrays, results in [0, box_d): the
+ * glass generator's drift (fastpm/glass.py:95-98). */
+int fpm_drift_wrap_f32(void* S, const void* P, int64_t n, double a, const double* box,
+                       fpm_stream_t stream);
+int fpm_drift_wrap_f64(void* S, const void* P, int64_t n, double a, const double* box,
+                       fpm_stream_t stream);
+
 /* y += table[counter[0] * stride + slot] * x, with the FP64 coefficient table
  * ([rows][stride]) and the step counter in device memory, so a captured graph
  * can replay the kick/drift updates of every step (fastpm/core.py:168-180

@@ -564,6 +564,54 @@ class Solver:
 
 
 # ----------------------------------------------------------------------------
+# glass initial conditions (fastpm/glass.py:8-118)
+def glass_stages(N=3):
+    """fastpm/glass.py:33: N + 1/4 periods of 2 pi, 8 stages per period."""
+    return np.linspace(0, np.pi * 2 * (N + 0.25), int(8 * (N + 0.25) + 1))
+
+
+def glass(nc, box, seed, B=2, spread=3.0, N=3, order=2, stages=None):
+    """Positions of a glass from the uniform grid of side nc (fastpm/glass.py:
+    8-118): random start from numpy's RandomState(seed), then the leapfrog
+    with the damped kick P += (af - ai)(F - P) (glass.py:89-93), the periodic
+    drift S = (S + (af - ai) P) mod L (glass.py:95-98) and the inverted PM
+    force F = -longrange(X, delta_k, 0, 1) (glass.py:100-118); returns X mod L."""
+    nmesh = (nc,) * 3
+    fnmesh = (nc * B,) * 3
+    boxv = (float(box),) * 3
+    Q = lattice(nmesh, boxv)
+    n = len(Q)
+    rng = np.random.RandomState(seed)
+    nbar = 1 / (float(box) ** 3 / n)
+    S = spread * (rng.uniform(size=(n, 3)) - 0.5) * (nbar ** -0.3333333)
+    P = np.zeros_like(S)
+    F = np.zeros_like(S)
+    L = float(box)
+
+    def force(S):
+        X = Q + S
+        nb = n / np.prod(fnmesh)
+        rho = paint(X, fnmesh, boxv, order) / nb
+        pot = r2c(rho) * k_laplace(fnmesh, boxv)
+        out = np.empty_like(S)
+        for d in range(3):
+            fd = c2r(pot * k_gradient(d, fnmesh, boxv, 1), fnmesh)
+            out[:, d] = -readout(fd, X, boxv, order)
+        return out
+
+    stages = glass_stages(N) if stages is None else stages
+    for action, ai, ac, af in leapfrog(stages):
+        fac = af - ai
+        if action == "K":
+            P = P + fac * (F - P)
+        elif action == "D":
+            S = (S + fac * P) % L
+        else:
+            F = force(S)
+    return (Q + S) % L
+
+
+# ----------------------------------------------------------------------------
 # binned spectra (utils.py:77-164 semantics, linear bins)
 def power_bins(f1, nmesh, box, nbins, kmin, dk, f2=None):
     """Sums per bin [w, w|k|, w Re(f1 f2*), w|f1|^2, w|f2|^2]; Hermitian weight 2

@@ -351,3 +351,30 @@ def test_power_brute_force():
     allb = O.power_bins(f1, nmesh, box, 1, 0.0, 100.0)
     full = np.fft.fftn(np.fft.irfftn(f1, s=nmesh, axes=(0, 1, 2)))
     assert allb[0, 3] == pytest.approx((np.abs(full) ** 2).sum() - abs(full[0, 0, 0]) ** 2, rel=1e-12)
+
+
+def _glass_power_ratio(X, nc, box):
+    """P(k) of a point set over that of uniform random points, on a 2nc mesh."""
+    nm = (2 * nc,) * 3
+    def pk(Y):
+        rho = O.paint(Y, nm, (box,) * 3, 2) / (len(Y) / np.prod(nm))
+        return O.power_spectrum(O.r2c(rho), nm, (box,) * 3, nbins=8)[1]
+    rng = np.random.default_rng(1)
+    return pk(X) / pk(rng.uniform(0, box, size=X.shape))
+
+
+def test_glass_oracle():
+    """Glass oracle (fastpm/glass.py): as many points as grid cells
+    (test_glass.py), all in [0, L), large-scale power far below a random
+    set's (a repulsive, damped relaxation; an attractive force or a missing
+    damping term fails this), the stage list of glass.py:33, and the
+    unperturbed lattice as an exact fixed point of the dynamics."""
+    X = O.glass(8, 128.0, 123)
+    assert X.shape == (8 ** 3, 3)
+    assert X.min() >= 0 and X.max() < 128.0
+    ratio = _glass_power_ratio(X, 8, 128.0)
+    assert np.all(ratio[1:4] < 0.02), ratio
+    st = O.glass_stages(3)
+    assert len(st) == 27 and st[0] == 0 and abs(st[-1] - 6.5 * np.pi) < 1e-12
+    X0 = O.glass(8, 128.0, 123, spread=0.0)
+    assert np.array_equal(X0, O.lattice((8,) * 3, (128.0,) * 3) % 128.0)

@@ -61,8 +61,7 @@ struct PRec {
 };
 
 struct Workspace {
-  int32_t* key;        // sub-bin of each particle
-  int32_t* rank;       // its slot within the sub-bin
+  int32_t* rank;       // each particle's slot within its sub-bin (thread order)
   int32_t* list;       // particle index per slot
   PRec* rec;           // particle record per slot
   int32_t* count;
@@ -80,34 +79,33 @@ constexpr int kScanBlock = kScanThreads * kScanPer;  // sub-bins per scan workgr
 
 static int64_t scan_blocks(const TileGeom& g) { return (g.nbins + kScanBlock - 1) / kScanBlock; }
 
-// Byte offsets of the workspace sections: key, rank, list, rec, count, offs,
-// misplaced, btot; off[8] is the total size.
-static void ws_offsets(const TileGeom& g, int64_t n, int64_t off[9]) {
-  const int64_t sizes[8] = {n * 4,       n * 4, n * 4, n * (int64_t)sizeof(PRec),
-                            g.nbins * 4, (g.nbins + 1) * 4, 4, scan_blocks(g) * 4};
+// Byte offsets of the workspace sections: rank, list, rec, count, offs,
+// misplaced, btot; off[7] is the total size.
+static void ws_offsets(const TileGeom& g, int64_t n, int64_t off[8]) {
+  const int64_t sizes[7] = {n * 4, n * 4, n * (int64_t)sizeof(PRec), g.nbins * 4,
+                            (g.nbins + 1) * 4, 4, scan_blocks(g) * 4};
   int64_t o = 0;
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < 7; ++i) {
     off[i] = o;
     o += align256(sizes[i]);
   }
-  off[8] = o;
+  off[7] = o;
 }
 
 static int64_t layout_ws(const TileGeom& g, int64_t n, char* base, Workspace* w) {
-  int64_t off[9];
+  int64_t off[8];
   ws_offsets(g, n, off);
   if (w) {
-    w->key = reinterpret_cast<int32_t*>(base + off[0]);
-    w->rank = reinterpret_cast<int32_t*>(base + off[1]);
-    w->list = reinterpret_cast<int32_t*>(base + off[2]);
-    w->rec = reinterpret_cast<PRec*>(base + off[3]);
-    w->count = reinterpret_cast<int32_t*>(base + off[4]);
-    w->offs = reinterpret_cast<int32_t*>(base + off[5]);
-    w->misplaced = reinterpret_cast<int32_t*>(base + off[6]);
-    w->btot = reinterpret_cast<int32_t*>(base + off[7]);
-    w->bytes = off[8];
+    w->rank = reinterpret_cast<int32_t*>(base + off[0]);
+    w->list = reinterpret_cast<int32_t*>(base + off[1]);
+    w->rec = reinterpret_cast<PRec*>(base + off[2]);
+    w->count = reinterpret_cast<int32_t*>(base + off[3]);
+    w->offs = reinterpret_cast<int32_t*>(base + off[4]);
+    w->misplaced = reinterpret_cast<int32_t*>(base + off[5]);
+    w->btot = reinterpret_cast<int32_t*>(base + off[6]);
+    w->bytes = off[7];
   }
-  return off[8];
+  return off[7];
 }
 
 // Window origin (wrapped) and the offset t it is weighted by (CIC: the
@@ -245,10 +243,7 @@ __global__ void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w, BinOrder o) 
       active &= ~same;
     }
     const int32_t r0 = __shfl(old, myleader);
-    if (valid) {
-      w.key[i] = t;
-      w.rank[i] = r0 + myrank;
-    }
+    if (valid) w.rank[i] = r0 + myrank;
   }
 }
 
@@ -311,27 +306,33 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int6
   if (blockIdx.x == gridDim.x - 1 && tid == 0) w.offs[nbins] = base + w.btot[blockIdx.x];
 }
 
-// Each particle goes to the start of its sub-bin plus its slot from the count,
-// with its record (window offsets and tile-local origin), so paint and
-// readout read their tile's particles contiguously and need no positions.
+// Each particle goes to the start of its sub-bin (recomputed from its
+// position with the count's arithmetic) plus its slot from the count, with its
+// record (window offsets and tile-local origin), so paint and readout read
+// their tile's particles contiguously and need no positions.
 template <typename T, int ORDER>
 __global__ void k_bin_scatter(PartDesc<T> P, TileGeom g, Workspace w, BinOrder o) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P.n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t p = bin_particle(o, i);
-    const int32_t key = w.key[i];
-    const int32_t slot = w.offs[key] + w.rank[i];
+    const int32_t rank = w.rank[i];
     int i0[3];
     T t[3];
     window_t<T, ORDER>(P, p, g.nm, i0, t);
+    const int32_t slot = w.offs[bin_of<ORDER>(i0, g)] + rank;
     PRec r;
     r.t[0] = (float)t[0];
     r.t[1] = (float)t[1];
     r.t[2] = (float)t[2];
     r.cells = (uint32_t)(i0[0] % TILE) | (uint32_t)(i0[1] % TILE) << 4 |
               (uint32_t)(i0[2] % TILE) << 8;
-    w.list[slot] = (int32_t)p;
-    w.rec[slot] = r;
+    // the count placed the particle with the same arithmetic (no FMA
+    // contraction in particle_cell), so the slot is in its sub-bin; the guard
+    // keeps the stores in bounds regardless
+    if (slot >= 0 && slot < P.n) {
+      w.list[slot] = (int32_t)p;
+      w.rec[slot] = r;
+    }
   }
 }
 
@@ -1034,9 +1035,9 @@ int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n) {
 
 int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n) {
   if (!tiles_ok(mesh) || (order != 2 && order != 3) || n < 0) return -1;
-  int64_t off[9];
+  int64_t off[8];
   ws_offsets(tile_geom(mesh), n, off);
-  return off[6];
+  return off[5];
 }
 
 int fpm_bin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,

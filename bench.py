@@ -59,7 +59,7 @@ def run_sim(args, dtype, fused, timed, resampler="cic", a0=0.1):
     cosmo = Cosmology(Om0=0.3, h=0.7, n_s=0.96, sigma8=0.8, Ob0=0.05, Tcmb0=2.7255)
     Pk = LinearPower(cosmo)
     pm = ParticleMesh(Nmesh=[args.nc] * 3, BoxSize=args.box, dtype=dtype, resampler=resampler)
-    solver = Solver(pm, cosmo, B=args.B, compensate=True, fused=fused)
+    solver = Solver(pm, cosmo, B=args.B, compensate=True, fused=fused, autotune=fused)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     dlin = solver.linear(solver.whitenoise(args.seed), Pk)

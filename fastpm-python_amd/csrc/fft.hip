@@ -211,13 +211,24 @@ constexpr int pow2floor(int x) {
   return p;
 }
 
-// Column tile width (z columns per tile), capped so the tile stays within
-// 96 KiB of LDS (the rest holds twiddles and multiplier tables).
-template <typename T, int N> struct ColW {
-  static constexpr int a = FPM_COL_BYTES / (int)sizeof(C<T>);
+// Column tile width (z columns per tile): row segments of FPM_COL_BYTES, or
+// twice that when WIDE, capped so the tile stays within 96 KiB of LDS (the
+// rest holds twiddles and multiplier tables).  Both widths are compiled; the
+// one used per axis length is chosen at run time (fpm_fft_set_tiling, set by
+// the autotuner).  A column's arithmetic does not depend on the width (up to
+// the compiler's multiply-add contraction in each instantiation).
+template <typename T, int N, int WIDE = 0> struct ColW {
+  static constexpr int a = (FPM_COL_BYTES << WIDE) / (int)sizeof(C<T>);
   static constexpr int b = pow2floor((96 * 1024 / (int)sizeof(C<T>)) / N);
   static constexpr int BZ = a < b ? a : b;
 };
+
+// Run-time tiling table: wide column tiles per (precision, axis length).
+static int g_col_wide[2][1025];
+
+static int col_wide(int elem_bytes, int n) {
+  return (n >= 0 && n <= 1024) ? g_col_wide[elem_bytes == 8 ? 1 : 0][n] : 0;
+}
 
 #ifndef FPM_Z_ELEMS
 #define FPM_Z_ELEMS 2048  // complex elements per z-pass workgroup
@@ -879,9 +890,9 @@ static bool supported_len(int n) {
   }
 }
 
-template <typename T, int N, bool INV, bool KOP>
-static void launch_col_n(ColArgs<T> a, hipStream_t st) {
-  constexpr int BZ = ColW<T, N>::BZ;
+template <typename T, int N, bool INV, bool KOP, int WIDE>
+static void launch_col_nw(ColArgs<T> a, hipStream_t st) {
+  constexpr int BZ = ColW<T, N, WIDE>::BZ;
   constexpr int NT = threads_for<T>(N * BZ);
   a.ntz = (a.ncols + BZ - 1) / BZ;
   a.nblk = (int64_t)a.ntz * a.nouter;
@@ -894,9 +905,15 @@ static void launch_col_n(ColArgs<T> a, hipStream_t st) {
                        dim3(NT), 0, st, a);
 }
 
-template <typename T, int N>
-static void launch_col_fused_n(ColArgs<T> a, hipStream_t st) {
-  constexpr int BZ = ColW<T, N>::BZ;
+template <typename T, int N, bool INV, bool KOP>
+static void launch_col_n(ColArgs<T> a, hipStream_t st) {
+  if (col_wide((int)sizeof(T), N)) launch_col_nw<T, N, INV, KOP, 1>(a, st);
+  else launch_col_nw<T, N, INV, KOP, 0>(a, st);
+}
+
+template <typename T, int N, int WIDE>
+static void launch_col_fused_nw(ColArgs<T> a, hipStream_t st) {
+  constexpr int BZ = ColW<T, N, WIDE>::BZ;
   constexpr int NT = threads_for<T>(N * BZ);
   a.ntz = (a.ncols + BZ - 1) / BZ;
   a.nblk = (int64_t)a.ntz * a.nouter;
@@ -907,6 +924,12 @@ static void launch_col_fused_n(ColArgs<T> a, hipStream_t st) {
   else
     hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, true, true, false, true>), dim3((unsigned)a.nblk),
                        dim3(NT), 0, st, a);
+}
+
+template <typename T, int N>
+static void launch_col_fused_n(ColArgs<T> a, hipStream_t st) {
+  if (col_wide((int)sizeof(T), N)) launch_col_fused_nw<T, N, 1>(a, st);
+  else launch_col_fused_nw<T, N, 0>(a, st);
 }
 
 // Forward transform along the e axis (stored to a.kout, times a.kscale) fused
@@ -1330,6 +1353,18 @@ using namespace fpm;
 extern "C" {
 
 int fpm_fft_supported(int n) { return supported_len(n) ? 1 : 0; }
+
+int fpm_fft_set_tiling(int elem_bytes, int n, int wide) {
+  if ((elem_bytes != 4 && elem_bytes != 8) || !supported_len(n) || (wide != 0 && wide != 1))
+    return FPM_ERR_ARG;
+  g_col_wide[elem_bytes == 8 ? 1 : 0][n] = wide;
+  return FPM_OK;
+}
+
+int fpm_fft_get_tiling(int elem_bytes, int n) {
+  if ((elem_bytes != 4 && elem_bytes != 8) || !supported_len(n)) return FPM_ERR_ARG;
+  return col_wide(elem_bytes, n);
+}
 int fpm_twiddles_f32(int n, float* o) { return twiddles_impl<float>(n, o); }
 int fpm_twiddles_f64(int n, double* o) { return twiddles_impl<double>(n, o); }
 

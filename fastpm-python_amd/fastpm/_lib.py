@@ -83,6 +83,8 @@ _PLAIN = {
     "fpm_version": (ctypes.c_char_p, []),
     "fpm_strerror": (ctypes.c_char_p, [_int]),
     "fpm_fft_supported": (_int, [_int]),
+    "fpm_fft_set_tiling": (_int, [_int, _int, _int]),
+    "fpm_fft_get_tiling": (_int, [_int, _int]),
     "fpm_power_work_len": (_i64, [_PM, _int]),
     "fpm_growth": (_int, [_dbl, _dbl, _dbl, _dbl, _vp, _int, _vp]),
     "fpm_tiles_supported": (_int, [_PM]),

@@ -137,9 +137,12 @@ class Solver:
     fused : True runs the force as one multi-output inverse transform and one
         four-component readout; False runs the plain sequence of the reference
         (one inverse transform and one readout per component).
+    autotune : time the transform tilings on the force mesh first and keep
+        the faster (fastpm.tune.autotune_fft).
     """
 
-    def __init__(self, pm, cosmology, B=1, a_linear=1.0, compensate=False, fused=True):
+    def __init__(self, pm, cosmology, B=1, a_linear=1.0, compensate=False, fused=True,
+                 autotune=False):
         if not hasattr(cosmology, "Om0"):
             raise TypeError("cosmology must provide Om0")
         fpm = ParticleMesh(Nmesh=pm.Nmesh * B, BoxSize=pm.BoxSize, dtype=pm.dtype, comm=pm.comm,
@@ -151,6 +154,11 @@ class Solver:
         self.a_linear = a_linear
         self.compensate = bool(compensate)
         self.fused = bool(fused)
+        if autotune:
+            # transform tiling of the force mesh (fastpm.tune)
+            from .tune import autotune_fft
+
+            autotune_fft(fpm)
 
     @property
     def nbodystep(self):

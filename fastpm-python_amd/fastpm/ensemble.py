@@ -36,12 +36,14 @@ class Box:
 
 class Ensemble:
     def __init__(self, specs, stages, dtype="f4", B=2, resampler="cic", compensate=True,
-                 nstreams=4):
+                 nstreams=4, autotune=True):
         """specs: iterable of dict(nc=, box=, cosmo=, seed=).  Boxes are spread
         over `nstreams` streams (largest first, each to the least loaded
         stream) so small boxes run concurrently; inside a captured graph the
-        streams become parallel branches."""
+        streams become parallel branches.  autotune selects the transform
+        tiling per force-mesh shape before the first step (fastpm.tune)."""
         self.stages = np.asarray(stages, dtype="f8")
+        self.autotune = bool(autotune)
         self.boxes = [Box(s["nc"], s["box"], s["cosmo"], s["seed"], dtype, B, resampler,
                           compensate) for s in specs]
         self.nsteps = len(self.stages) - 1
@@ -66,6 +68,16 @@ class Ensemble:
         every box; builds the per-box coefficient tables."""
         Pk_cache = {} if Pk_cache is None else Pk_cache
         a0 = self.stages[0]
+        if self.autotune:
+            # one tiling choice per distinct force-mesh shape (fastpm.tune)
+            from .tune import autotune_fft
+
+            seen = set()
+            for b in self.boxes:
+                shape = tuple(int(n) for n in b.solver.fpm.Nmesh)
+                if shape not in seen:
+                    seen.add(shape)
+                    autotune_fft(b.solver.fpm)
         for b in self.boxes:
             key = (b.cosmo.sigma8, b.cosmo.Om0, b.cosmo.h, b.cosmo.n_s)
             if key not in Pk_cache:

@@ -86,6 +86,17 @@ int fpm_fft_supported(int n);
 int fpm_twiddles_f32(int n, float* host_out);
 int fpm_twiddles_f64(int n, double* host_out);
 
+/* Run-time tiling of the strided (y, x) transform passes: for axis length n
+ * and element size elem_bytes (4: float32, 8: float64), wide = 1 selects
+ * column tiles of twice the default row segment (128 instead of 64 bytes),
+ * wide = 0 the default.  Results agree to rounding (each column is
+ * transformed with the same arithmetic, up to multiply-add contraction).  The setting is
+ * process-wide: set it before launching transforms from other threads.
+ * fastpm.tune.autotune_fft times both and keeps the faster per mesh.  No
+ * counterpart in the reference (pmesh plans its FFTs through pfft). */
+int fpm_fft_set_tiling(int elem_bytes, int n, int wide);
+int fpm_fft_get_tiling(int elem_bytes, int n);
+
 /* Mass assignment.  Replaces pmesh ParticleMesh.paint / RealField.paint
  * (fastpm/core.py:191 prepare_force, fastpm/core.py:54 to_mesh).
  * order 2 = CIC, 3 = TSC.  Each particle deposits `mass` with periodic wrap.

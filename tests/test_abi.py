@@ -80,3 +80,20 @@ def test_growth_matches_oracle_and_eds():
     np.testing.assert_allclose(e[:, 4] / e[:, 3], 2.0, rtol=1e-12)
     with pytest.raises(RuntimeError):
         _lib.growth(0.3, 0.7, 0.0, 1.0, [-1.0])
+
+
+def test_fft_tiling_table():
+    """The run-time tiling table (host-side): valid lengths and element sizes
+    are stored and read back, anything else is refused."""
+    from fastpm import _lib
+
+    L = _lib.lib()
+    assert L.fpm_fft_get_tiling(4, 512) in (0, 1)
+    assert L.fpm_fft_set_tiling(4, 96, 1) == 0 and L.fpm_fft_get_tiling(4, 96) == 1
+    assert L.fpm_fft_set_tiling(4, 96, 0) == 0 and L.fpm_fft_get_tiling(4, 96) == 0
+    assert L.fpm_fft_set_tiling(8, 256, 1) == 0 and L.fpm_fft_get_tiling(8, 256) == 1
+    assert L.fpm_fft_set_tiling(8, 256, 0) == 0
+    assert L.fpm_fft_set_tiling(4, 100, 1) < 0      # no plan for 100
+    assert L.fpm_fft_set_tiling(2, 64, 1) < 0       # element size
+    assert L.fpm_fft_set_tiling(4, 64, 2) < 0       # flag
+    assert L.fpm_fft_get_tiling(4, 2048) < 0

@@ -450,3 +450,32 @@ def test_r2c_c2r_fused(gpu, dtype, nmesh):
         o0 = O.c2r(ref_k * O.k_laplace(nmesh, box) * 0.45 *
                    (O.k_deconv(nmesh, box, 2) if len(ops) == 1 else 1.0), nmesh)
         assert _rel(outs[0].value.cpu().numpy(), o0) < _tol(dtype) * 8
+
+
+@pytest.mark.parametrize("dtype", ["f4", "f8"])
+@pytest.mark.parametrize("nmesh", [(64, 64, 64), (96, 48, 32), (32, 128, 16)])
+def test_fft_tiling_bitwise(gpu, dtype, nmesh):
+    """Both column tile widths give the same transforms to rounding (r2c, fused
+    r2c/c2r with a multiplier, c2r_grad3: the per-column arithmetic is the
+    same, up to the compiler's contraction of multiply-adds in each
+    instantiation); the autotuner picks one of them."""
+    from fastpm.pm import r2c_c2r_many, c2r_grad3
+    from fastpm.force.kernels import laplace, compensation
+    from fastpm import tune
+
+    rng = np.random.default_rng(3)
+    pm = _pm(nmesh, (90.0, 70.0, 50.0), dtype)
+    x = pm.create("real", value=torch.from_numpy(rng.standard_normal(nmesh).astype(dtype)).to(gpu))
+    res = []
+    for wide in (0, 1):
+        tune.set_tiling(pm, wide)
+        assert set(tune.get_tiling(pm).values()) == {wide}
+        k, outs = r2c_c2r_many(x, [laplace * 0.7 * compensation(2)])
+        g = c2r_grad3(x.r2c(), laplace * 0.3, grad_order=1)
+        res.append([k.value.clone(), outs[0].value.clone()] + [o.value.clone() for o in g])
+    tol = 1e-6 if dtype == "f4" else 1e-14
+    for a, b in zip(*res):
+        assert (a - b).abs().max().item() <= tol * b.abs().max().item()
+    tune._CACHE.clear()
+    best = tune.autotune_fft(pm, reps=3)
+    assert best in (0, 1) and set(tune.get_tiling(pm).values()) == {best}

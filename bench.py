@@ -12,7 +12,11 @@ float32 displacement) is timed over --steps steps after --warmup untimed
 steps, then run on to a = 1; the float64 plain run (one transform and one
 readout per force component) is timed the same way.  The final states are
 compared: P(k) relative error for k < k_Nyq/2, r(k), and the largest periodic
-displacement difference.  One JSON line is printed.
+displacement difference.  Further legs (each can be skipped): the float64
+oracle's KDK step on the host cores (cpu_baseline), 64 boxes of uneven sizes
+replayed from a captured graph (ensemble), the differentiable model's forward
+and adjoint pass at 128^3 / 256^3 (gradient) and, with --large, 512^3
+particles on a 1024^3 TSC mesh.  One JSON line is printed.
 """
 import argparse
 import json
@@ -41,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-box batched ensemble (graph replay) measurement")
+    ap.add_argument("--no-grad", action="store_true",
+                    help="skip the differentiable-model (forward + adjoint) measurement")
     ap.add_argument("--large", action="store_true",
                     help="also run the large box: 512^3 particles on 1024^3, TSC, FP32 vs FP64")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -167,6 +173,62 @@ def run_ensemble():
     return out
 
 
+def region_bandwidth(kern, Np, Nc, Nk, bpe=4):
+    """Algorithmic bytes of each timed region of one KDK step of the float32
+    stencil path (per call) and the rate they imply: kick/drift 36 N_p, bin
+    12 N_p (S read), paint 4 N_c, transforms 8 N_c + 16 N_k (density in,
+    delta_k and potential out), readout 8 N_c + 28 N_p."""
+    s = bpe / 4.0
+    model = {"kick": 36 * Np, "drift": 36 * Np, "bin": 12 * Np, "paint": 4 * Nc,
+             "fft": 8 * Nc + 16 * Nk, "readout": 8 * Nc + 28 * Np}
+    out = {}
+    for k, (c, m) in kern.items():
+        if k in model:
+            gbs = model[k] * s / (m * 1e-3) / 1e9
+            out[k] = {"bytes": int(model[k] * s), "GB_s": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
+def run_gradient():
+    """The differentiable forward model: L = 500 Mpc/h, 128^3 particles on a
+    256^3 mesh (B = 2), 10 KDK steps from a = 0.1, float32, loss against the
+    density of seed 99 divided by P(k); times one forward pass with per-step
+    checkpoints and one adjoint pass to the white-noise gradient."""
+    import numpy as np
+    import torch
+    from fastpm.core import Solver
+    from fastpm.pm import ParticleMesh
+    from fastpm.cosmology import Cosmology, LinearPower
+    from fastpm.adjoint import ForwardModel, DensityLoss
+
+    cosmo = Cosmology(Om0=0.3, h=0.7, n_s=0.96, sigma8=0.8)
+    Pk = LinearPower(cosmo)
+    pm = ParticleMesh(Nmesh=[128] * 3, BoxSize=500.0, dtype="f4")
+    solver = Solver(pm, cosmo, B=2)
+    model = ForwardModel(solver, Pk, np.linspace(0.1, 1.0, 11))
+    loss = DensityLoss(solver, Pk, None)
+    loss.target = loss.density_k(model.forward(solver.whitenoise(99)))
+    w = solver.whitenoise(7)
+    for _ in range(2):  # warm-up
+        L, gS = loss(model.forward(w))
+        model.backward(gS)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = model.forward(w)
+    L, gS = loss(st)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g = model.backward(gS)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    n = 128 ** 3 * 10
+    return dict(workload="L=500 Mpc/h, 128^3 particles, 256^3 mesh, 10 steps, FP32, "
+                         "density loss / P(k), gradient w.r.t. the white-noise modes",
+                forward_ms=(t1 - t0) * 1e3, adjoint_ms=(t2 - t1) * 1e3,
+                particle_steps_per_s=n / (t2 - t0), loss=L,
+                grad_norm=float(torch.linalg.vector_norm(g.value).item()))
+
+
 def run_large(args):
     """512^3 particles on a 1024^3 mesh, L = 2000 Mpc/h, TSC with window
     compensation, 2LPT at a = 0.05, 40 steps to a = 1: FP32 against FP64
@@ -187,6 +249,9 @@ def run_large(args):
                fp32_kernels_ms={k: {"calls": c, "mean_ms": m}
                                 for k, (c, m) in res["f4"]["kern"].items()},
                accuracy=g)
+    nm = res["f4"]["nmesh"]
+    out["fp32_kernels_bandwidth"] = region_bandwidth(res["f4"]["kern"], res["f4"]["npart"],
+                                                     nm ** 3, nm * nm * (nm // 2 + 1))
     return out
 
 
@@ -255,6 +320,8 @@ def main():
     # roofline of the dominant kernel (readout: one kernel per call)
     kern = r32["kern"]
     out["kernels_ms"] = {k: {"calls": c, "mean_ms": m} for k, (c, m) in kern.items()}
+    out["kernels_bandwidth"] = region_bandwidth(
+        kern, Np, Nc, r32["nmesh"] ** 2 * (r32["nmesh"] // 2 + 1))
     ro_calls, ro_ms = kern["readout"]
     # density and potential meshes read once, S in, RHO and F out
     ro_bytes = 8 * Nc + 28 * Np
@@ -297,6 +364,9 @@ def main():
     if not args.no_ensemble:
         print("bench: batched ensemble", file=sys.stderr, flush=True)
         out["ensemble"] = run_ensemble()
+    if not args.no_grad:
+        print("bench: differentiable model", file=sys.stderr, flush=True)
+        out["gradient"] = run_gradient()
     if args.large:
         print("bench: large box", file=sys.stderr, flush=True)
         out["large_box"] = run_large(args)

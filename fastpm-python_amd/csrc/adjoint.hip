@@ -340,6 +340,90 @@ static int lpt2_vjp_impl(const void* gs, const void* const* phi, void* const* gp
   return FPM_OK;
 }
 
+// ---------------------------------------------------------------- stencils
+// The four-point gradient as a real-space stencil,
+//   D_d f(x) = [8 (f(x+h) - f(x-h)) - (f(x+2h) - f(x-2h))] / (12 h),
+// equal to the k-space multiplier i (8 sin kh - sin 2kh) / (6h) (the force's
+// order-1 gradient) and antisymmetric (D_d^T = -D_d).  One thread per cell,
+// periodic wrap; neighbours come through L1/L2.
+template <typename T>
+__device__ __forceinline__ T d4(const T* f, int64_t base, int i, int n, int64_t stride) {
+  auto at = [&](int s) {
+    int j = i + s;
+    j = j < 0 ? j + n : (j >= n ? j - n : j);
+    j = j < 0 ? j + n : (j >= n ? j - n : j);
+    return f[base + (int64_t)j * stride];
+  };
+  return (T)8 * (at(1) - at(-1)) - (at(2) - at(-2));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_grad4_mesh(const T* phi, T* o0, T* o1, T* o2, int n0,
+                                                    int n1, int n2, T s0, T s1, T s2) {
+  const int64_t ncell = (int64_t)n0 * n1 * n2;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(c % n2);
+    const int64_t t = c / n2;
+    const int y = (int)(t % n1), x = (int)(t / n1);
+    const int64_t bx = c - (int64_t)x * n1 * n2, by = c - (int64_t)y * n2, bz = c - z;
+    o0[c] = s0 * d4(phi, bx, x, n0, (int64_t)n1 * n2);
+    o1[c] = s1 * d4(phi, by, y, n1, (int64_t)n2);
+    o2[c] = s2 * d4(phi, bz, z, n2, (int64_t)1);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_grad4_div(const T* g0, const T* g1, const T* g2, T* out,
+                                                   int n0, int n1, int n2, T s0, T s1, T s2,
+                                                   int accumulate) {
+  const int64_t ncell = (int64_t)n0 * n1 * n2;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(c % n2);
+    const int64_t t = c / n2;
+    const int y = (int)(t % n1), x = (int)(t / n1);
+    const int64_t bx = c - (int64_t)x * n1 * n2, by = c - (int64_t)y * n2, bz = c - z;
+    const T v = s0 * d4(g0, bx, x, n0, (int64_t)n1 * n2) + s1 * d4(g1, by, y, n1, (int64_t)n2) +
+                s2 * d4(g2, bz, z, n2, (int64_t)1);
+    out[c] = accumulate ? out[c] + v : v;
+  }
+}
+
+template <typename T>
+static int grad4_mesh_impl(const void* phi, void* const* outs, const fpm_mesh* m, double scale,
+                           hipStream_t st) {
+  if (!phi || !outs || !m || !outs[0] || !outs[1] || !outs[2]) return FPM_ERR_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (m->nmesh[d] <= 0 || !(m->box[d] > 0)) return FPM_ERR_ARG;
+  T s[3];
+  for (int d = 0; d < 3; ++d) s[d] = (T)(scale * m->nmesh[d] / (12.0 * m->box[d]));
+  const int64_t ncell = (int64_t)m->nmesh[0] * m->nmesh[1] * m->nmesh[2];
+  hipLaunchKernelGGL(k_grad4_mesh<T>, dim3(grid_n(ncell)), dim3(256), 0, st,
+                     static_cast<const T*>(phi), static_cast<T*>(outs[0]),
+                     static_cast<T*>(outs[1]), static_cast<T*>(outs[2]), m->nmesh[0],
+                     m->nmesh[1], m->nmesh[2], s[0], s[1], s[2]);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
+static int grad4_div_impl(const void* const* g, void* out, const fpm_mesh* m, double scale,
+                          int accumulate, hipStream_t st) {
+  if (!g || !out || !m || !g[0] || !g[1] || !g[2]) return FPM_ERR_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (m->nmesh[d] <= 0 || !(m->box[d] > 0) || g[d] == out) return FPM_ERR_ARG;
+  T s[3];
+  for (int d = 0; d < 3; ++d) s[d] = (T)(scale * m->nmesh[d] / (12.0 * m->box[d]));
+  const int64_t ncell = (int64_t)m->nmesh[0] * m->nmesh[1] * m->nmesh[2];
+  hipLaunchKernelGGL(k_grad4_div<T>, dim3(grid_n(ncell)), dim3(256), 0, st,
+                     static_cast<const T*>(g[0]), static_cast<const T*>(g[1]),
+                     static_cast<const T*>(g[2]), static_cast<T*>(out), m->nmesh[0],
+                     m->nmesh[1], m->nmesh[2], s[0], s[1], s[2], accumulate ? 1 : 0);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
 }  // namespace fpm
 
 using namespace fpm;
@@ -376,6 +460,15 @@ FPM_BOTH(fpm_lpt2_source_vjp, lpt2_vjp_impl,
          (const void* gsource, const void* const* phi, void* const* gphi, int64_t ncells,
           fpm_stream_t s),
          (gsource, phi, gphi, ncells, (hipStream_t)s))
+
+FPM_BOTH(fpm_grad4_mesh, grad4_mesh_impl,
+         (const void* phi, void* const* outs, const fpm_mesh* mesh, double scale, fpm_stream_t s),
+         (phi, outs, mesh, scale, (hipStream_t)s))
+
+FPM_BOTH(fpm_grad4_div, grad4_div_impl,
+         (const void* const* g, void* out, const fpm_mesh* mesh, double scale, int accumulate,
+          fpm_stream_t s),
+         (g, out, mesh, scale, accumulate, (hipStream_t)s))
 
 #undef FPM_BOTH
 

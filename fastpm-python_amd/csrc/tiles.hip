@@ -375,26 +375,16 @@ __device__ __forceinline__ void deposit_rec(const PRec& rec, int o, const TileGe
   }
 }
 
-// The up to 64 (source tile, sub-bin) ranges feeding a tile are gathered
-// first (one lane each) and scanned, then all their particles are processed
-// as one flat index space, so their loads are issued together.
-template <typename T, int ORDER>
-__global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, TileGeom g,
-                                                               Workspace w, T* mesh, double mass_fx,
-                                                               double inv_fx) {
-  constexpr int NB = TILE * TILE * TILE;
-  __shared__ unsigned long long acc[NB];
-  __shared__ int32_t rstart[64];
-  __shared__ int32_t rpre[65];
-  __shared__ int32_t rdir[64];
-  const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
-  int lo[3];
-  tile_coords(t, g, lo);
-  for (int i = threadIdx.x; i < NB; i += kTileThreads) acc[i] = 0ull;
+// The up to 64 (source tile, sub-bin) ranges whose particles deposit into the
+// tile at lo: lane = o * 8 + m takes source tile t - o (o in {0,1}^3; along an
+// axis with a single tile only o = 0, the neighbour being the tile itself)
+// and sub-bin m, whose crossing mask must contain the direction o.  Fills
+// rstart (first slot), rdir (o) and rpre (exclusive prefix of the lengths,
+// rpre[64] = total); the caller synchronises.
+__device__ __forceinline__ void gather_ranges(const TileGeom& g, const Workspace& w,
+                                              const int lo[3], int32_t* rstart, int32_t* rpre,
+                                              int32_t* rdir) {
   if (threadIdx.x < 64) {
-    // lane = o * 8 + m: source tile t - o (o in {0,1}^3; along an axis with a
-    // single tile only o = 0, the neighbour being the tile itself), sub-bin m
-    // whose crossing mask must contain the direction o
     const int lane = threadIdx.x;
     const int o = lane >> 3, m = lane & 7;
     const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
@@ -423,6 +413,25 @@ __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, Til
     rpre[lane + 1] = inc;
     if (lane == 0) rpre[0] = 0;
   }
+}
+
+// The up to 64 (source tile, sub-bin) ranges feeding a tile are gathered
+// first (one lane each) and scanned, then all their particles are processed
+// as one flat index space, so their loads are issued together.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, TileGeom g,
+                                                               Workspace w, T* mesh, double mass_fx,
+                                                               double inv_fx) {
+  constexpr int NB = TILE * TILE * TILE;
+  __shared__ unsigned long long acc[NB];
+  __shared__ int32_t rstart[64];
+  __shared__ int32_t rpre[65];
+  __shared__ int32_t rdir[64];
+  const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
+  int lo[3];
+  tile_coords(t, g, lo);
+  for (int i = threadIdx.x; i < NB; i += kTileThreads) acc[i] = 0ull;
+  gather_ranges(g, w, lo, rstart, rpre, rdir);
   __syncthreads();
   const int32_t total = rpre[64];
   for (int32_t f = threadIdx.x; f < total; f += kTileThreads) {
@@ -438,6 +447,86 @@ __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, Til
     const int z = i % TILE, y = (i / TILE) % TILE, x = i / (TILE * TILE);
     mesh[((int64_t)(lo[0] + x) * g.nm[1] + (lo[1] + y)) * g.nm[2] + (lo[2] + z)] =
         (T)((double)(long long)acc[i] * inv_fx);
+  }
+}
+
+// Weighted paint of NC per-particle weight arrays at once (the adjoint of a
+// readout): the same gather over source ranges as k_paint_tiles, with float
+// accumulators in LDS (adds in arrival order, so rounding can differ from run
+// to run), each cell of every output written once.
+template <typename T>
+struct TilePaintW {
+  const T* w[3];
+  int64_t wstride[3];
+  T* out[3];
+  T scale;
+};
+
+template <typename T, int ORDER, int NC>
+__global__ __launch_bounds__(kTileThreads) void k_paint_tiles_weighted(TileGeom g, Workspace w,
+                                                                        TilePaintW<T> A) {
+  constexpr int NB = TILE * TILE * TILE;
+  __shared__ float acc[NC][NB];
+  __shared__ int32_t rstart[64];
+  __shared__ int32_t rpre[65];
+  __shared__ int32_t rdir[64];
+  const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
+  int lo[3];
+  tile_coords(t, g, lo);
+  for (int i = threadIdx.x; i < NB; i += kTileThreads)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c][i] = 0.0f;
+  gather_ranges(g, w, lo, rstart, rpre, rdir);
+  __syncthreads();
+  const int32_t total = rpre[64];
+  for (int32_t f = threadIdx.x; f < total; f += kTileThreads) {
+    int r = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+      if (rpre[r + step] <= f) r += step;
+    const int32_t sl = rstart[r] + (f - rpre[r]);
+    const int64_t p = w.list[sl];
+    T wc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) wc[c] = A.w[c][p * A.wstride[c]] * A.scale;
+    int l[3];
+    T wt[3][ORDER];
+    window_rec<T, ORDER>(w.rec[sl], l, wt);
+    const int o = rdir[r];
+    const int od[3] = {(o >> 2) & 1, (o >> 1) & 1, o & 1};
+    int u[3][ORDER];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int a = 0; a < ORDER; ++a) {
+        int v = l[d] + a - TILE * od[d];
+        if (g.nt[d] == 1 && v >= TILE) v -= TILE;
+        u[d][a] = v;
+      }
+#pragma unroll
+    for (int a = 0; a < ORDER; ++a) {
+      if ((unsigned)u[0][a] >= (unsigned)TILE) continue;
+#pragma unroll
+      for (int b = 0; b < ORDER; ++b) {
+        if ((unsigned)u[1][b] >= (unsigned)TILE) continue;
+        const T wab = wt[0][a] * wt[1][b];
+#pragma unroll
+        for (int e = 0; e < ORDER; ++e) {
+          if ((unsigned)u[2][e] >= (unsigned)TILE) continue;
+          const int cell = (u[0][a] * TILE + u[1][b]) * TILE + u[2][e];
+          const T wv = wab * wt[2][e];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) atomicAdd(&acc[c][cell], (float)(wv * wc[c]));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NB; i += kTileThreads) {
+    const int z = i % TILE, y = (i / TILE) % TILE, x = i / (TILE * TILE);
+    const int64_t o = ((int64_t)(lo[0] + x) * g.nm[1] + (lo[1] + y)) * g.nm[2] + (lo[2] + z);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) A.out[c][o] = (T)acc[c][i];
   }
 }
 
@@ -987,6 +1076,41 @@ static int readout_tiles_impl(const fpm_particles* part, const fpm_mesh* m, int 
 }
 
 template <typename T>
+static int paint_tiles_w_impl(const fpm_particles* part, const fpm_mesh* m, int order, int ncomp,
+                              const void* const* weights, const int64_t* wstrides, double scale,
+                              void* const* outs, void* ws, int64_t ws_bytes, hipStream_t st) {
+  PartDesc<T> P;
+  TileGeom g;
+  Workspace w;
+  int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
+  if (rc) return rc;
+  if (ncomp < 1 || ncomp > 3 || !weights || !wstrides || !outs) return FPM_ERR_ARG;
+  TilePaintW<T> A{};
+  for (int c = 0; c < ncomp; ++c) {
+    if (!weights[c] || !outs[c] || wstrides[c] < 1) return FPM_ERR_ARG;
+    A.w[c] = static_cast<const T*>(weights[c]);
+    A.wstride[c] = wstrides[c];
+    A.out[c] = static_cast<T*>(outs[c]);
+  }
+  A.scale = (T)scale;
+  const dim3 grid((unsigned)g.ntiles), blk(kTileThreads);
+#define FPM_PW(O_, C_)                                                                        \
+  hipLaunchKernelGGL((k_paint_tiles_weighted<T, O_, C_>), grid, blk, 0, st, g, w, A)
+  if (order == 2) {
+    if (ncomp == 1) FPM_PW(2, 1);
+    else if (ncomp == 2) FPM_PW(2, 2);
+    else FPM_PW(2, 3);
+  } else {
+    if (ncomp == 1) FPM_PW(3, 1);
+    else if (ncomp == 2) FPM_PW(3, 2);
+    else FPM_PW(3, 3);
+  }
+#undef FPM_PW
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
 static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int order,
                              const void* rho, const void* phi, void* rho_out, int64_t rho_stride,
                              void* f_out, int64_t f_stride, double rho_scale, double f_scale,
@@ -1052,6 +1176,14 @@ int fpm_readout_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int o
                           const void* const* meshes, void* const* outs, const int64_t* strides,
                           const double* scales, void* ws, int64_t ws_bytes, fpm_stream_t s) {
   return readout_tiles_impl<float>(part, mesh, order, ncomp, meshes, outs, strides, scales, ws,
+                                   ws_bytes, (hipStream_t)s);
+}
+
+int fpm_paint_tiled_weighted_f32(const fpm_particles* part, const fpm_mesh* mesh, int order,
+                                 int ncomp, const void* const* weights, const int64_t* wstrides,
+                                 double scale, void* const* outs, void* ws, int64_t ws_bytes,
+                                 fpm_stream_t s) {
+  return paint_tiles_w_impl<float>(part, mesh, order, ncomp, weights, wstrides, scale, outs, ws,
                                    ws_bytes, (hipStream_t)s);
 }
 

@@ -59,6 +59,8 @@ _TYPED = {
     "fpm_readout": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _vp, _vp]),
     "fpm_fft_r2c": (_int, [_vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
     "fpm_axpby": (_int, [_vp, _vp, _i64, _dbl, _dbl, _vp]),
+    "fpm_grad4_mesh": (_int, [_vp, _vp, _PM, _dbl, _vp]),
+    "fpm_grad4_div": (_int, [_vp, _vp, _PM, _dbl, _int, _vp]),
     "fpm_drift_wrap": (_int, [_vp, _vp, _i64, _dbl, _vp, _vp]),
     "fpm_fft_c2r": (_int, [_vp, _int, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
     "fpm_fft_r2c_c2r": (_int, [_vp, _vp, _int, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
@@ -93,6 +95,8 @@ _PLAIN = {
     "fpm_bin_particles_f32": (_int, [_PP, _PM, _int, _vp, _i64, _vp]),
     "fpm_paint_tiled_f32": (_int, [_PP, _PM, _int, _dbl, _vp, _vp, _i64, _vp]),
     "fpm_readout_tiled_f32": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "fpm_paint_tiled_weighted_f32": (_int, [_PP, _PM, _int, _int, _vp, _vp, _dbl, _vp, _vp, _i64,
+                                            _vp]),
     "fpm_readout_tiled_grad4_f32": (_int, [_PP, _PM, _int, _vp, _vp, _vp, _i64, _vp, _i64, _dbl,
                                            _dbl, _vp, _i64, _vp]),
     "fpm_counter_add": (_int, [_vp, ctypes.c_int32, _vp]),

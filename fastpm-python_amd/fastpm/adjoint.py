@@ -29,7 +29,7 @@ import torch
 
 from . import _lib
 from .core import StateVector, leapfrog
-from .pm import RealField, ComplexField, Particles, c2r_grad3, c2r_many, _stream
+from .pm import RealField, ComplexField, Particles, c2r_many, _stream
 from .force.kernels import laplace, gradient, compensation, identity
 from .background import MatterDominated
 
@@ -99,25 +99,61 @@ def _scale_by_table(pm, field, tab):
 
 
 # --------------------------------------------------------------------------- force
+def grad4_mesh(pm, phi, outs, scale=1.0):
+    """outs[d] = scale * D_d phi, the four-point gradient as a stencil."""
+    _lib.call("fpm_grad4_mesh", pm.dtype, phi.value.data_ptr(),
+              _lib.ptr_array([o.value.data_ptr() for o in outs]), pm._desc, float(scale),
+              _stream())
+
+
+def grad4_div(pm, g, out, scale=1.0, accumulate=False):
+    """out (+)= scale * sum_d D_d g[d]."""
+    _lib.call("fpm_grad4_div", pm.dtype, _lib.ptr_array([m.value.data_ptr() for m in g]),
+              out.value.data_ptr(), pm._desc, float(scale), 1 if accumulate else 0, _stream())
+
+
 def force_vjp(step, S, Q, gF, gS):
-    """gS += (d F / d S)^T gF for the PM force F(S) of FastPMStep.Force
-    (gravity_gradient, fastpm/operators.py:218-253)."""
+    """gS += (d F / d S)^T gF for the PM force of FastPMStep.Force
+    (gravity_gradient, fastpm/operators.py:218-253).
+
+    F_d = readout(D_d phi) with phi = c2r(common r2c(rho)) and D_d the
+    four-point gradient (a stencil, equal to the k-space kernel).  Its adjoint
+    with respect to the mesh is g_phi = sum_d D_d^T paint(gF_d) =
+    -sum_d D_d paint(gF_d), and since common is real and r2c^T c2r^T cancel
+    their Hermitian weights, g_rho = c2r(common r2c(g_phi)): the forward's own
+    fused transform.  The position gradient collects the window derivatives
+    of the force readout and of the paint."""
+    from .pm import r2c_c2r_many
+
     pm = step.pm
     parts = Particles(Q, S)
     nbar = 1.0 * len(Q) / float(np.prod(pm.Nmesh))
-    rho = pm.paint(parts, mass=1.0 / nbar)
-    delta_k = rho.r2c()
     common = step.force_common()
-    phi = c2r_grad3(delta_k, common)
+    tiled = pm.tiles_supported()
+    if tiled:
+        # float32 on tile-divisible meshes: the binned paints
+        pm.bin_particles(parts)
+        rho = pm.paint_tiled(parts, mass=1.0 / nbar)
+    else:
+        rho = pm.paint(parts, mass=1.0 / nbar)
+    kbuf = ComplexField(pm)
+    phi = RealField(pm)
+    work = [pm._complex_tensor()]
+    r2c_c2r_many(rho, [common], kout=kbuf, outs=[phi], work=work)
+    m = [RealField(pm) for _ in range(3)]
+    grad4_mesh(pm, phi, m)
     # readout's position derivative
-    window_grad(pm, parts, phi, [gF[:, 0], gF[:, 1], gF[:, 2]], gS, 1.0, accumulate=True)
-    # readout's mesh adjoint, then the conjugated multipliers back to rho
-    acc = ComplexField(pm)
-    for d in range(3):
-        gphi = paint_weighted(pm, parts, gF[:, d])
-        gk = gphi.r2c()
-        _kaccum(pm, acc, gk, common * gradient(d, 1), conj=True, accumulate=d > 0)
-    grho = acc.c2r()
+    window_grad(pm, parts, m, [gF[:, 0], gF[:, 1], gF[:, 2]], gS, 1.0, accumulate=True)
+    # readout's mesh adjoint, the stencil's adjoint, then the multiplier back to rho
+    if tiled:
+        pm.paint_tiled_weighted(parts, [gF[:, 0], gF[:, 1], gF[:, 2]], m)
+    else:
+        for d in range(3):
+            paint_weighted(pm, parts, gF[:, d], out=m[d])
+    gphi = rho  # rho is no longer needed
+    grad4_div(pm, m, gphi, scale=-1.0)
+    grho = phi
+    r2c_c2r_many(gphi, [common], kout=kbuf, outs=[grho], work=work)
     # paint's position derivative (rho = paint(X) / nbar)
     window_grad(pm, parts, [grho], None, gS, 1.0 / nbar, accumulate=True)
 

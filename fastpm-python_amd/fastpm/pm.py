@@ -303,6 +303,21 @@ class ParticleMesh:
         _lib.call_plain("fpm_readout_tiled_f32", parts.desc(self.dtype), self._desc, order, n,
                         meshes, optrs, strides, sc, ws.data_ptr(), ws.numel(), _stream())
 
+    def paint_tiled_weighted(self, pos, weights, outs, scale=1.0, resampler=None):
+        """outs[c] = scale * paint(pos, mass=weights[c]) for up to three 1-D
+        weight views at once (the adjoint of a readout); `pos` must have been
+        binned with bin_particles."""
+        parts = self._particles(pos)
+        order = resampler_order(resampler or self.resampler)
+        ws = self.tile_workspace(len(parts), order)
+        n = len(weights)
+        wp = _lib.ptr_array([w.data_ptr() for w in weights])
+        st = (ctypes_i64 * n)(*[int(w.stride(0)) for w in weights])
+        op = _lib.ptr_array([o.value.data_ptr() for o in outs])
+        _lib.call_plain("fpm_paint_tiled_weighted_f32", parts.desc(self.dtype), self._desc, order,
+                        n, wp, st, float(scale), op, ws.data_ptr(), ws.numel(), _stream())
+        return outs
+
     def grad4_supported(self, resampler=None):
         """Whether readout_tiled_grad4 runs on this mesh (tiled, and at least
         16 + order + 3 cells per side for the stencil box)."""

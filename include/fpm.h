@@ -142,6 +142,17 @@ int fpm_readout_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int o
                           const void* const* meshes, void* const* outs, const int64_t* out_strides,
                           const double* scales, void* ws, int64_t ws_bytes, fpm_stream_t stream);
 
+/* Weighted tiled paint of ncomp (<= 3) weight arrays at once:
+ * outs[c](cell) = scale * sum_p weights[c][p * wstrides[c]] W(x_p - cell), the
+ * adjoint of a readout (readout_gradient of fastpm/operators.py:236-247, as
+ * used by gravity_gradient).  Uses the binning of the last
+ * fpm_bin_particles_f32 call; accumulates in float in LDS (summation order
+ * may vary between runs); each output cell is written once. */
+int fpm_paint_tiled_weighted_f32(const fpm_particles* part, const fpm_mesh* mesh, int order,
+                                 int ncomp, const void* const* weights, const int64_t* wstrides,
+                                 double scale, void* const* outs, void* ws, int64_t ws_bytes,
+                                 fpm_stream_t stream);
+
 /* Force readout with the four-point gradient applied in real space
  * (replaces fastpm/force/gravity.py:17-24 -- c2r(i a_d(k) pot_k) and a readout
  * for each d -- and the RHO readout of fastpm/core.py:203).  The four-point
@@ -347,6 +358,23 @@ int fpm_hermitian_scale_f32(void* field, double mid, double edge, const fpm_mesh
                             int64_t pitch2, fpm_stream_t stream);
 int fpm_hermitian_scale_f64(void* field, double mid, double edge, const fpm_mesh* mesh,
                             int64_t pitch2, fpm_stream_t stream);
+
+/* The force's four-point gradient as a real-space stencil,
+ * D_d f = [8 (f(x+h) - f(x-h)) - (f(x+2h) - f(x-2h))] / (12 h), h = box_d / n_d,
+ * equal to c2r(i a_d(k) r2c(f)) with the order-1 kernel of
+ * fastpm/force/kernels.py:17-24 (periodic wrap).
+ * fpm_grad4_mesh: outs[d] = scale * D_d phi (three meshes, not aliasing phi).
+ * fpm_grad4_div: out (+)= scale * sum_d D_d g[d]; D_d^T = -D_d, so scale = -1
+ * gives the adjoint of fpm_grad4_mesh.  Used by the force's adjoint
+ * (gravity_gradient, fastpm/operators.py:218-253). */
+int fpm_grad4_mesh_f32(const void* phi, void* const* outs, const fpm_mesh* mesh, double scale,
+                       fpm_stream_t stream);
+int fpm_grad4_mesh_f64(const void* phi, void* const* outs, const fpm_mesh* mesh, double scale,
+                       fpm_stream_t stream);
+int fpm_grad4_div_f32(const void* const* g, void* out, const fpm_mesh* mesh, double scale,
+                      int accumulate, fpm_stream_t stream);
+int fpm_grad4_div_f64(const void* const* g, void* out, const fpm_mesh* mesh, double scale,
+                      int accumulate, fpm_stream_t stream);
 
 /* gphi[i] = gsource * d source / d phi_i for the 2LPT source of
  * fpm_lpt2_source (fastpm/operators.py:140-193). */

@@ -246,3 +246,34 @@ def test_config4_gradient(gpu):
     print("config4 corr32/64 %.8f median rel %.3e max rel %.3e" % (corr, np.median(rel), rel.max()))
     assert np.median(rel) < 1e-3
     assert np.linalg.norm(fds - gs) / np.linalg.norm(gs) < 5e-2
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+@pytest.mark.parametrize("nmesh", [(16, 16, 16), (8, 12, 32), (32, 24, 6)])
+def test_grad4_stencil(gpu, dtype, nmesh):
+    """fpm_grad4_mesh equals the oracle's c2r(i a_d(k) r2c(f)) with the
+    four-point kernel; fpm_grad4_div with scale -1 is its adjoint
+    (<D f, g> = -<f, D g> summed over d)."""
+    from fastpm.pm import ParticleMesh
+    from fastpm.adjoint import grad4_mesh, grad4_div
+
+    box = (50.0, 70.0, 90.0)
+    rng = np.random.default_rng(12)
+    pm = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype=dtype)
+    f = rng.standard_normal(nmesh)
+    fk = O.r2c(f)
+    F = pm.create("real", value=torch.from_numpy(f.astype(dtype)).to(gpu))
+    outs = [pm.create("real") for _ in range(3)]
+    grad4_mesh(pm, F, outs, scale=2.0)
+    tol = 1e-12 if dtype == "f8" else 2e-5
+    for d in range(3):
+        ref = 2.0 * O.c2r(fk * O.k_gradient(d, nmesh, box, 1), nmesh)
+        got = outs[d].value.cpu().numpy()
+        assert np.abs(got - ref).max() < tol * np.abs(ref).max(), d
+    g = [rng.standard_normal(nmesh) for _ in range(3)]
+    G = [pm.create("real", value=torch.from_numpy(x.astype(dtype)).to(gpu)) for x in g]
+    div = pm.create("real")
+    grad4_div(pm, G, div, scale=-1.0)
+    lhs = sum(float((outs[d].value.double() * G[d].value.double()).sum()) for d in range(3)) / 2.0
+    rhs = float((F.value.double() * div.value.double()).sum())
+    assert abs(lhs - rhs) < (1e-10 if dtype == "f8" else 1e-3) * abs(lhs)

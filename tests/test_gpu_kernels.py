@@ -479,3 +479,39 @@ def test_fft_tiling_bitwise(gpu, dtype, nmesh):
     tune._CACHE.clear()
     best = tune.autotune_fft(pm, reps=3)
     assert best in (0, 1) and set(tune.get_tiling(pm).values()) == {best}
+
+
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("nmesh,kind", [((32, 48, 64), "random"), ((16, 16, 16), "lattice"),
+                                        ((96, 32, 16), "cluster")])
+def test_paint_tiled_weighted(gpu, order, nmesh, kind):
+    """Three weighted paints at once from the binned records equal the
+    oracle's weighted paint of each weight array (the adjoint of readout)."""
+    from fastpm.pm import Particles
+
+    box = (64.0, 96.0, 128.0)
+    rng = np.random.default_rng(4)
+    pm = _pm(nmesh, box, "f4", "cic" if order == 2 else "tsc")
+    if kind == "lattice":
+        nlat = tuple(n // 2 for n in nmesh)
+        Q = _pm(nlat, box, "f4").generate_uniform_particle_grid(shift=0.0)
+        s = rng.normal(0, 2.0, size=(len(Q), 3)).astype("f4")
+        parts = Particles(Q, torch.from_numpy(s).to(gpu))
+    else:
+        n = 20000
+        q = rng.uniform(0, 1, size=(n, 3)) * np.array(box)
+        if kind == "cluster":
+            q[: n // 2] = np.array(box) * 0.5 + rng.normal(0, 0.01, size=(n // 2, 3))
+        parts = Particles(torch.from_numpy(q.astype("f4")).to(gpu), None)
+    X = pm.positions(parts).cpu().numpy().astype("f8")
+    W = torch.from_numpy(rng.standard_normal((len(X), 3)).astype("f4")).to(gpu)
+    pm.bin_particles(parts)
+    outs = [pm.create("real") for _ in range(3)]
+    pm.paint_tiled_weighted(parts, [W[:, 0], W[:, 1], W[:, 2]], outs, scale=0.5)
+    Wh = W.cpu().numpy().astype("f8")
+    for c in range(3):
+        ref = O.paint(X, nmesh, box, order, mass=0.5 * Wh[:, c])
+        # float accumulation: bounded by the paint of |w| (signed weights
+        # cancel in the clustered cell)
+        scale = O.paint(X, nmesh, box, order, mass=0.5 * np.abs(Wh[:, c])).max()
+        assert np.abs(outs[c].value.cpu().numpy() - ref).max() < 1e-6 * scale, c

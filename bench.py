@@ -176,11 +176,12 @@ def run_ensemble():
 def region_bandwidth(kern, Np, Nc, Nk, bpe=4):
     """Algorithmic bytes of each timed region of one KDK step of the float32
     stencil path (per call) and the rate they imply: kick/drift 36 N_p, bin
-    12 N_p (S read), paint 4 N_c, transforms 8 N_c + 16 N_k (density in,
-    delta_k and potential out), readout 8 N_c + 28 N_p."""
+    12 N_p (S read), paint 4 N_c, transforms 8 N_c (density in, potential
+    out; without a monitor the spectrum is not stored), readout
+    8 N_c + 28 N_p."""
     s = bpe / 4.0
     model = {"kick": 36 * Np, "drift": 36 * Np, "bin": 12 * Np, "paint": 4 * Nc,
-             "fft": 8 * Nc + 16 * Nk, "readout": 8 * Nc + 28 * Np}
+             "fft": 8 * Nc, "readout": 8 * Nc + 28 * Np}
     out = {}
     for k, (c, m) in kern.items():
         if k in model:
@@ -334,9 +335,10 @@ def main():
                        "algorithmic_bytes": ro_bytes}
     Nk = r32["nmesh"] ** 2 * (r32["nmesh"] // 2 + 1)
     # compulsory bytes of this dataflow (one inverse transform of the
-    # potential, stencil in the readout) and of the reference's (three
-    # gradient transforms, four-mesh readout)
-    step_bytes = 20 * Nc + 16 * Nk + 148 * Np
+    # potential, stencil in the readout, the spectrum not stored without a
+    # monitor) and of the reference's (delta_k stored, three gradient
+    # transforms, four-mesh readout)
+    step_bytes = 20 * Nc + 148 * Np
     ref_bytes = 36 * Nc + 32 * Nk + 148 * Np
     out["step_roofline"] = {"algorithmic_bytes": step_bytes,
                             "achieved_GBs": step_bytes / (ms32 * 1e-3) / 1e9,

@@ -562,7 +562,7 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
     constexpr int RS = NSTG == 1 ? R0 : RL;
     constexpr int PS = NSTG == 1 ? PER0 : PERL;
     constexpr int NBS = NSTG == 1 ? NB0 : NBL;
-    V* kdst = a.kout + (int64_t)outer * a.ostride + z0;
+    // (kout may be null: the forward result is then only handed on)
 #pragma unroll
     for (int q = 0; q < PS; ++q) {
       const int b = threadIdx.x + q * NT;
@@ -572,7 +572,8 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
         for (int r = 0; r < RS; ++r) {
           vf[q][r] = cscale(vf[q][r], a.kscale);
           const int e = NSTG == 1 ? j * R0 + r : j + r * (N / RL);
-          if (z0 + l < a.ncols) kdst[(int64_t)e * a.estride + l] = vf[q][r];
+          if (a.kout && z0 + l < a.ncols)
+            a.kout[(int64_t)outer * a.ostride + z0 + (int64_t)e * a.estride + l] = vf[q][r];
         }
       }
     }
@@ -936,7 +937,7 @@ static void launch_col_fused_n(ColArgs<T> a, hipStream_t st) {
 // with the multiplier and inverse transform of every output.
 template <typename T>
 static int launch_col_fused(int n, ColArgs<T> a, hipStream_t st) {
-  if (a.ncomp < 1 || a.ncomp > 6 || !a.kout) return FPM_ERR_ARG;
+  if (a.ncomp < 1 || a.ncomp > 6) return FPM_ERR_ARG;
   for (int c = 0; c < a.ncomp; ++c)
     if (!a.out[c]) return FPM_ERR_ARG;
   switch (n) {
@@ -1132,14 +1133,17 @@ static int c2r_impl(const void* in, int ncomp, const fpm_kop* ops, void* const* 
 // Forward transform of `rin` into `kout` and inverse transforms of ncomp
 // multiplier outputs, sharing the x pass: the forward x pass of r2c and the
 // first inverse pass of c2r work on the same x columns, so one kernel does
-// both (the forward result is still stored) and the mesh is read once less.
+// both (storing the forward result unless kout is null) and the mesh is read
+// once less.  Without kout the forward passes run in work[0], which the x
+// pass overwrites in place with component 0: each workgroup reads its whole
+// column tile before it writes the same columns.
 template <typename T>
 static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* ops,
                         void* const* outs, void* const* work, const fpm_mesh* m, int64_t pitch2,
                         const void* tw0, const void* tw1, const void* tw2, hipStream_t st) {
   int rc = check_mesh(m, pitch2);
   if (rc) return rc;
-  if (ncomp < 1 || ncomp > 6 || !rin || !kout || !outs || !work || !tw0 || !tw1 || !tw2)
+  if (ncomp < 1 || ncomp > 6 || !rin || !outs || !work || !tw0 || !tw1 || !tw2)
     return FPM_ERR_ARG;
   for (int c = 0; c < ncomp; ++c) {
     if (!outs[c] || !work[c] || work[c] == kout) return FPM_ERR_ARG;
@@ -1148,7 +1152,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   }
   const int n0 = m->nmesh[0], n1 = m->nmesh[1], n2 = m->nmesh[2];
   const int nh = n2 / 2 + 1;
-  C<T>* ck = static_cast<C<T>*>(kout);
+  C<T>* ck = static_cast<C<T>*>(kout ? kout : work[0]);
   ZArgs<T> z{};
   z.rin = static_cast<const T*>(rin);
   z.cout = ck;
@@ -1171,7 +1175,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   FPM_LAUNCH_CHECK();
   ColArgs<T> x{};
   x.in = ck;
-  x.kout = ck;
+  x.kout = static_cast<C<T>*>(kout);
   x.kscale = (T)(1.0 / ((double)n0 * n1 * n2));
   for (int c = 0; c < ncomp; ++c) x.out[c] = static_cast<C<T>*>(work[c]);
   x.ncomp = ncomp;

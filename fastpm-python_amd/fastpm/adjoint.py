@@ -136,10 +136,10 @@ def force_vjp(step, S, Q, gF, gS):
         rho = pm.paint_tiled(parts, mass=1.0 / nbar)
     else:
         rho = pm.paint(parts, mass=1.0 / nbar)
-    kbuf = ComplexField(pm)
     phi = RealField(pm)
     work = [pm._complex_tensor()]
-    r2c_c2r_many(rho, [common], kout=kbuf, outs=[phi], work=work)
+    # the spectra themselves are not needed (kout=False)
+    r2c_c2r_many(rho, [common], kout=False, outs=[phi], work=work)
     m = [RealField(pm) for _ in range(3)]
     grad4_mesh(pm, phi, m)
     # readout's position derivative
@@ -153,7 +153,7 @@ def force_vjp(step, S, Q, gF, gS):
     gphi = rho  # rho is no longer needed
     grad4_div(pm, m, gphi, scale=-1.0)
     grho = phi
-    r2c_c2r_many(gphi, [common], kout=kbuf, outs=[grho], work=work)
+    r2c_c2r_many(gphi, [common], kout=False, outs=[grho], work=work)
     # paint's position derivative (rho = paint(X) / nbar)
     window_grad(pm, parts, [grho], None, gS, 1.0 / nbar, accumulate=True)
 

@@ -243,13 +243,23 @@ class FastPMStep:
         # transform of the potential instead of three)
         self.tiled = solver.fused and self.pm.tiles_supported()
         self.stencil = self.tiled and self.pm.grad4_supported()
+        # whether Force stores delta_k; run() clears it for a Force whose
+        # event no monitor receives
+        self.keep_delta_k = True
 
     def _mark(self, name):
         return self.timer.region(name) if self.timer is not None else _NullRegion()
 
     def run(self, action, ai, ac, af, state, monitor):
         actions = dict(K=self.Kick, D=self.Drift, F=self.Force)
-        event = actions[action](state, ai, ac, af)
+        fn = actions[action]
+        # without a monitor the Force's delta_k event is never seen, so the
+        # fused transform need not store the spectrum
+        self.keep_delta_k = monitor is not None
+        try:
+            event = fn(state, ai, ac, af)
+        finally:
+            self.keep_delta_k = True
         if monitor is not None:
             monitor(action, ai, ac, af, state, event)
 
@@ -321,13 +331,15 @@ class FastPMStep:
         three fused multiplier + inverse transforms and one fused
         four-component readout."""
         assert ac == state.a["S"]
-        delta_k = self.force_kernels(state)
+        delta_k = self.force_kernels(state, keep_delta_k=self.keep_delta_k)
         state.a["F"] = af
         return dict(delta_k=delta_k)
 
-    def force_kernels(self, state):
+    def force_kernels(self, state, keep_delta_k=True):
         """Enqueues the force computation (no host-side bookkeeping, so it can
-        be captured in a graph); returns the delta_k field."""
+        be captured in a graph); returns the delta_k field, or None on the
+        stencil path when keep_delta_k is false (the spectrum is then not
+        stored)."""
         ws = self.workspace()
         layout, X1, rho = self.prepare_force(state, smoothing=None)
         F = state.F
@@ -335,7 +347,8 @@ class FastPMStep:
             # delta_k = r2c(rho) and phi = c2r(common * delta_k), sharing the x pass
             phi = ws["force"][0]
             with self._mark("fft"):
-                delta_k, _ = r2c_c2r_many(rho, [self.force_common()], kout=ws["delta_k"],
+                kout = ws["delta_k"] if keep_delta_k else False
+                delta_k, _ = r2c_c2r_many(rho, [self.force_common()], kout=kout,
                                           outs=[phi], work=ws["work"][:1])
             with self._mark("readout"):
                 self.pm.readout_tiled_grad4(phi, X1, F, rho=rho, rho_out=state.RHO)

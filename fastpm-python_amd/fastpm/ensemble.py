@@ -105,7 +105,7 @@ class Ensemble:
                   b.table.data_ptr(), cnt, self.nsteps, 3, 0, st)
         _lib.call("fpm_axpy_table", dt, s.S.data_ptr(), s.P.data_ptr(), n,
                   b.table.data_ptr(), cnt, self.nsteps, 3, 1, st)
-        b.step.force_kernels(s)
+        b.step.force_kernels(s, keep_delta_k=False)
         _lib.call("fpm_axpy_table", dt, s.P.data_ptr(), s.F.data_ptr(), n,
                   b.table.data_ptr(), cnt, self.nsteps, 3, 2, st)
 

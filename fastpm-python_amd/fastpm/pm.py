@@ -482,7 +482,8 @@ def c2r_grad3(cfield, common, outs=None, work=None, grad_order=1):
 
 def r2c_c2r_many(rfield, ops, kout=None, outs=None, work=None):
     """kout = rfield.r2c() and outs[c] = c2r(ops[c] * kout) in one call whose
-    last forward and first inverse pass share a kernel; returns (kout, outs)."""
+    last forward and first inverse pass share a kernel; returns (kout, outs).
+    kout=False skips storing the spectrum (returned as None)."""
     pm = rfield.pm
     n = len(ops)
     if kout is None:
@@ -498,7 +499,10 @@ def r2c_c2r_many(rfield, ops, kout=None, outs=None, work=None):
         else:
             kops[c] = o.fpm_kop() if not isinstance(o, _lib.FpmKop) else o
     tw = pm.twiddles()
-    _lib.call("fpm_fft_r2c_c2r", pm.dtype, rfield.value.data_ptr(), kout._t.data_ptr(), n, kops,
+    if kout is False:
+        kout = None
+    _lib.call("fpm_fft_r2c_c2r", pm.dtype, rfield.value.data_ptr(),
+              kout._t.data_ptr() if kout is not None else None, n, kops,
               _lib.ptr_array([o.value.data_ptr() for o in outs]),
               _lib.ptr_array([w.data_ptr() for w in work]), pm._desc, pm.pitch2,
               tw[0].data_ptr(), tw[1].data_ptr(), tw[2].data_ptr(), _stream())

@@ -450,6 +450,13 @@ def test_r2c_c2r_fused(gpu, dtype, nmesh):
         o0 = O.c2r(ref_k * O.k_laplace(nmesh, box) * 0.45 *
                    (O.k_deconv(nmesh, box, 2) if len(ops) == 1 else 1.0), nmesh)
         assert _rel(outs[0].value.cpu().numpy(), o0) < _tol(dtype) * 8
+        # without storing the spectrum (kout NULL): the same arithmetic, with
+        # the forward passes in work[0] and the x pass in place there
+        none, outs2 = r2c_c2r_many(rf, ops, kout=False)
+        assert none is None
+        for o, o2 in zip(outs, outs2):
+            assert torch.equal(o.value, o2.value)
+        assert torch.equal(rf.value.cpu(), torch.from_numpy(x.astype(dtype)))
 
 
 @pytest.mark.parametrize("dtype", ["f4", "f8"])

@@ -215,3 +215,31 @@ def test_state_restart_bitwise(gpu, tmp_path):
     solver.nbody(back, steps)
     for key in "SPF":
         assert torch.equal(getattr(back, key), getattr(ref, key)), key
+
+
+def test_monitor_sees_delta_k_and_unmonitored_run_matches(gpu):
+    """Float32 stencil path: a run without a monitor (its Forces skip storing
+    the spectrum) ends in the same state, bit for bit, as a run whose monitor
+    receives each Force's delta_k; every delta_k is the forward transform of
+    the painted 1 + delta, so its zero mode is 1 (core.py:161-166,195-210)."""
+    from fastpm.core import leapfrog
+
+    solver, _, Pk, pm = _setup(16, 100.0, 2, "f4")
+    assert solver.nbodystep.stencil
+    dlin = solver.linear(solver.whitenoise(5), Pk)
+    Q = pm.generate_uniform_particle_grid()
+    a = solver.lpt(dlin, Q, 0.1)
+    b = solver.lpt(dlin, Q, 0.1)
+    seen = []
+
+    def monitor(action, ai, ac, af, state, event):
+        if action == "F":
+            seen.append(complex(event["delta_k"].value[0, 0, 0].item()))
+
+    stages = [0.1, 0.4, 0.7, 1.0]
+    solver.nbody(a, leapfrog(stages), monitor=monitor)
+    solver.nbody(b, leapfrog(stages))
+    for key in ("S", "P", "F", "RHO"):
+        assert torch.equal(getattr(a, key), getattr(b, key)), key
+    assert len(seen) == len(stages)
+    assert all(abs(z - 1) < 1e-5 for z in seen), seen

@@ -82,20 +82,19 @@ def run_sim(args, dtype, fused, timed, resampler="cic", a0=0.1):
     W, K = args.warmup, timed
     if W + K > args.nsteps:
         raise SystemExit("--warmup + --steps exceeds the %d steps of the run" % args.nsteps)
-    for act in body[: 4 * W]:
-        step.run(*act, state, None)
+    # the actions of whole steps, as Solver.nbody applies them (no monitor:
+    # kicks and drifts at step boundaries fused)
+    step.run_actions(body[: 4 * W], state)
     timer = KernelTimer()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     step.timer = timer
-    for act in body[4 * W: 4 * (W + K)]:
-        step.run(*act, state, None)
+    step.run_actions(body[4 * W: 4 * (W + K)], state)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     step.timer = None
     kern = timer.summary()
-    for act in body[4 * (W + K):]:
-        step.run(*act, state, None)
+    step.run_actions(body[4 * (W + K):], state)
     torch.cuda.synchronize()
     return dict(state=state, solver=solver, elapsed=elapsed, kern=kern, t_ic=t_ic,
                 npart=len(Q), nmesh=int(solver.fpm.Nmesh[0]))
@@ -175,12 +174,14 @@ def run_ensemble():
 
 def region_bandwidth(kern, Np, Nc, Nk, bpe=4):
     """Algorithmic bytes of each timed region of one KDK step of the float32
-    stencil path (per call) and the rate they imply: kick/drift 36 N_p, bin
+    stencil path (per call) and the rate they imply: kick/drift 36 N_p, the
+    fused kicks + drift of a step boundary 60 N_p (P, F, S in; P, S out), bin
     12 N_p (S read), paint 4 N_c, transforms 8 N_c (density in, potential
     out; without a monitor the spectrum is not stored), readout
     8 N_c + 28 N_p."""
     s = bpe / 4.0
-    model = {"kick": 36 * Np, "drift": 36 * Np, "bin": 12 * Np, "paint": 4 * Nc,
+    model = {"kick": 36 * Np, "drift": 36 * Np, "kick_drift": 60 * Np, "bin": 12 * Np,
+             "paint": 4 * Nc,
              "fft": 8 * Nc, "readout": 8 * Nc + 28 * Np}
     out = {}
     for k, (c, m) in kern.items():
@@ -334,11 +335,12 @@ def main():
                        "frac": ach / HBM_PEAK_GBS, "traffic": _profiled_traffic(kname),
                        "algorithmic_bytes": ro_bytes}
     Nk = r32["nmesh"] ** 2 * (r32["nmesh"] // 2 + 1)
-    # compulsory bytes of this dataflow (one inverse transform of the
-    # potential, stencil in the readout, the spectrum not stored without a
-    # monitor) and of the reference's (delta_k stored, three gradient
+    # compulsory bytes of this dataflow (kicks and drift of a step boundary
+    # in one pass, one inverse transform of the potential, stencil in the
+    # readout, the spectrum not stored without a monitor) and of the
+    # reference's (separate kicks and drift, delta_k stored, three gradient
     # transforms, four-mesh readout)
-    step_bytes = 20 * Nc + 148 * Np
+    step_bytes = 20 * Nc + 100 * Np
     ref_bytes = 36 * Nc + 32 * Nk + 148 * Np
     out["step_roofline"] = {"algorithmic_bytes": step_bytes,
                             "achieved_GBs": step_bytes / (ms32 * 1e-3) / 1e9,

@@ -14,11 +14,34 @@ static unsigned grid_elems(int64_t n) {
   return (unsigned)g;
 }
 
+// y + a x rounded once: every kick and drift update (alone, table-driven or
+// fused) uses this one expression, so fused and separate updates agree bit
+// for bit.
+__device__ __forceinline__ float madd(float a, float x, float y) { return __builtin_fmaf(a, x, y); }
+__device__ __forceinline__ double madd(double a, double x, double y) { return __builtin_fma(a, x, y); }
+
 template <typename T>
 __global__ void k_axpy(T* y, const T* x, int64_t n, T a) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    y[i] += a * x[i];
+    y[i] = madd(a, x[i], y[i]);
+}
+
+// nkick kicks then a drift in one pass: P += k0 F, P += k1 F, S += d P, each
+// rounded as k_axpy rounds it, with P, F and S read once.
+template <typename T>
+__global__ void k_kick_drift(T* S, T* P, const T* F, int64_t n, T k0, T k1, int nkick, T d) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    T p = P[i];
+    if (nkick > 0) {
+      const T f = F[i];
+      p = madd(k0, f, p);
+      if (nkick > 1) p = madd(k1, f, p);
+      P[i] = p;
+    }
+    S[i] = madd(d, p, S[i]);
+  }
 }
 
 // y += table[counter[0] * stride + slot] * x: the coefficient is read on the
@@ -32,7 +55,7 @@ __global__ void k_axpy_table(T* y, const T* x, int64_t n, const double* table,
   const T a = (T)table[(int64_t)row * stride + slot];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    y[i] += a * x[i];
+    y[i] = madd(a, x[i], y[i]);
 }
 
 // y = a y + b x (the damped kick of the glass generator).
@@ -102,6 +125,20 @@ static int axpy_impl(void* y, const void* x, int64_t n, double a, hipStream_t st
   if (n == 0) return FPM_OK;
   hipLaunchKernelGGL(k_axpy<T>, dim3(grid_elems(n)), dim3(256), 0, st, static_cast<T*>(y),
                      static_cast<const T*>(x), n, (T)a);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
+static int kick_drift_impl(void* S, void* P, const void* F, int64_t n, const double* kicks,
+                           int nkick, double drift, hipStream_t st) {
+  if (n < 0 || nkick < 0 || nkick > 2 || (nkick > 0 && !kicks) ||
+      (n > 0 && (!S || !P || (nkick > 0 && !F))))
+    return FPM_ERR_ARG;
+  if (n == 0) return FPM_OK;
+  const T k0 = nkick > 0 ? (T)kicks[0] : (T)0, k1 = nkick > 1 ? (T)kicks[1] : (T)0;
+  hipLaunchKernelGGL(k_kick_drift<T>, dim3(grid_elems(n)), dim3(256), 0, st, static_cast<T*>(S),
+                     static_cast<T*>(P), static_cast<const T*>(F), n, k0, k1, nkick, (T)drift);
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }
@@ -185,6 +222,14 @@ int fpm_axpy_f32(void* y, const void* x, int64_t n, double a, fpm_stream_t s) {
 }
 int fpm_axpy_f64(void* y, const void* x, int64_t n, double a, fpm_stream_t s) {
   return axpy_impl<double>(y, x, n, a, (hipStream_t)s);
+}
+int fpm_kick_drift_f32(void* S, void* P, const void* F, int64_t n, const double* kicks,
+                      int nkick, double drift, fpm_stream_t s) {
+  return kick_drift_impl<float>(S, P, F, n, kicks, nkick, drift, (hipStream_t)s);
+}
+int fpm_kick_drift_f64(void* S, void* P, const void* F, int64_t n, const double* kicks,
+                      int nkick, double drift, fpm_stream_t s) {
+  return kick_drift_impl<double>(S, P, F, n, kicks, nkick, drift, (hipStream_t)s);
 }
 int fpm_axpy_table_f32(void* y, const void* x, int64_t n, const double* table,
                        const int32_t* counter, int rows, int stride, int slot, fpm_stream_t s) {

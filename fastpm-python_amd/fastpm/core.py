@@ -6,6 +6,7 @@ meanings, ownership rules and errors; the arrays of a state are device tensors
 and every particle or mesh operation runs in libfpm's HIP kernels.
 """
 import json
+import ctypes
 import os
 
 import numpy as np
@@ -135,8 +136,11 @@ class Solver:
     compensate : divide the force by the mass-assignment window (off in the
         reference; used by the window-compensated benchmark configurations).
     fused : True runs the force as one multi-output inverse transform and one
-        four-component readout; False runs the plain sequence of the reference
-        (one inverse transform and one readout per component).
+        four-component readout (on float32 tile-divisible meshes: binned
+        paint, one transform of the potential and a stencil readout) and,
+        in nbody without a monitor, each step boundary's kicks and drift as
+        one pass; False runs the plain sequence of the reference (one inverse
+        transform and one readout per component, separate updates).
     autotune : time the transform tilings on the force mesh first and keep
         the faster (fastpm.tune.autotune_fft).
     """
@@ -220,9 +224,7 @@ class Solver:
 
     def nbody(self, state, stepping, monitor=None):
         """Apply an action list; mutates and returns ``state`` (core.py:147-152)."""
-        step = self.nbodystep
-        for action, ai, ac, af in stepping:
-            step.run(action, ai, ac, af, state, monitor)
+        self.nbodystep.run_actions(stepping, state, monitor)
         return state
 
 
@@ -262,6 +264,50 @@ class FastPMStep:
             self.keep_delta_k = True
         if monitor is not None:
             monitor(action, ai, ac, af, state, event)
+
+    def run_actions(self, actions, state, monitor=None):
+        """run() over an action list.  Without a monitor, on a fused solver and
+        when Kick and Drift are this class's own, each run of Kick actions followed by a
+        Drift (K, D at the first step; K, K, D at every later step boundary)
+        is one fused pass (fpm_kick_drift) that gives the separate updates'
+        result bit for bit; every other action, and any sequence whose time
+        tags would fail an assertion, goes through run()."""
+        actions = list(actions)
+        fuse = (monitor is None and self.solver.fused and type(self).Kick is FastPMStep.Kick
+                and type(self).Drift is FastPMStep.Drift)
+        i = 0
+        while i < len(actions):
+            j = i
+            while fuse and j < len(actions) and j - i < 2 and actions[j][0] == "K":
+                j += 1
+            if j > i and j < len(actions) and actions[j][0] == "D" and \
+                    self._kick_drift(state, actions[i:j], actions[j]):
+                i = j + 1
+                continue
+            self.run(*actions[i], state, monitor)
+            i += 1
+
+    def _kick_drift(self, state, kicks, drift):
+        """Fused Kick(s) + Drift; False (nothing done) if a time tag would
+        fail one of their assertions."""
+        aF, aP = state.a["F"], state.a["P"]
+        facs = []
+        for _, ai, ac, af in kicks:
+            if ac != aF:
+                return False
+            facs.append(self._pt.kick_factor(ai, ac, af))
+            aP = af
+        _, ai, ac, af = drift
+        if ac != aP:
+            return False
+        fac = self._pt.drift_factor(ai, ac, af)
+        kc = (ctypes.c_double * 2)(*(facs + [0.0] * (2 - len(facs))))
+        with self._mark("kick_drift"):
+            _lib.call("fpm_kick_drift", state.dtype, state.S.data_ptr(), state.P.data_ptr(),
+                      state.F.data_ptr(), state.S.numel(), kc, len(facs), float(fac), _stream())
+        state.a["P"] = aP
+        state.a["S"] = af
+        return True
 
     def Kick(self, state, ai, ac, af):
         assert ac == state.a["F"]

@@ -261,6 +261,17 @@ int fpm_lpt2_source_f64(const void* const* phi, void* out, int64_t ncells, fpm_s
 int fpm_axpy_f32(void* y, const void* x, int64_t n, double a, fpm_stream_t stream);
 int fpm_axpy_f64(void* y, const void* x, int64_t n, double a, fpm_stream_t stream);
 
+/* Consecutive kicks and a drift in one pass over n elements:
+ * P += kicks[i] * F for i < nkick (0, 1 or 2, coefficients in double), then
+ * S += drift * P, each update rounded exactly as fpm_axpy rounds it, so the
+ * result equals the separate fpm_axpy calls bit for bit.  Replaces the
+ * Kick, [Kick,] Drift sequence of a leapfrog step boundary
+ * (fastpm/core.py:168-180 applied to the actions of core.py:261-288). */
+int fpm_kick_drift_f32(void* S, void* P, const void* F, int64_t n, const double* kicks,
+                       int nkick, double drift, fpm_stream_t stream);
+int fpm_kick_drift_f64(void* S, void* P, const void* F, int64_t n, const double* kicks,
+                       int nkick, double drift, fpm_stream_t stream);
+
 /* y = a y + b x over n elements: the damped kick of the glass generator,
  * P = P + (af - ai) (F - P) (fastpm/glass.py:89-93). */
 int fpm_axpby_f32(void* y, const void* x, int64_t n, double a, double b, fpm_stream_t stream);

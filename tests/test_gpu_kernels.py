@@ -3,6 +3,8 @@
 Tolerances: float64 storage must agree to rounding (rtol 1e-11 relative to the
 field's scale); float32 storage to a few float32 ulps of the field's scale
 (documented per test)."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -522,3 +524,69 @@ def test_paint_tiled_weighted(gpu, order, nmesh, kind):
         # cancel in the clustered cell)
         scale = O.paint(X, nmesh, box, order, mass=0.5 * np.abs(Wh[:, c])).max()
         assert np.abs(outs[c].value.cpu().numpy() - ref).max() < 1e-6 * scale, c
+
+
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+@pytest.mark.parametrize("nkick", [0, 1, 2])
+def test_kick_drift_fused_bitwise(gpu, dtype, nkick):
+    """fpm_kick_drift equals nkick fpm_axpy kicks then an fpm_axpy drift bit
+    for bit, and both match P += k F (per kick), S += d P in float64."""
+    from fastpm import _lib
+    from fastpm.pm import _stream
+
+    rng = np.random.default_rng(21 + nkick)
+    n = 3 * 12345
+    S0, P0, F0 = (torch.from_numpy(rng.standard_normal(n).astype(dtype)).cuda() for _ in range(3))
+    kicks, drift = [0.37, -1.25][:nkick], 0.81
+    S1, P1 = S0.clone(), P0.clone()
+    for k in kicks:
+        _lib.call("fpm_axpy", dtype, P1.data_ptr(), F0.data_ptr(), n, k, _stream())
+    _lib.call("fpm_axpy", dtype, S1.data_ptr(), P1.data_ptr(), n, drift, _stream())
+    S2, P2 = S0.clone(), P0.clone()
+    kc = (ctypes.c_double * 2)(*(kicks + [0.0] * (2 - nkick)))
+    _lib.call("fpm_kick_drift", dtype, S2.data_ptr(), P2.data_ptr(), F0.data_ptr(), n, kc, nkick,
+              drift, _stream())
+    assert torch.equal(S1, S2) and torch.equal(P1, P2)
+    p = P0.double().cpu().numpy()
+    f = F0.double().cpu().numpy()
+    for k in kicks:
+        p = p + k * f
+    s = S0.double().cpu().numpy() + drift * p
+    assert _rel(P2.double().cpu().numpy(), p) < _tol(dtype) * 4
+    assert _rel(S2.double().cpu().numpy(), s) < _tol(dtype) * 4
+    # argument checks
+    with pytest.raises(RuntimeError):
+        _lib.call("fpm_kick_drift", dtype, S2.data_ptr(), P2.data_ptr(), F0.data_ptr(), n, kc, 3,
+                  drift, _stream())
+
+
+def test_nbody_fused_steps_bitwise(gpu):
+    """Solver.nbody (kicks and drift of each step boundary fused, spectrum
+    not stored) ends in the state of action-by-action run() calls: bit for
+    bit on the float32 stencil path (deterministic binned paint), to rounding
+    on the float64 path, whose atomic paint sums in arrival order."""
+    from fastpm.core import Solver, leapfrog
+    from fastpm.pm import ParticleMesh
+    from fastpm.cosmology import Cosmology, LinearPower
+
+    cosmo = Cosmology(Om0=0.3, h=0.7, n_s=0.96, sigma8=0.8)
+    Pk = LinearPower(cosmo)
+    for dtype, nc in (("f4", 16), ("f8", 8)):
+        pm = ParticleMesh(Nmesh=[nc] * 3, BoxSize=100.0, dtype=dtype)
+        solver = Solver(pm, cosmo, B=2)
+        dlin = solver.linear(solver.whitenoise(8), Pk)
+        Q = pm.generate_uniform_particle_grid()
+        a = solver.lpt(dlin, Q, 0.1)
+        b = solver.lpt(dlin, Q, 0.1)
+        stages = np.linspace(0.1, 1.0, 5)
+        solver.nbody(a, leapfrog(stages))
+        step = solver.nbodystep
+        for act in leapfrog(stages):
+            step.run(*act, b, None)
+        assert a.a == b.a
+        for key in ("S", "P", "F", "RHO"):
+            x, y = getattr(a, key), getattr(b, key)
+            if dtype == "f4":
+                assert torch.equal(x, y), (dtype, key)
+            else:
+                assert ((x - y).abs().max() / y.abs().max()).item() < 1e-11, (dtype, key)

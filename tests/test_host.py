@@ -95,3 +95,38 @@ def test_gpu_required():
         pytest.skip("a GPU is visible")
     with pytest.raises(RuntimeError):
         ParticleMesh(Nmesh=8, BoxSize=1.0)
+
+
+def test_run_actions_groups_kicks_and_drift():
+    """FastPMStep.run_actions fuses each K[,K],D run when no monitor is given
+    (K D at the first step, K K D at every later boundary, the last K alone),
+    and runs everything one by one with a monitor, on an unfused solver, or
+    when a time tag would fail an assertion.  The step's methods are replaced
+    by recorders, so no GPU is needed."""
+    from fastpm.core import FastPMStep, leapfrog
+
+    class Rec(FastPMStep):
+        def __init__(self, fused=True, refuse=False):
+            self.solver = type("S", (), {"fused": fused})()
+            self.log = []
+            self.refuse = refuse
+
+        def run(self, action, ai, ac, af, state, monitor):
+            self.log.append(action)
+
+        def _kick_drift(self, state, kicks, drift):
+            if self.refuse:
+                return False
+            self.log.append("".join(k[0] for k in kicks) + drift[0])
+            return True
+
+    acts = list(leapfrog([0.1, 0.4, 0.7, 1.0]))
+    s = Rec()
+    s.run_actions(acts, None)
+    assert s.log == ["F", "KD", "F", "KKD", "F", "KKD", "F", "K"]
+    for step in (Rec(fused=False), Rec(refuse=True)):
+        step.run_actions(iter(acts), None)
+        assert step.log == [a[0] for a in acts]
+    s = Rec()
+    s.run_actions(acts, None, monitor=lambda *args: None)
+    assert s.log == [a[0] for a in acts]

@@ -38,8 +38,14 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nblk) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// Periodic wrap of a cell index into [0, n).  Indices within one period of
+// the box (every particle the solver moves) take the compare-and-add paths;
+// only others pay for the 64-bit remainder.
 __device__ __forceinline__ int64_t wrapi(int64_t i, int64_t n) {
-  int64_t r = i % n;
+  if (i >= 0 && i < n) return i;
+  if (i < 0 && i >= -n) return i + n;
+  if (i >= n && i < 2 * n) return i - n;
+  const int64_t r = i % n;
   return r < 0 ? r + n : r;
 }
 
@@ -58,27 +64,34 @@ struct PartDesc {
   T inv_cell[3];      // Nmesh / BoxSize
 };
 
+// Lattice coordinates of lattice particle p (C order over P.lat).
+template <typename T>
+__device__ __forceinline__ void lattice_of(const PartDesc<T>& P, int64_t p, int32_t li[3]) {
+  if (p <= 0x7fffffffLL) {  // 32-bit index arithmetic (far cheaper than 64-bit division)
+    const uint32_t nz = (uint32_t)P.lat[2], ny = (uint32_t)P.lat[1];
+    const uint32_t pu = (uint32_t)p;
+    const uint32_t t = pu / nz;
+    li[2] = (int32_t)(pu - t * nz);
+    li[0] = (int32_t)(t / ny);
+    li[1] = (int32_t)(t - (uint32_t)li[0] * ny);
+  } else {
+    const int64_t nz = P.lat[2], ny = P.lat[1];
+    const int64_t t = p / nz;
+    li[2] = (int32_t)(p - t * nz);
+    li[0] = (int32_t)(t / ny);
+    li[1] = (int32_t)(t - (int64_t)li[0] * ny);
+  }
+}
+
+// Cells are exact for positions within 2^31 cells of the box.
 template <typename T>
 __device__ __forceinline__ void particle_cell(const PartDesc<T>& P, int64_t p, int64_t cell[3], T frac[3]) {
   // No FMA contraction: every kernel that places a particle (binning, paint,
   // readout) must round its position identically.
 #pragma clang fp contract(off)
   if (P.q == nullptr) {
-    int64_t li[3];
-    if (p <= 0x7fffffffLL) {  // 32-bit index arithmetic (far cheaper than 64-bit division)
-      const uint32_t nz = (uint32_t)P.lat[2], ny = (uint32_t)P.lat[1];
-      const uint32_t pu = (uint32_t)p;
-      const uint32_t t = pu / nz;
-      li[2] = pu - t * nz;
-      li[0] = t / ny;
-      li[1] = t - (uint32_t)li[0] * ny;
-    } else {
-      const int64_t nz = P.lat[2], ny = P.lat[1];
-      const int64_t t = p / nz;
-      li[2] = p - t * nz;
-      li[0] = t / ny;
-      li[1] = t - li[0] * ny;
-    }
+    int32_t li[3];
+    lattice_of(P, p, li);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double base = ((double)li[d] + P.lat_shift) * P.ratio[d];
@@ -86,7 +99,7 @@ __device__ __forceinline__ void particle_cell(const PartDesc<T>& P, int64_t p, i
       T t2 = (T)(base - fb);
       if (P.s) t2 += P.s[3 * p + d] * P.inv_cell[d];
       const T ft = floor(t2);
-      cell[d] = (int64_t)fb + (int64_t)ft;
+      cell[d] = (int64_t)(int32_t)fb + (int64_t)(int32_t)ft;
       frac[d] = t2 - ft;
     }
   } else {
@@ -96,11 +109,17 @@ __device__ __forceinline__ void particle_cell(const PartDesc<T>& P, int64_t p, i
       if (P.s) x += P.s[3 * p + d];
       x *= P.inv_cell[d];
       const T fx = floor(x);
-      cell[d] = (int64_t)fx;
+      cell[d] = (int64_t)(int32_t)fx;
       frac[d] = x - fx;
     }
   }
 }
+
+// y + a x rounded once: every kick and drift update (alone, table-driven or
+// fused) uses this one expression, so fused and separate updates agree bit
+// for bit.
+__device__ __forceinline__ float madd(float a, float x, float y) { return __builtin_fmaf(a, x, y); }
+__device__ __forceinline__ double madd(double a, double x, double y) { return __builtin_fma(a, x, y); }
 
 }  // namespace fpm
 

@@ -14,12 +14,6 @@ static unsigned grid_elems(int64_t n) {
   return (unsigned)g;
 }
 
-// y + a x rounded once: every kick and drift update (alone, table-driven or
-// fused) uses this one expression, so fused and separate updates agree bit
-// for bit.
-__device__ __forceinline__ float madd(float a, float x, float y) { return __builtin_fmaf(a, x, y); }
-__device__ __forceinline__ double madd(double a, double x, double y) { return __builtin_fma(a, x, y); }
-
 template <typename T>
 __global__ void k_axpy(T* y, const T* x, int64_t n, T a) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;

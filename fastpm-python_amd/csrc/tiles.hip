@@ -169,11 +169,15 @@ __device__ __forceinline__ void window_rec(const PRec& r, int l[3], T w[3][ORDER
 
 template <int ORDER>
 __device__ __forceinline__ int32_t bin_of(const int i0[3], const TileGeom& g) {
-  const int64_t t = ((int64_t)(i0[0] / TILE) * g.nt[1] + i0[1] / TILE) * g.nt[2] + i0[2] / TILE;
-  int mask = 0;
+  // i0 is wrapped (non-negative) and the bin count is below 2^31 (checked
+  // by the host), so 32-bit unsigned arithmetic is exact
+  const uint32_t u[3] = {(uint32_t)i0[0], (uint32_t)i0[1], (uint32_t)i0[2]};
+  const uint32_t t = ((u[0] / TILE) * (uint32_t)g.nt[1] + u[1] / TILE) * (uint32_t)g.nt[2] +
+                     u[2] / TILE;
+  uint32_t mask = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d)
-    if (i0[d] % TILE > TILE - ORDER) mask |= 1 << d;
+    if (u[d] % TILE > (uint32_t)(TILE - ORDER)) mask |= 1u << d;
   return (int32_t)(t * kSub + mask);
 }
 
@@ -202,7 +206,8 @@ struct BinOrder {
 
 __device__ __forceinline__ int64_t bin_particle(const BinOrder& o, int64_t i) {
   if (!o.blocked) return i;
-  const uint32_t w = (uint32_t)(i >> 6), lane = (uint32_t)(i & 63);
+  const uint32_t w = (uint32_t)(i >> 6);
+  const uint32_t lane = (uint32_t)(i & 63);
   const uint32_t t = w / o.nz8, pz = w - t * o.nz8;
   const uint32_t ix = t / o.ny8, py = t - ix * o.ny8;
   return ((int64_t)ix * o.lat1 + py * 8 + (lane >> 3)) * o.lat2 + pz * 8 + (lane & 7);
@@ -212,38 +217,71 @@ __device__ __forceinline__ int64_t bin_particle(const BinOrder& o, int64_t i) {
 // the counters, so each distinct sub-bin in a wave costs one atomic.  The
 // count before the add plus the lane's rank among its group is the
 // particle's slot within its sub-bin; the results are consumed after the
-// merge loop, so all of a wave's atomics are in flight together.
-template <typename T, int ORDER>
-__global__ void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w, BinOrder o) {
+// merge loops, so all of a wave's atomics are in flight together.  Each
+// thread takes kBinILP particles (one per 256-particle stripe of its
+// block's span), whose position loads and atomics overlap: the pass is
+// bound by their latency, not by bandwidth.
+#ifndef FPM_BIN_ILP
+#define FPM_BIN_ILP 2
+#endif
+constexpr int kBinILP = FPM_BIN_ILP;
+constexpr int kBinThreads = 256;
+
+// Merges a wave's sub-bin keys (t[u] < 0: no particle) and adds each group
+// to its counter; rank[i] = the counter before the add + the lane's rank in
+// its group.
+__device__ __forceinline__ void count_ranks(const int32_t t[kBinILP], int64_t base, int64_t n,
+                                            const Workspace& w) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < P.n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = base + threadIdx.x;
-    const bool valid = i < P.n;
-    int32_t t = -1;
-    if (valid) {
-      int i0[3];
-      T wt[3][ORDER];
-      window_of<T, ORDER>(P, bin_particle(o, i), g.nm, i0, wt);
-      t = bin_of<ORDER>(i0, g);
-    }
+  int32_t old[kBinILP];
+  int myleader[kBinILP], myrank[kBinILP];
+#pragma unroll
+  for (int u = 0; u < kBinILP; ++u) {
+    const bool valid = t[u] >= 0;
     unsigned long long active = __ballot(valid);
-    int myleader = 0, myrank = 0;
-    int32_t old = 0;
+    myleader[u] = 0;
+    myrank[u] = 0;
+    old[u] = 0;
     while (active) {
       const int leader = __ffsll(active) - 1;
-      const int32_t tl = __shfl(t, leader);
-      const unsigned long long same = __ballot(valid && t == tl);
-      if (lane == leader) old = atomicAdd(&w.count[tl], (int32_t)__popcll(same));
-      if (valid && t == tl) {
-        myleader = leader;
-        myrank = __popcll(same & below);
+      const int32_t tl = __shfl(t[u], leader);
+      const unsigned long long same = __ballot(valid && t[u] == tl);
+      if (lane == leader) old[u] = atomicAdd(&w.count[tl], (int32_t)__popcll(same));
+      if (valid && t[u] == tl) {
+        myleader[u] = leader;
+        myrank[u] = __popcll(same & below);
       }
       active &= ~same;
     }
-    const int32_t r0 = __shfl(old, myleader);
-    if (valid) w.rank[i] = r0 + myrank;
+  }
+#pragma unroll
+  for (int u = 0; u < kBinILP; ++u) {
+    const int64_t i = base + u * kBinThreads + threadIdx.x;
+    const int32_t r0 = __shfl(old[u], myleader[u]);
+    if (i < n) w.rank[i] = r0 + myrank[u];
+  }
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kBinThreads) void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w,
+                                                           BinOrder o) {
+  const int64_t span = (int64_t)kBinThreads * kBinILP;
+  for (int64_t base = blockIdx.x * span; base < P.n; base += (int64_t)gridDim.x * span) {
+    int32_t t[kBinILP];
+#pragma unroll
+    for (int u = 0; u < kBinILP; ++u) {
+      const int64_t i = base + u * kBinThreads + threadIdx.x;
+      t[u] = -1;
+      if (i < P.n) {
+        const int64_t p = bin_particle(o, i);
+        int i0[3];
+        T tt[3];
+        window_t<T, ORDER>(P, p, g.nm, i0, tt);
+        t[u] = bin_of<ORDER>(i0, g);
+      }
+    }
+    count_ranks(t, base, P.n, w);
   }
 }
 
@@ -311,27 +349,41 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int6
 // record (window offsets and tile-local origin), so paint and readout read
 // their tile's particles contiguously and need no positions.
 template <typename T, int ORDER>
-__global__ void k_bin_scatter(PartDesc<T> P, TileGeom g, Workspace w, BinOrder o) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P.n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = bin_particle(o, i);
-    const int32_t rank = w.rank[i];
-    int i0[3];
-    T t[3];
-    window_t<T, ORDER>(P, p, g.nm, i0, t);
-    const int32_t slot = w.offs[bin_of<ORDER>(i0, g)] + rank;
-    PRec r;
-    r.t[0] = (float)t[0];
-    r.t[1] = (float)t[1];
-    r.t[2] = (float)t[2];
-    r.cells = (uint32_t)(i0[0] % TILE) | (uint32_t)(i0[1] % TILE) << 4 |
-              (uint32_t)(i0[2] % TILE) << 8;
-    // the count placed the particle with the same arithmetic (no FMA
-    // contraction in particle_cell), so the slot is in its sub-bin; the guard
+__global__ __launch_bounds__(kBinThreads) void k_bin_scatter(PartDesc<T> P, TileGeom g,
+                                                             Workspace w, BinOrder o) {
+  const int64_t span = (int64_t)kBinThreads * kBinILP;
+  for (int64_t base = blockIdx.x * span; base < P.n; base += (int64_t)gridDim.x * span) {
+    int32_t slot[kBinILP];
+    int32_t pp[kBinILP];
+    PRec r[kBinILP];
+#pragma unroll
+    for (int u = 0; u < kBinILP; ++u) {
+      const int64_t i = base + u * kBinThreads + threadIdx.x;
+      slot[u] = -1;
+      if (i < P.n) {
+        const int64_t p = bin_particle(o, i);
+        const int32_t rank = w.rank[i];
+        int i0[3];
+        T t[3];
+        window_t<T, ORDER>(P, p, g.nm, i0, t);
+        slot[u] = w.offs[bin_of<ORDER>(i0, g)] + rank;
+        pp[u] = (int32_t)p;
+        r[u].t[0] = (float)t[0];
+        r[u].t[1] = (float)t[1];
+        r[u].t[2] = (float)t[2];
+        r[u].cells = (uint32_t)(i0[0] % TILE) | (uint32_t)(i0[1] % TILE) << 4 |
+                     (uint32_t)(i0[2] % TILE) << 8;
+      }
+    }
+    // the count placed each particle with the same arithmetic (no FMA
+    // contraction in particle_cell), so its slot is in its sub-bin; the guard
     // keeps the stores in bounds regardless
-    if (slot >= 0 && slot < P.n) {
-      w.list[slot] = (int32_t)p;
-      w.rec[slot] = r;
+#pragma unroll
+    for (int u = 0; u < kBinILP; ++u) {
+      if (slot[u] >= 0 && slot[u] < P.n) {
+        w.list[slot[u]] = pp[u];
+        w.rec[slot[u]] = r[u];
+      }
     }
   }
 }
@@ -946,6 +998,14 @@ static unsigned grid_parts(int64_t n) {
   return (unsigned)g;
 }
 
+static unsigned grid_bin(int64_t n) {
+  const int64_t span = (int64_t)kBinThreads * kBinILP;
+  int64_t g = (n + span - 1) / span;
+  if (g < 1) g = 1;
+  if (g > (1 << 20)) g = 1 << 20;
+  return (unsigned)g;
+}
+
 template <typename T>
 static int prep(const fpm_particles* part, const fpm_mesh* m, int order, void* ws, int64_t ws_bytes,
                 PartDesc<T>& P, TileGeom& g, Workspace& w) {
@@ -960,14 +1020,7 @@ static int prep(const fpm_particles* part, const fpm_mesh* m, int order, void* w
   return FPM_OK;
 }
 
-template <typename T>
-static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, void* ws,
-                    int64_t ws_bytes, hipStream_t st) {
-  PartDesc<T> P;
-  TileGeom g;
-  Workspace w;
-  int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
-  if (rc) return rc;
+static BinOrder bin_order(const fpm_particles* part) {
   BinOrder o{};
   if (!part->q && part->lattice[2] % 8 == 0 && part->lattice[1] % 8 == 0) {
     o.blocked = 1;
@@ -976,25 +1029,52 @@ static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, voi
     o.lat1 = (uint32_t)part->lattice[1];
     o.lat2 = (uint32_t)part->lattice[2];
   }
+  return o;
+}
+
+static void bin_clear(const TileGeom& g, const Workspace& w, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_i32, dim3(grid_parts(g.nbins)), dim3(256), 0, st, w.count, g.nbins);
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, w.misplaced, (int64_t)1);
-  if (P.n > 0) {
-    if (order == 2)
-      hipLaunchKernelGGL((k_bin_count<T, 2>), dim3(grid_parts(P.n)), dim3(256), 0, st, P, g, w, o);
-    else
-      hipLaunchKernelGGL((k_bin_count<T, 3>), dim3(grid_parts(P.n)), dim3(256), 0, st, P, g, w, o);
-  }
+}
+
+// the scan of the counts and the scatter, after a count pass
+template <typename T>
+static int bin_finish(const PartDesc<T>& P, const TileGeom& g, const Workspace& w,
+                      const BinOrder& o, int order, hipStream_t st) {
   const unsigned nsb = (unsigned)scan_blocks(g);
   hipLaunchKernelGGL(k_bin_scan_local, dim3(nsb), dim3(kScanThreads), 0, st, w, g.nbins);
   hipLaunchKernelGGL(k_bin_scan_add, dim3(nsb), dim3(kScanThreads), 0, st, w, g.nbins);
   if (P.n > 0) {
     if (order == 2)
-      hipLaunchKernelGGL((k_bin_scatter<T, 2>), dim3(grid_parts(P.n)), dim3(256), 0, st, P, g, w, o);
+      hipLaunchKernelGGL((k_bin_scatter<T, 2>), dim3(grid_bin(P.n)), dim3(kBinThreads), 0, st, P,
+                         g, w, o);
     else
-      hipLaunchKernelGGL((k_bin_scatter<T, 3>), dim3(grid_parts(P.n)), dim3(256), 0, st, P, g, w, o);
+      hipLaunchKernelGGL((k_bin_scatter<T, 3>), dim3(grid_bin(P.n)), dim3(kBinThreads), 0, st, P,
+                         g, w, o);
   }
   FPM_LAUNCH_CHECK();
   return FPM_OK;
+}
+
+template <typename T>
+static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, void* ws,
+                    int64_t ws_bytes, hipStream_t st) {
+  PartDesc<T> P;
+  TileGeom g;
+  Workspace w;
+  int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
+  if (rc) return rc;
+  const BinOrder o = bin_order(part);
+  bin_clear(g, w, st);
+  if (P.n > 0) {
+    if (order == 2)
+      hipLaunchKernelGGL((k_bin_count<T, 2>), dim3(grid_bin(P.n)), dim3(kBinThreads), 0, st, P, g,
+                         w, o);
+    else
+      hipLaunchKernelGGL((k_bin_count<T, 3>), dim3(grid_bin(P.n)), dim3(kBinThreads), 0, st, P, g,
+                         w, o);
+  }
+  return bin_finish<T>(P, g, w, o, order, st);
 }
 
 template <typename T>

@@ -43,7 +43,9 @@ typedef struct hipStream_t_opaque* fpm_stream_t; /* ABI-compatible with hipStrea
 
 typedef struct fpm_particles {
   const void* q;          /* explicit Lagrangian positions [n][3], or NULL for the lattice */
-  const void* s;          /* displacement [n][3], or NULL for zero */
+  const void* s;          /* displacement [n][3], or NULL for zero; positions are
+                             wrapped periodically and must lie within 2^31 cells
+                             of the box */
   int64_t n;              /* particle count */
   int32_t lattice[3];     /* lattice particles per side (used when q == NULL) */
   int32_t pad_;

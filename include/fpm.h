@@ -148,8 +148,10 @@ int fpm_readout_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int o
  * outs[c](cell) = scale * sum_p weights[c][p * wstrides[c]] W(x_p - cell), the
  * adjoint of a readout (readout_gradient of fastpm/operators.py:236-247, as
  * used by gravity_gradient).  Uses the binning of the last
- * fpm_bin_particles_f32 call; accumulates in float in LDS (summation order
- * may vary between runs); each output cell is written once. */
+ * fpm_bin_particles_f32 call; accumulates each component in 64-bit fixed
+ * point in LDS, scaled per tile by the count of depositing particles and
+ * their largest |weight|, so the result is bitwise reproducible; weights
+ * must be finite; each output cell is written once. */
 int fpm_paint_tiled_weighted_f32(const fpm_particles* part, const fpm_mesh* mesh, int order,
                                  int ncomp, const void* const* weights, const int64_t* wstrides,
                                  double scale, void* const* outs, void* ws, int64_t ws_bytes,

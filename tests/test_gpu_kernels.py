@@ -520,10 +520,19 @@ def test_paint_tiled_weighted(gpu, order, nmesh, kind):
     Wh = W.cpu().numpy().astype("f8")
     for c in range(3):
         ref = O.paint(X, nmesh, box, order, mass=0.5 * Wh[:, c])
-        # float accumulation: bounded by the paint of |w| (signed weights
-        # cancel in the clustered cell)
+        # fixed-point sums, each deposit rounded once in float: bounded by the
+        # paint of |w| (signed weights cancel in the clustered cell)
         scale = O.paint(X, nmesh, box, order, mass=0.5 * np.abs(Wh[:, c])).max()
         assert np.abs(outs[c].value.cpu().numpy() - ref).max() < 1e-6 * scale, c
+    # exact integer sums: bitwise the same on a repeat, and component c of a
+    # three-component call equals a one-component call of the same weights
+    again = [pm.create("real") for _ in range(3)]
+    pm.paint_tiled_weighted(parts, [W[:, 0], W[:, 1], W[:, 2]], again, scale=0.5)
+    for c in range(3):
+        assert torch.equal(outs[c].value, again[c].value), c
+    one = [pm.create("real")]
+    pm.paint_tiled_weighted(parts, [W[:, 1]], one, scale=0.5)
+    assert torch.equal(one[0].value, outs[1].value)
 
 
 @pytest.mark.parametrize("dtype", ["f8", "f4"])

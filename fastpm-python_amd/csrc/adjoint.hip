@@ -24,6 +24,13 @@ static unsigned grid_n(int64_t n, int64_t cap = 1 << 20) {
   return (unsigned)g;
 }
 
+// threads of a per-row workgroup: the row length rounded up to a whole
+// wave, at most 256
+static unsigned row_threads(int n2) {
+  const int t = (n2 + 63) / 64 * 64;
+  return (unsigned)(t > 256 ? 256 : (t < 64 ? 64 : t));
+}
+
 // Window origin, weights and weight derivatives (per unit of the mesh-cell
 // coordinate) for one axis.
 template <typename T, int ORDER>
@@ -344,32 +351,48 @@ static int lpt2_vjp_impl(const void* gs, const void* const* phi, void* const* gp
 // The four-point gradient as a real-space stencil,
 //   D_d f(x) = [8 (f(x+h) - f(x-h)) - (f(x+2h) - f(x-2h))] / (12 h),
 // equal to the k-space multiplier i (8 sin kh - sin 2kh) / (6h) (the force's
-// order-1 gradient) and antisymmetric (D_d^T = -D_d).  One thread per cell,
-// periodic wrap; neighbours come through L1/L2.
+// order-1 gradient) and antisymmetric (D_d^T = -D_d).  A workgroup per mesh
+// row (x, y) and a thread per z, so the only index arithmetic is one 32-bit
+// division per workgroup and the periodic wraps; neighbours come through
+// L1/L2.
+__device__ __forceinline__ int wrap1(int j, int n) {
+  j = j < 0 ? j + n : (j >= n ? j - n : j);
+  return j < 0 ? j + n : (j >= n ? j - n : j);
+}
+
 template <typename T>
 __device__ __forceinline__ T d4(const T* f, int64_t base, int i, int n, int64_t stride) {
-  auto at = [&](int s) {
-    int j = i + s;
-    j = j < 0 ? j + n : (j >= n ? j - n : j);
-    j = j < 0 ? j + n : (j >= n ? j - n : j);
-    return f[base + (int64_t)j * stride];
-  };
+  auto at = [&](int s) { return f[base + (int64_t)wrap1(i + s, n) * stride]; };
   return (T)8 * (at(1) - at(-1)) - (at(2) - at(-2));
+}
+
+// stencil row geometry: the row (x, y) of this workgroup
+struct Rows {
+  int x, y;
+  int64_t sx, sy;  // strides of x and y
+  int64_t row;     // offset of (x, y, 0)
+};
+
+__device__ __forceinline__ Rows row_of(int n1, int n2) {
+  Rows r;
+  const uint32_t b = blockIdx.x;
+  r.x = (int)(b / (uint32_t)n1);
+  r.y = (int)(b - (uint32_t)r.x * (uint32_t)n1);
+  r.sy = n2;
+  r.sx = (int64_t)n1 * n2;
+  r.row = r.x * r.sx + r.y * r.sy;
+  return r;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_grad4_mesh(const T* phi, T* o0, T* o1, T* o2, int n0,
                                                     int n1, int n2, T s0, T s1, T s2) {
-  const int64_t ncell = (int64_t)n0 * n1 * n2;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell;
-       c += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(c % n2);
-    const int64_t t = c / n2;
-    const int y = (int)(t % n1), x = (int)(t / n1);
-    const int64_t bx = c - (int64_t)x * n1 * n2, by = c - (int64_t)y * n2, bz = c - z;
-    o0[c] = s0 * d4(phi, bx, x, n0, (int64_t)n1 * n2);
-    o1[c] = s1 * d4(phi, by, y, n1, (int64_t)n2);
-    o2[c] = s2 * d4(phi, bz, z, n2, (int64_t)1);
+  const Rows r = row_of(n1, n2);
+  for (int z = threadIdx.x; z < n2; z += blockDim.x) {
+    const int64_t c = r.row + z;
+    o0[c] = s0 * d4(phi, c - r.x * r.sx, r.x, n0, r.sx);
+    o1[c] = s1 * d4(phi, c - r.y * r.sy, r.y, n1, r.sy);
+    o2[c] = s2 * d4(phi, r.row, z, n2, (int64_t)1);
   }
 }
 
@@ -377,15 +400,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_grad4_div(const T* g0, const T* g1, const T* g2, T* out,
                                                    int n0, int n1, int n2, T s0, T s1, T s2,
                                                    int accumulate) {
-  const int64_t ncell = (int64_t)n0 * n1 * n2;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell;
-       c += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(c % n2);
-    const int64_t t = c / n2;
-    const int y = (int)(t % n1), x = (int)(t / n1);
-    const int64_t bx = c - (int64_t)x * n1 * n2, by = c - (int64_t)y * n2, bz = c - z;
-    const T v = s0 * d4(g0, bx, x, n0, (int64_t)n1 * n2) + s1 * d4(g1, by, y, n1, (int64_t)n2) +
-                s2 * d4(g2, bz, z, n2, (int64_t)1);
+  const Rows r = row_of(n1, n2);
+  for (int z = threadIdx.x; z < n2; z += blockDim.x) {
+    const int64_t c = r.row + z;
+    const T v = s0 * d4(g0, c - r.x * r.sx, r.x, n0, r.sx) +
+                s1 * d4(g1, c - r.y * r.sy, r.y, n1, r.sy) + s2 * d4(g2, r.row, z, n2, (int64_t)1);
     out[c] = accumulate ? out[c] + v : v;
   }
 }
@@ -398,8 +417,9 @@ static int grad4_mesh_impl(const void* phi, void* const* outs, const fpm_mesh* m
     if (m->nmesh[d] <= 0 || !(m->box[d] > 0)) return FPM_ERR_ARG;
   T s[3];
   for (int d = 0; d < 3; ++d) s[d] = (T)(scale * m->nmesh[d] / (12.0 * m->box[d]));
-  const int64_t ncell = (int64_t)m->nmesh[0] * m->nmesh[1] * m->nmesh[2];
-  hipLaunchKernelGGL(k_grad4_mesh<T>, dim3(grid_n(ncell)), dim3(256), 0, st,
+  const int64_t nrows = (int64_t)m->nmesh[0] * m->nmesh[1];
+  if (nrows >= ((int64_t)1 << 31)) return FPM_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(k_grad4_mesh<T>, dim3((unsigned)nrows), dim3(row_threads(m->nmesh[2])), 0, st,
                      static_cast<const T*>(phi), static_cast<T*>(outs[0]),
                      static_cast<T*>(outs[1]), static_cast<T*>(outs[2]), m->nmesh[0],
                      m->nmesh[1], m->nmesh[2], s[0], s[1], s[2]);
@@ -415,8 +435,9 @@ static int grad4_div_impl(const void* const* g, void* out, const fpm_mesh* m, do
     if (m->nmesh[d] <= 0 || !(m->box[d] > 0) || g[d] == out) return FPM_ERR_ARG;
   T s[3];
   for (int d = 0; d < 3; ++d) s[d] = (T)(scale * m->nmesh[d] / (12.0 * m->box[d]));
-  const int64_t ncell = (int64_t)m->nmesh[0] * m->nmesh[1] * m->nmesh[2];
-  hipLaunchKernelGGL(k_grad4_div<T>, dim3(grid_n(ncell)), dim3(256), 0, st,
+  const int64_t nrows = (int64_t)m->nmesh[0] * m->nmesh[1];
+  if (nrows >= ((int64_t)1 << 31)) return FPM_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(k_grad4_div<T>, dim3((unsigned)nrows), dim3(row_threads(m->nmesh[2])), 0, st,
                      static_cast<const T*>(g[0]), static_cast<const T*>(g[1]),
                      static_cast<const T*>(g[2]), static_cast<T*>(out), m->nmesh[0],
                      m->nmesh[1], m->nmesh[2], s[0], s[1], s[2], accumulate ? 1 : 0);

@@ -981,6 +981,109 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_grad_tiles(PartDesc<T>
   }
 }
 
+// Position gradient of the stencil force readout (the window-derivative
+// part of gravity_gradient, fastpm/operators.py:236-247, with the four-point
+// gradient as a stencil): for each particle
+//   gx[e] (+)= scale * inv_cell[e] * sum_d gf[d] d/du_e readout(D_d phi)(u)
+// with D_d including 1 / (12 h_d), from the same LDS-staged potential box as
+// k_readout_grad_tiles, so the three gradient meshes are never formed.
+template <typename T>
+struct PosGrad {
+  const T* phi;
+  const T* gf;        // gf[p * gf_stride + d]
+  T* gx;              // gx[3 p + e]
+  int64_t gf_stride;
+  T dscale[3];        // 1 / (12 h_d)
+  T oscale[3];        // scale * Nmesh_e / L_e
+  int accumulate;
+};
+
+template <typename T, int ORDER>
+__device__ __forceinline__ void window_dw(const PRec& r, int l[3], T w[3][ORDER], T dw[3][ORDER]) {
+  window_rec<T, ORDER>(r, l, w);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const T t = (T)r.t[d];
+    if constexpr (ORDER == 2) {
+      dw[d][0] = (T)-1;
+      dw[d][1] = (T)1;
+    } else {
+      dw[d][0] = -((T)0.5 - t);
+      dw[d][1] = (T)-2 * t;
+      dw[d][2] = (T)0.5 + t;
+    }
+  }
+}
+
+template <typename T, int ORDER>
+__device__ __forceinline__ void pos_grad_one(const T* pbox, const PosGrad<T>& G, int64_t p,
+                                             const int l[3], const T w[3][ORDER],
+                                             const T dw[3][ORDER]) {
+  constexpr int TP = TILE + ORDER + 3;
+  constexpr int NLN = ORDER + 4;
+  const int lv[3] = {l[0], l[1], l[2]};
+  auto at = [&](int x, int y, int z) { return pbox[(x * TP + y) * TP + z]; };
+  T gfv[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) gfv[d] = G.gf[p * G.gf_stride + d] * G.dscale[d];
+  T g[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+#pragma unroll
+    for (int b = 0; b < ORDER; ++b)
+#pragma unroll
+      for (int c = 0; c < ORDER; ++c) {
+        T line[NLN];
+#pragma unroll
+        for (int u = 0; u < NLN; ++u) {
+          int ix[3];
+          ix[d] = lv[d] + u;
+          ix[e1] = lv[e1] + 2 + b;
+          ix[e2] = lv[e2] + 2 + c;
+          line[u] = at(ix[0], ix[1], ix[2]);
+        }
+        T s = (T)0, sd = (T)0;
+#pragma unroll
+        for (int a = 0; a < ORDER; ++a) {
+          const T st = (T)8 * (line[a + 3] - line[a + 1]) - (line[a + 4] - line[a]);
+          s += w[d][a] * st;
+          sd += dw[d][a] * st;
+        }
+        g[d] += gfv[d] * w[e1][b] * w[e2][c] * sd;
+        g[e1] += gfv[d] * dw[e1][b] * w[e2][c] * s;
+        g[e2] += gfv[d] * w[e1][b] * dw[e2][c] * s;
+      }
+  }
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const T v = g[e] * G.oscale[e];
+    G.gx[3 * p + e] = G.accumulate ? G.gx[3 * p + e] + v : v;
+  }
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kTileThreads) void k_pos_grad_tiles(TileGeom g, Workspace w,
+                                                                 PosGrad<T> G) {
+  constexpr int TP = TILE + ORDER + 3;
+  __shared__ T pbox[TP * TP * TP];
+  const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
+  int lo[3];
+  tile_coords(t, g, lo);
+  const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
+  if (b == e) return;  // uniform across the workgroup
+  ScalarBox<T, 2, ORDER + 1> pb;
+  pb.load(g, G.phi, lo);
+  pb.store(pbox);
+  __syncthreads();
+  for (int32_t s = b + threadIdx.x; s < e; s += kTileThreads) {
+    int l[3];
+    T wt[3][ORDER], dwt[3][ORDER];
+    window_dw<T, ORDER>(w.rec[s], l, wt, dwt);
+    pos_grad_one<T, ORDER>(pbox, G, (int64_t)w.list[s], l, wt, dwt);
+  }
+}
+
 // Readout without staging: the tile's particles gather straight from the
 // meshes (through L1/L2), in tile order so that a wave's gathers share lines.
 template <typename T, int ORDER, int NC>
@@ -1265,6 +1368,38 @@ static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int o
   return FPM_OK;
 }
 
+
+template <typename T>
+static int pos_grad_impl(const fpm_particles* part, const fpm_mesh* m, int order, const void* phi,
+                         const void* gf, int64_t gf_stride, void* gx, double scale, int accumulate,
+                         void* ws, int64_t ws_bytes, hipStream_t st) {
+  PartDesc<T> P;
+  TileGeom g;
+  Workspace w;
+  int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
+  if (rc) return rc;
+  if (!phi || !gf || !gx || gf_stride < 3) return FPM_ERR_ARG;
+  for (int d = 0; d < 3; ++d)
+    if (m->nmesh[d] < TILE + order + 3) return FPM_ERR_UNSUPPORTED;
+  PosGrad<T> G{};
+  G.phi = static_cast<const T*>(phi);
+  G.gf = static_cast<const T*>(gf);
+  G.gx = static_cast<T*>(gx);
+  G.gf_stride = gf_stride;
+  for (int d = 0; d < 3; ++d) {
+    G.dscale[d] = (T)((double)m->nmesh[d] / (12.0 * m->box[d]));
+    G.oscale[d] = (T)(scale * (double)m->nmesh[d] / m->box[d]);
+  }
+  G.accumulate = accumulate ? 1 : 0;
+  if (P.n == 0) return FPM_OK;
+  const dim3 grid((unsigned)g.ntiles), blk(kTileThreads);
+  if (order == 2)
+    hipLaunchKernelGGL((k_pos_grad_tiles<T, 2>), grid, blk, 0, st, g, w, G);
+  else
+    hipLaunchKernelGGL((k_pos_grad_tiles<T, 3>), grid, blk, 0, st, g, w, G);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
 }  // namespace fpm
 
 using namespace fpm;
@@ -1316,6 +1451,14 @@ int fpm_readout_tiled_grad4_f32(const fpm_particles* part, const fpm_mesh* mesh,
   return readout_grad_impl<float>(part, mesh, order, rho, phi, rho_out, rho_stride, force_out,
                                   force_stride, rho_scale, force_scale, ws, ws_bytes,
                                   (hipStream_t)s);
+}
+
+int fpm_window_grad_tiled_grad4_f32(const fpm_particles* part, const fpm_mesh* mesh, int order,
+                                    const void* phi, const void* gf, int64_t gf_stride, void* gx,
+                                    double scale, int accumulate, void* ws, int64_t ws_bytes,
+                                    fpm_stream_t s) {
+  return pos_grad_impl<float>(part, mesh, order, phi, gf, gf_stride, gx, scale, accumulate, ws,
+                              ws_bytes, (hipStream_t)s);
 }
 
 }  // extern "C"

@@ -100,6 +100,8 @@ _PLAIN = {
                                             _vp]),
     "fpm_readout_tiled_grad4_f32": (_int, [_PP, _PM, _int, _vp, _vp, _vp, _i64, _vp, _i64, _dbl,
                                            _dbl, _vp, _i64, _vp]),
+    "fpm_window_grad_tiled_grad4_f32": (_int, [_PP, _PM, _int, _vp, _vp, _i64, _vp, _dbl, _int,
+                                               _vp, _i64, _vp]),
     "fpm_counter_add": (_int, [_vp, ctypes.c_int32, _vp]),
 }
 

@@ -141,9 +141,14 @@ def force_vjp(step, S, Q, gF, gS):
     # the spectra themselves are not needed (kout=False)
     r2c_c2r_many(rho, [common], kout=False, outs=[phi], work=work)
     m = [RealField(pm) for _ in range(3)]
-    grad4_mesh(pm, phi, m)
     # readout's position derivative
-    window_grad(pm, parts, m, [gF[:, 0], gF[:, 1], gF[:, 2]], gS, 1.0, accumulate=True)
+    if tiled and pm.grad4_supported():
+        # stencil applied to the staged potential: the D_d phi meshes are
+        # never formed
+        pm.window_grad_tiled_grad4(phi, parts, gF, gS, 1.0, accumulate=True)
+    else:
+        grad4_mesh(pm, phi, m)
+        window_grad(pm, parts, m, [gF[:, 0], gF[:, 1], gF[:, 2]], gS, 1.0, accumulate=True)
     # readout's mesh adjoint, the stencil's adjoint, then the multiplier back to rho
     if tiled:
         pm.paint_tiled_weighted(parts, [gF[:, 0], gF[:, 1], gF[:, 2]], m)
@@ -284,7 +289,13 @@ class DensityLoss:
     def density_k(self, state):
         fpm = self.fpm
         nbar = 1.0 * state.csize / float(np.prod(fpm.Nmesh))
-        rho = fpm.paint(state.particles, mass=1.0 / nbar, resampler="cic")
+        parts = state.particles
+        if fpm.tiles_supported():
+            # binned fixed-point paint: deterministic, no float atomics
+            fpm.bin_particles(parts, resampler="cic")
+            rho = fpm.paint_tiled(parts, mass=1.0 / nbar, resampler="cic")
+        else:
+            rho = fpm.paint(parts, mass=1.0 / nbar, resampler="cic")
         return rho.r2c()
 
     def __call__(self, state, grad=True):

@@ -342,6 +342,23 @@ class ParticleMesh:
                         rp, phi.value.data_ptr(), ro, rs, force.data_ptr(), int(force.stride(0)),
                         float(rho_scale), float(force_scale), ws.data_ptr(), ws.numel(), _stream())
 
+    def window_grad_tiled_grad4(self, phi, pos, gf, gx, scale=1.0, accumulate=True,
+                                resampler=None):
+        """gx (+)= scale * grad_x sum_d gf[:, d] readout(D_d phi) with D_d the
+        four-point gradient stencil (the position part of the stencil force's
+        adjoint); `pos` must have been binned with bin_particles."""
+        parts = self._particles(pos)
+        order = resampler_order(resampler or self.resampler)
+        ws = self.tile_workspace(len(parts), order)
+        if gf.dim() != 2 or gf.shape[1] != 3 or gf.stride(1) != 1:
+            raise ValueError("gf must be an [N, 3] array with unit stride along the last axis")
+        if not gx.is_contiguous() or gx.shape != (len(parts), 3):
+            raise ValueError("gx must be a contiguous [N, 3] array")
+        _lib.call_plain("fpm_window_grad_tiled_grad4_f32", parts.desc(self.dtype), self._desc,
+                        order, phi.value.data_ptr(), gf.data_ptr(), int(gf.stride(0)),
+                        gx.data_ptr(), float(scale), 1 if accumulate else 0, ws.data_ptr(),
+                        ws.numel(), _stream())
+
     def generate_whitenoise(self, seed, type="complex", unitary=False, mean=0):
         """Counter-based complex white noise (replaces pmesh generate_whitenoise,
         fastpm/core.py:114)."""

@@ -174,6 +174,19 @@ int fpm_readout_tiled_grad4_f32(const fpm_particles* part, const fpm_mesh* mesh,
                                 double rho_scale, double force_scale, void* ws, int64_t ws_bytes,
                                 fpm_stream_t stream);
 
+/* Position gradient of the stencil force readout: with D_d the four-point
+ * stencil of fpm_readout_tiled_grad4_f32 (including 1 / (12 h_d)),
+ *   gx[3 p + e] (+)= scale * d/dx_e sum_d gf[p * gf_stride + d] readout(D_d phi)(x_p)
+ * (accumulate selects += or =), i.e. fpm_window_grad of the three meshes
+ * D_d phi weighted by gf, without forming those meshes.  The window-derivative
+ * part of gravity_gradient (fastpm/operators.py:236-247) for the stencil force.
+ * Uses the binning of the last fpm_bin_particles_f32 call; same mesh
+ * requirements as fpm_readout_tiled_grad4_f32. */
+int fpm_window_grad_tiled_grad4_f32(const fpm_particles* part, const fpm_mesh* mesh, int order,
+                                    const void* phi, const void* gf, int64_t gf_stride, void* gx,
+                                    double scale, int accumulate, void* ws, int64_t ws_bytes,
+                                    fpm_stream_t stream);
+
 /* Real-to-half-complex 3D transform, normalised by 1/(n0 n1 n2).  `in` is left
  * intact.  tw0/tw1/tw2: device twiddle tables for n0, n1, n2.
  * Replaces RealField.r2c (fastpm/core.py:205, fastpm/force/lpt.py:64). */

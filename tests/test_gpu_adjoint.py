@@ -277,3 +277,48 @@ def test_grad4_stencil(gpu, dtype, nmesh):
     lhs = sum(float((outs[d].value.double() * G[d].value.double()).sum()) for d in range(3)) / 2.0
     rhs = float((F.value.double() * div.value.double()).sum())
     assert abs(lhs - rhs) < (1e-10 if dtype == "f8" else 1e-3) * abs(lhs)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("nmesh,kind", [((32, 32, 32), "lattice"), ((32, 48, 64), "random"),
+                                        ((48, 32, 32), "cluster")])
+def test_window_grad_tiled_grad4(gpu, order, nmesh, kind):
+    """fpm_window_grad_tiled_grad4_f32 (the stencil applied to the staged
+    potential) equals fpm_window_grad of the three D_d phi meshes from
+    fpm_grad4_mesh weighted by gf, accumulating into gx."""
+    from fastpm.pm import ParticleMesh, Particles
+    from fastpm.adjoint import grad4_mesh, window_grad
+
+    print("pos_grad", order, nmesh, kind, flush=True)
+    box = (60.0, 80.0, 100.0)
+    rng = np.random.default_rng(40 + order)
+    pm = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4", resampler="cic" if order == 2 else "tsc")
+    if kind == "lattice":
+        Q = ParticleMesh(Nmesh=[n // 2 for n in nmesh], BoxSize=box,
+                         dtype="f4").generate_uniform_particle_grid(shift=0.0)
+        S = torch.from_numpy(rng.normal(0, 2.0, size=(len(Q), 3)).astype("f4")).to(gpu)
+    else:
+        n = 20000
+        q = rng.uniform(0, 1, size=(n, 3)) * np.array(box)
+        if kind == "cluster":
+            q[: n // 2] = np.array(box) * 0.5 + rng.normal(0, 0.05, size=(n // 2, 3))
+        Q = torch.from_numpy(q.astype("f4")).to(gpu)
+        S = torch.from_numpy(rng.normal(0, 1.0, size=(n, 3)).astype("f4")).to(gpu)
+    parts = Particles(Q, S)
+    n = len(parts)
+    phi = pm.create("real", value=torch.from_numpy(rng.standard_normal(nmesh).astype("f4")).to(gpu))
+    gf = torch.from_numpy(rng.standard_normal((n, 3)).astype("f4")).to(gpu)
+    g0 = torch.from_numpy(rng.standard_normal((n, 3)).astype("f4")).to(gpu)
+    m = [pm.create("real") for _ in range(3)]
+    grad4_mesh(pm, phi, m)
+    ref = g0.clone()
+    window_grad(pm, parts, m, [gf[:, 0], gf[:, 1], gf[:, 2]], ref, 0.7, accumulate=True)
+    got = g0.clone()
+    pm.bin_particles(parts)
+    pm.window_grad_tiled_grad4(phi, parts, gf, got, 0.7, accumulate=True)
+    d = (got - ref).abs().max().item()
+    assert d < 2e-5 * (ref - g0).abs().max().item(), d
+    # overwrite mode
+    got2 = torch.full_like(g0, 123.0)
+    pm.window_grad_tiled_grad4(phi, parts, gf, got2, 0.7, accumulate=False)
+    assert (got2 - (ref - g0)).abs().max().item() < 2e-5 * (ref - g0).abs().max().item()

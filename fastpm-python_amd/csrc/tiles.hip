@@ -825,8 +825,11 @@ struct GradRead {
 
 // Scalar box of (TILE + L + U)^3 cells starting at lo - L, staged with the
 // aligned TILE-cell middle of every row as four-wide vectors and L + U wrapped
-// edge cells.
-template <typename T, int L, int U>
+// edge cells.  With CORE > 0 only the cells a stencil readout can reach are
+// loaded: the core [L, L + CORE)^3 extended along one axis at a time (the
+// four-point stencil runs along one axis while the window stays in the core
+// of the other two); the rest of the box is left unwritten and never read.
+template <typename T, int L, int U, int CORE = 0>
 struct ScalarBox {
   typedef T V4 __attribute__((ext_vector_type(4)));
   static constexpr int TB = TILE + L + U;
@@ -842,16 +845,26 @@ struct ScalarBox {
     return i < 0 ? i + n : (i >= n ? i - n : i);
   }
 
+  // in the core along one axis (box coordinate b)
+  __device__ __forceinline__ static bool in_core(int b) { return b >= L && b < L + CORE; }
+
   __device__ __forceinline__ void load(const TileGeom& g, const T* mesh, const int lo[3]) {
 #pragma unroll
     for (int k = 0; k < PV; ++k) {
       const int i = threadIdx.x + k * kTileThreads;
       if (i < NV) {
         const int q = i % NQ, r = i / NQ;
-        const int gy = wrap(lo[1] - L + r % TB, g.nm[1]);
-        const int gx = wrap(lo[0] - L + r / TB, g.nm[0]);
-        vr[k] = *reinterpret_cast<const V4*>(mesh + ((int64_t)gx * g.nm[1] + gy) * g.nm[2] +
-                                             lo[2] + 4 * q);
+        const int by = r % TB, bx = r / TB;
+        // the middle of a row is in the z core: the row is needed unless both
+        // x and y are outside the core
+        if (CORE == 0 || in_core(bx) || in_core(by)) {
+          const int gy = wrap(lo[1] - L + by, g.nm[1]);
+          const int gx = wrap(lo[0] - L + bx, g.nm[0]);
+          vr[k] = *reinterpret_cast<const V4*>(mesh + ((int64_t)gx * g.nm[1] + gy) * g.nm[2] +
+                                               lo[2] + 4 * q);
+        } else {
+          vr[k] = V4{(T)0, (T)0, (T)0, (T)0};
+        }
       }
     }
 #pragma unroll
@@ -859,10 +872,18 @@ struct ScalarBox {
       const int i = threadIdx.x + k * kTileThreads;
       if (i < NE) {
         const int h = i % (L + U), r = i / (L + U);
-        const int gy = wrap(lo[1] - L + r % TB, g.nm[1]);
-        const int gx = wrap(lo[0] - L + r / TB, g.nm[0]);
-        const int gz = wrap(h < L ? lo[2] - L + h : lo[2] + TILE + (h - L), g.nm[2]);
-        er[k] = mesh[((int64_t)gx * g.nm[1] + gy) * g.nm[2] + gz];
+        const int by = r % TB, bx = r / TB;
+        const int bz = h < L ? h : TILE + h;
+        // an edge cell outside the z core is needed only on a core row
+        if (CORE == 0 || (in_core(bx) && in_core(by)) ||
+            (in_core(bz) && (in_core(bx) || in_core(by)))) {
+          const int gy = wrap(lo[1] - L + by, g.nm[1]);
+          const int gx = wrap(lo[0] - L + bx, g.nm[0]);
+          const int gz = wrap(h < L ? lo[2] - L + h : lo[2] + TILE + (h - L), g.nm[2]);
+          er[k] = mesh[((int64_t)gx * g.nm[1] + gy) * g.nm[2] + gz];
+        } else {
+          er[k] = (T)0;
+        }
       }
     }
   }
@@ -958,7 +979,7 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_grad_tiles(PartDesc<T>
   const int64_t p1 = s1 < e ? w.list[s1] : 0;
   const PRec r0 = s0 < e ? w.rec[s0] : PRec{};
   const PRec r1 = s1 < e ? w.rec[s1] : PRec{};
-  ScalarBox<T, 2, ORDER + 1> pb;
+  ScalarBox<T, 2, ORDER + 1, TILE + ORDER - 1> pb;
   pb.load(g, G.phi, lo);
   ScalarBox<T, 0, ORDER - 1> rb;
   if (G.rho_out) rb.load(g, G.rho, lo);
@@ -1072,7 +1093,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pos_grad_tiles(TileGeom g, Wor
   tile_coords(t, g, lo);
   const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
   if (b == e) return;  // uniform across the workgroup
-  ScalarBox<T, 2, ORDER + 1> pb;
+  ScalarBox<T, 2, ORDER + 1, TILE + ORDER - 1> pb;
   pb.load(g, G.phi, lo);
   pb.store(pbox);
   __syncthreads();

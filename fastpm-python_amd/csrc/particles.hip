@@ -52,6 +52,31 @@ __global__ void k_axpy_table(T* y, const T* x, int64_t n, const double* table,
     y[i] = madd(a, x[i], y[i]);
 }
 
+// Table-driven step boundary for a captured graph: with row = counter[0],
+// P += table[row-1][close] F (the previous step's closing kick, row > 0 only),
+// P += table[row][open] F, S += table[row][drift] P, each rounded as k_axpy.
+// A counter outside [0, rows) leaves S and P unchanged.
+template <typename T>
+__global__ void k_kick_drift_table(T* S, T* P, const T* F, int64_t n, const double* table,
+                                   const int32_t* counter, int rows, int stride, int close,
+                                   int open, int drift) {
+  const int32_t row = counter[0];
+  if (row < 0 || row >= rows) return;
+  const bool prev = row > 0;
+  const T k0 = prev ? (T)table[(int64_t)(row - 1) * stride + close] : (T)0;
+  const T k1 = (T)table[(int64_t)row * stride + open];
+  const T d = (T)table[(int64_t)row * stride + drift];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T f = F[i];
+    T p = P[i];
+    if (prev) p = madd(k0, f, p);
+    p = madd(k1, f, p);
+    P[i] = p;
+    S[i] = madd(d, p, S[i]);
+  }
+}
+
 // y = a y + b x (the damped kick of the glass generator).
 template <typename T>
 __global__ void k_axpby(T* y, const T* x, int64_t n, T a, T b) {
@@ -152,6 +177,22 @@ static int axpy_table_impl(void* y, const void* x, int64_t n, const double* tabl
 }
 
 template <typename T>
+static int kick_drift_table_impl(void* S, void* P, const void* F, int64_t n, const double* table,
+                                 const int32_t* counter, int rows, int stride, int close,
+                                 int open, int drift, hipStream_t st) {
+  auto bad = [stride](int s) { return s < 0 || s >= stride; };
+  if (n < 0 || !table || !counter || rows < 1 || stride < 1 || bad(close) || bad(open) ||
+      bad(drift) || (n > 0 && (!S || !P || !F)))
+    return FPM_ERR_ARG;
+  if (n == 0) return FPM_OK;
+  hipLaunchKernelGGL(k_kick_drift_table<T>, dim3(grid_elems(n)), dim3(256), 0, st,
+                     static_cast<T*>(S), static_cast<T*>(P), static_cast<const T*>(F), n, table,
+                     counter, rows, stride, close, open, drift);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
 static int axpby_impl(void* y, const void* x, int64_t n, double a, double b, hipStream_t st) {
   if (n < 0 || (n > 0 && (!y || !x))) return FPM_ERR_ARG;
   if (n == 0) return FPM_OK;
@@ -232,6 +273,18 @@ int fpm_axpy_table_f32(void* y, const void* x, int64_t n, const double* table,
 int fpm_axpy_table_f64(void* y, const void* x, int64_t n, const double* table,
                        const int32_t* counter, int rows, int stride, int slot, fpm_stream_t s) {
   return axpy_table_impl<double>(y, x, n, table, counter, rows, stride, slot, (hipStream_t)s);
+}
+int fpm_kick_drift_table_f32(void* S, void* P, const void* F, int64_t n, const double* table,
+                             const int32_t* counter, int rows, int stride, int close, int open,
+                             int drift, fpm_stream_t s) {
+  return kick_drift_table_impl<float>(S, P, F, n, table, counter, rows, stride, close, open, drift,
+                                      (hipStream_t)s);
+}
+int fpm_kick_drift_table_f64(void* S, void* P, const void* F, int64_t n, const double* table,
+                             const int32_t* counter, int rows, int stride, int close, int open,
+                             int drift, fpm_stream_t s) {
+  return kick_drift_table_impl<double>(S, P, F, n, table, counter, rows, stride, close, open,
+                                       drift, (hipStream_t)s);
 }
 int fpm_axpby_f32(void* y, const void* x, int64_t n, double a, double b, fpm_stream_t s) {
   return axpby_impl<float>(y, x, n, a, b, (hipStream_t)s);

@@ -81,6 +81,8 @@ _TYPED = {
     "fpm_hermitian_scale": (_int, [_vp, _dbl, _dbl, _PM, _i64, _vp]),
     "fpm_lpt2_source_vjp": (_int, [_vp, _vp, _vp, _i64, _vp]),
     "fpm_axpy_table": (_int, [_vp, _vp, _i64, _vp, _vp, _int, _int, _int, _vp]),
+    "fpm_kick_drift_table": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _int, _int, _int, _int, _int,
+                                    _vp]),
 }
 _PLAIN = {
     "fpm_version": (ctypes.c_char_p, []),

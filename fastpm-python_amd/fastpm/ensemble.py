@@ -3,9 +3,12 @@
 Each box has its own particle mesh, force mesh (Nmesh * B), cosmology
 (sigma8 may differ) and white-noise seed; all share the stage list.  The
 kick/drift coefficients of every step are precomputed in FP64 into a device
-table per box and read through a device step counter (`fpm_axpy_table`), so
-the whole K, D, F, K sequence of all boxes is captured once as a graph and
-replayed for every step.  Paint, transforms and readout are the same kernels
+table per box and read through a device step counter, so one step of all
+boxes is captured once as a graph and replayed for every step.  The replayed
+unit is a step boundary, as in Solver.nbody: the previous step's closing
+kick, this step's opening kick and drift in one pass
+(`fpm_kick_drift_table`), then the force; the last closing kick follows the
+final replay.  Paint, transforms and readout are the same kernels
 the single-box Solver runs (the tiled, deterministic paint on meshes that are
 multiples of 16), so a replayed ensemble reproduces the per-box eager runs
 bit for bit.
@@ -32,6 +35,7 @@ class Box:
         self.step = self.solver.nbodystep
         self.state = None
         self.table = None
+        self.rows = None
 
 
 class Ensemble:
@@ -92,6 +96,7 @@ class Ensemble:
                 ah = (s0 * s1) ** 0.5
                 rows.append([pt.kick_factor(s0, s0, ah), pt.drift_factor(s0, ah, s1),
                              pt.kick_factor(ah, s1, s1)])
+            b.rows = rows
             b.table = torch.tensor(np.array(rows, dtype="f8"), device=self.counter.device)
         self.counter.zero_()
 
@@ -99,18 +104,15 @@ class Ensemble:
         st = _stream()
         cnt = self.counter.data_ptr()
         s = b.state
-        dt = b.pm.dtype
         n = s.P.numel()
-        _lib.call("fpm_axpy_table", dt, s.P.data_ptr(), s.F.data_ptr(), n,
-                  b.table.data_ptr(), cnt, self.nsteps, 3, 0, st)
-        _lib.call("fpm_axpy_table", dt, s.S.data_ptr(), s.P.data_ptr(), n,
-                  b.table.data_ptr(), cnt, self.nsteps, 3, 1, st)
+        # previous closing kick (slot 2 of the previous row), opening kick
+        # (slot 0) and drift (slot 1) of this row
+        _lib.call("fpm_kick_drift_table", b.pm.dtype, s.S.data_ptr(), s.P.data_ptr(),
+                  s.F.data_ptr(), n, b.table.data_ptr(), cnt, self.nsteps, 3, 2, 0, 1, st)
         b.step.force_kernels(s, keep_delta_k=False)
-        _lib.call("fpm_axpy_table", dt, s.P.data_ptr(), s.F.data_ptr(), n,
-                  b.table.data_ptr(), cnt, self.nsteps, 3, 2, st)
 
     def _enqueue_step(self):
-        """K, D, F, K of every box for the step held in the device counter:
+        """[K,] K, D, F of every box for the step held in the device counter:
         fork to the box streams, join, then advance the counter."""
         main = torch.cuda.current_stream()
         for lane, stream in zip(self.lanes, self.streams):
@@ -134,6 +136,16 @@ class Ensemble:
         self.graph = g
         return g
 
+    def finish(self):
+        """The closing kick of the last step of every box (after the last
+        step or replay), and the final time tags."""
+        if self.nsteps > 0:
+            for b in self.boxes:
+                s = b.state
+                _lib.call("fpm_axpy", b.pm.dtype, s.P.data_ptr(), s.F.data_ptr(), s.P.numel(),
+                          float(b.rows[-1][2]), _stream())
+        self._finish_tags()
+
     def run(self, use_graph=True):
         """Runs all steps: the first eagerly, the rest by graph replay."""
         done = 0
@@ -149,7 +161,7 @@ class Ensemble:
         else:
             for _ in range(self.nsteps - done):
                 self.step_eager()
-        self._finish_tags()
+        self.finish()
 
     def _finish_tags(self):
         a = self.stages[-1]

@@ -315,6 +315,20 @@ int fpm_axpy_table_f64(void* y, const void* x, int64_t n, const double* table,
                        fpm_stream_t stream);
 int fpm_counter_add(int32_t* counter, int32_t v, fpm_stream_t stream);
 
+/* The table-driven form of fpm_kick_drift for a captured step graph: with
+ * row = counter[0], P += table[row-1][close] F (the previous step's closing
+ * kick; skipped for row 0), P += table[row][open] F, S += table[row][drift] P,
+ * each rounded as fpm_axpy, so a replay equals the separate updates bit for
+ * bit.  A counter outside [0, rows) leaves S and P unchanged.  The leapfrog
+ * step boundary of fastpm/core.py:168-180,261-288 with the coefficients
+ * precomputed per step. */
+int fpm_kick_drift_table_f32(void* S, void* P, const void* F, int64_t n, const double* table,
+                             const int32_t* counter, int rows, int stride, int close, int open,
+                             int drift, fpm_stream_t stream);
+int fpm_kick_drift_table_f64(void* S, void* P, const void* F, int64_t n, const double* table,
+                             const int32_t* counter, int rows, int stride, int close, int open,
+                             int drift, fpm_stream_t stream);
+
 /* S = c0 dx1 + c1 dx2; P = c2 dx1 + c3 dx2; F = c4 dx1 + c5 dx2 over n
  * elements (dx2 may be NULL for first order).  fastpm/core.py:128-141. */
 int fpm_lpt_combine_f32(const void* dx1, const void* dx2, void* S, void* P, void* F, int64_t n,

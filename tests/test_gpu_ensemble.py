@@ -60,6 +60,7 @@ def test_ensemble_replays_with_eager_work_between(gpu):
         a.graph.replay()
         junk.mul_(1.0001).add_(1.0)
         torch.cuda.synchronize()
+    a.finish()  # the last closing kick
     b = Ensemble(_specs(), stages)
     b.initialize()
     b.run(use_graph=False)

@@ -326,15 +326,26 @@ class FastPMStep:
         state.a["S"] = af
 
     def workspace(self):
+        """Force-mesh buffers, allocated once: the stencil path needs the
+        density, one real and one complex mesh (the spectrum buffer only when
+        a monitor receives delta_k, see delta_k_buffer); the k-space path
+        three of each and the spectrum."""
         if self._ws is None:
             pm = self.pm
+            n = 1 if self.stencil else 3
             self._ws = dict(
                 rho=RealField(pm),
-                delta_k=ComplexField(pm),
-                force=[RealField(pm) for _ in range(3)],
-                work=[pm._complex_tensor() for _ in range(3)],
+                delta_k=None if self.stencil else ComplexField(pm),
+                force=[RealField(pm) for _ in range(n)],
+                work=[pm._complex_tensor() for _ in range(n)],
             )
         return self._ws
+
+    def delta_k_buffer(self):
+        ws = self.workspace()
+        if ws["delta_k"] is None:
+            ws["delta_k"] = ComplexField(self.pm)
+        return ws["delta_k"]
 
     def prepare_force(self, state, smoothing):
         """Paint 1 + delta of the particles on the force mesh (core.py:182-193)."""
@@ -393,14 +404,14 @@ class FastPMStep:
             # delta_k = r2c(rho) and phi = c2r(common * delta_k), sharing the x pass
             phi = ws["force"][0]
             with self._mark("fft"):
-                kout = ws["delta_k"] if keep_delta_k else False
+                kout = self.delta_k_buffer() if keep_delta_k else False
                 delta_k, _ = r2c_c2r_many(rho, [self.force_common()], kout=kout,
                                           outs=[phi], work=ws["work"][:1])
             with self._mark("readout"):
                 self.pm.readout_tiled_grad4(phi, X1, F, rho=rho, rho_out=state.RHO)
             return delta_k
         with self._mark("r2c"):
-            delta_k = rho.r2c(out=ws["delta_k"])
+            delta_k = rho.r2c(out=self.delta_k_buffer())
         ops = self.force_ops()
         if self.solver.fused:
             with self._mark("c2r"):

@@ -149,14 +149,6 @@ __device__ __forceinline__ void weights_t(const T t[3], T w[3][ORDER]) {
   }
 }
 
-template <typename T, int ORDER>
-__device__ __forceinline__ void window_of(const PartDesc<T>& P, int64_t p, const int nm[3],
-                                          int i0[3], T w[3][ORDER]) {
-  T t[3];
-  window_t<T, ORDER>(P, p, nm, i0, t);
-  weights_t<T, ORDER>(t, w);
-}
-
 // A record's window: origin local to its tile and weights.
 template <typename T, int ORDER>
 __device__ __forceinline__ void window_rec(const PRec& r, int l[3], T w[3][ORDER]) {
@@ -1105,48 +1097,6 @@ __global__ __launch_bounds__(kTileThreads) void k_pos_grad_tiles(TileGeom g, Wor
   }
 }
 
-// Readout without staging: the tile's particles gather straight from the
-// meshes (through L1/L2), in tile order so that a wave's gathers share lines.
-template <typename T, int ORDER, int NC>
-__global__ __launch_bounds__(256) void k_readout_list(PartDesc<T> P, TileGeom g, Workspace w,
-                                                      TileRead<T> R) {
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < P.n;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = w.list[s];
-    int i0[3];
-    T wt[3][ORDER];
-    window_of<T, ORDER>(P, p, g.nm, i0, wt);
-    T acc[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] = (T)0;
-#pragma unroll
-    for (int a = 0; a < ORDER; ++a) {
-      int ix = i0[0] + a;
-      if (ix >= g.nm[0]) ix -= g.nm[0];
-#pragma unroll
-      for (int c2 = 0; c2 < ORDER; ++c2) {
-        int iy = i0[1] + c2;
-        if (iy >= g.nm[1]) iy -= g.nm[1];
-        const T wac = wt[0][a] * wt[1][c2];
-        const int64_t row = ((int64_t)ix * g.nm[1] + iy) * g.nm[2];
-#pragma unroll
-        for (int d = 0; d < ORDER; ++d) {
-          int iz = i0[2] + d;
-          if (iz >= g.nm[2]) iz -= g.nm[2];
-          const T wv = wac * wt[2][d];
-#pragma unroll
-          for (int c = 0; c < NC; ++c) acc[c] += wv * R.mesh[c][row + iz];
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) R.out[c][p * R.stride[c]] = acc[c] * R.scale[c];
-  }
-}
-
-#ifndef FPM_READOUT_LDS
-#define FPM_READOUT_LDS 1
-#endif
 
 // ---------------------------------------------------------------- host side
 static bool tiles_ok(const fpm_mesh* m) {
@@ -1277,16 +1227,6 @@ static void launch_rt_nc(const PartDesc<T>& P, const TileGeom& g, const Workspac
 template <typename T, int ORDER>
 static void launch_rt(const PartDesc<T>& P, const TileGeom& g, const Workspace& w,
                       const TileRead<T>& R, int nc, hipStream_t st) {
-  if (!FPM_READOUT_LDS) {
-    const dim3 gl(grid_parts(P.n)), bl(256);
-    switch (nc) {
-      case 1: hipLaunchKernelGGL((k_readout_list<T, ORDER, 1>), gl, bl, 0, st, P, g, w, R); break;
-      case 2: hipLaunchKernelGGL((k_readout_list<T, ORDER, 2>), gl, bl, 0, st, P, g, w, R); break;
-      case 3: hipLaunchKernelGGL((k_readout_list<T, ORDER, 3>), gl, bl, 0, st, P, g, w, R); break;
-      default: hipLaunchKernelGGL((k_readout_list<T, ORDER, 4>), gl, bl, 0, st, P, g, w, R); break;
-    }
-    return;
-  }
   switch (nc) {
     case 1: launch_rt_nc<T, ORDER, 1>(P, g, w, R, st); break;
     case 2: launch_rt_nc<T, ORDER, 2>(P, g, w, R, st); break;

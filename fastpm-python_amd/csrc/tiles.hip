@@ -31,6 +31,11 @@ int make_desc(const fpm_particles* part, const fpm_mesh* m, PartDesc<T>& P);
 
 constexpr int TILE = 16;
 constexpr int kTileThreads = 512;
+// The CIC stencil readout runs 1024 threads per tile: it fits in 63 VGPRs,
+// so two 1024-thread workgroups share a CU (twice the waves in flight for
+// the latency-bound staging; 0.634 -> 0.615 ms at 512^3).  The TSC readout
+// needs 124 VGPRs and keeps 512.
+template <int ORDER> constexpr int grad_threads() { return ORDER == 2 ? 1024 : kTileThreads; }
 constexpr int kSub = 8;  // sub-bins per tile
 
 struct TileGeom {
@@ -821,15 +826,15 @@ struct GradRead {
 // loaded: the core [L, L + CORE)^3 extended along one axis at a time (the
 // four-point stencil runs along one axis while the window stays in the core
 // of the other two); the rest of the box is left unwritten and never read.
-template <typename T, int L, int U, int CORE = 0>
+template <typename T, int L, int U, int CORE = 0, int NT = kTileThreads>
 struct ScalarBox {
   typedef T V4 __attribute__((ext_vector_type(4)));
   static constexpr int TB = TILE + L + U;
   static constexpr int NQ = TILE / 4;
   static constexpr int NV = TB * TB * NQ;
-  static constexpr int PV = (NV + kTileThreads - 1) / kTileThreads;
+  static constexpr int PV = (NV + NT - 1) / NT;
   static constexpr int NE = TB * TB * (L + U);
-  static constexpr int PE = (NE + kTileThreads - 1) / kTileThreads;
+  static constexpr int PE = (NE + NT - 1) / NT;
   V4 vr[PV];
   T er[PE];
 
@@ -843,7 +848,7 @@ struct ScalarBox {
   __device__ __forceinline__ void load(const TileGeom& g, const T* mesh, const int lo[3]) {
 #pragma unroll
     for (int k = 0; k < PV; ++k) {
-      const int i = threadIdx.x + k * kTileThreads;
+      const int i = threadIdx.x + k * NT;
       if (i < NV) {
         const int q = i % NQ, r = i / NQ;
         const int by = r % TB, bx = r / TB;
@@ -861,7 +866,7 @@ struct ScalarBox {
     }
 #pragma unroll
     for (int k = 0; k < PE; ++k) {
-      const int i = threadIdx.x + k * kTileThreads;
+      const int i = threadIdx.x + k * NT;
       if (i < NE) {
         const int h = i % (L + U), r = i / (L + U);
         const int by = r % TB, bx = r / TB;
@@ -883,7 +888,7 @@ struct ScalarBox {
   __device__ __forceinline__ void store(T* box) const {
 #pragma unroll
     for (int k = 0; k < PV; ++k) {
-      const int i = threadIdx.x + k * kTileThreads;
+      const int i = threadIdx.x + k * NT;
       if (i < NV) {
         const int q = i % NQ, r = i / NQ;
         T* dst = &box[r * TB + L + 4 * q];
@@ -895,7 +900,7 @@ struct ScalarBox {
     }
 #pragma unroll
     for (int k = 0; k < PE; ++k) {
-      const int i = threadIdx.x + k * kTileThreads;
+      const int i = threadIdx.x + k * NT;
       if (i < NE) {
         const int h = i % (L + U), r = i / (L + U);
         box[r * TB + (h < L ? h : TILE + h)] = er[k];
@@ -954,8 +959,8 @@ __device__ __forceinline__ void grad_one(const T* rbox, const T* pbox, const Gra
   for (int d = 0; d < 3; ++d) G.f_out[p * G.f_stride + d] = f[d] * G.f_scale[d];
 }
 
-template <typename T, int ORDER>
-__global__ __launch_bounds__(kTileThreads) void k_readout_grad_tiles(PartDesc<T> P, TileGeom g,
+template <typename T, int ORDER, int NT = grad_threads<ORDER>()>
+__global__ __launch_bounds__(NT) void k_readout_grad_tiles(PartDesc<T> P, TileGeom g,
                                                                       Workspace w, GradRead<T> G) {
   constexpr int TR = TILE + ORDER - 1;
   constexpr int TP = TILE + ORDER + 3;
@@ -966,14 +971,14 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_grad_tiles(PartDesc<T>
   tile_coords(t, g, lo);
   const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
   if (b == e) return;  // uniform across the workgroup
-  const int32_t s0 = b + threadIdx.x, s1 = s0 + kTileThreads;
+  const int32_t s0 = b + threadIdx.x, s1 = s0 + NT;
   const int64_t p0 = s0 < e ? w.list[s0] : 0;
   const int64_t p1 = s1 < e ? w.list[s1] : 0;
   const PRec r0 = s0 < e ? w.rec[s0] : PRec{};
   const PRec r1 = s1 < e ? w.rec[s1] : PRec{};
-  ScalarBox<T, 2, ORDER + 1, TILE + ORDER - 1> pb;
+  ScalarBox<T, 2, ORDER + 1, TILE + ORDER - 1, NT> pb;
   pb.load(g, G.phi, lo);
-  ScalarBox<T, 0, ORDER - 1> rb;
+  ScalarBox<T, 0, ORDER - 1, 0, NT> rb;
   if (G.rho_out) rb.load(g, G.rho, lo);
   pb.store(pbox);
   if (G.rho_out) rb.store(rbox);
@@ -988,7 +993,7 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_grad_tiles(PartDesc<T>
     window_rec<T, ORDER>(r1, l, wt);
     grad_one<T, ORDER>(rbox, pbox, G, p1, l, wt);
   }
-  for (int32_t s = s1 + kTileThreads; s < e; s += kTileThreads) {
+  for (int32_t s = s1 + NT; s < e; s += NT) {
     window_rec<T, ORDER>(w.rec[s], l, wt);
     grad_one<T, ORDER>(rbox, pbox, G, (int64_t)w.list[s], l, wt);
   }
@@ -1320,11 +1325,13 @@ static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int o
   for (int d = 0; d < 3; ++d)
     G.f_scale[d] = (T)(f_scale * (double)m->nmesh[d] / (12.0 * m->box[d]));
   if (P.n == 0) return FPM_OK;
-  const dim3 grid((unsigned)g.ntiles), blk(kTileThreads);
+  const dim3 grid((unsigned)g.ntiles);
   if (order == 2)
-    hipLaunchKernelGGL((k_readout_grad_tiles<T, 2>), grid, blk, 0, st, P, g, w, G);
+    hipLaunchKernelGGL((k_readout_grad_tiles<T, 2>), grid, dim3(grad_threads<2>()), 0, st, P, g,
+                       w, G);
   else
-    hipLaunchKernelGGL((k_readout_grad_tiles<T, 3>), grid, blk, 0, st, P, g, w, G);
+    hipLaunchKernelGGL((k_readout_grad_tiles<T, 3>), grid, dim3(grad_threads<3>()), 0, st, P, g,
+                       w, G);
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }

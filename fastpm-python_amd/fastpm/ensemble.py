@@ -40,11 +40,12 @@ class Box:
 
 class Ensemble:
     def __init__(self, specs, stages, dtype="f4", B=2, resampler="cic", compensate=True,
-                 nstreams=4, autotune=True):
+                 nstreams=16, autotune=True):
         """specs: iterable of dict(nc=, box=, cosmo=, seed=).  Boxes are spread
         over `nstreams` streams (largest first, each to the least loaded
         stream) so small boxes run concurrently; inside a captured graph the
-        streams become parallel branches.  autotune selects the transform
+        streams become parallel branches (64 boxes, 20 steps: 8.49 / 8.21 /
+        8.12 / 8.02 ms per step with 2 / 4 / 8 / 16 streams).  autotune selects the transform
         tiling per force-mesh shape before the first step (fastpm.tune)."""
         self.stages = np.asarray(stages, dtype="f8")
         self.autotune = bool(autotune)

@@ -256,7 +256,7 @@ static int paint_w_impl(const fpm_particles* part, const fpm_mesh* m, int order,
   PartDesc<T> P;
   int rc = make_desc<T>(part, m, P);
   if (rc) return rc;
-  if (!out || !wts || wstride < 1 || (order != 2 && order != 3)) return FPM_ERR_ARG;
+  if (!out || (P.n > 0 && !wts) || wstride < 1 || (order != 2 && order != 3)) return FPM_ERR_ARG;
   T* mesh = static_cast<T*>(out);
   const int n0 = m->nmesh[0], n1 = m->nmesh[1], n2 = m->nmesh[2];
   if (!accumulate && (rc = fill_impl<T>(mesh, (int64_t)n0 * n1 * n2, (T)0, st))) return rc;
@@ -280,7 +280,8 @@ static int window_grad_impl(const fpm_particles* part, const fpm_mesh* m, int or
   PartDesc<T> P;
   int rc = make_desc<T>(part, m, P);
   if (rc) return rc;
-  if (!gx || !meshes || ncomp < 1 || ncomp > 4 || (order != 2 && order != 3)) return FPM_ERR_ARG;
+  if ((P.n > 0 && !gx) || !meshes || ncomp < 1 || ncomp > 4 || (order != 2 && order != 3))
+    return FPM_ERR_ARG;
   GradArgs<T> G{};
   G.ncomp = ncomp;
   G.scale = (T)scale;

@@ -130,6 +130,21 @@ __global__ __launch_bounds__(256) void k_readout(PartDesc<T> P, ReadArgs<T> R, i
 template <typename T>
 int make_desc(const fpm_particles* part, const fpm_mesh* m, PartDesc<T>& P) {
   if (!part || !m || part->n < 0) return FPM_ERR_ARG;
+  P.lat_shift = 0.0;
+  if (part->n == 0) {
+    // no particles: no arrays needed (every entry point returns after its
+    // mesh outputs are written)
+    P.q = nullptr;
+    P.s = nullptr;
+    P.n = 0;
+    for (int d = 0; d < 3; ++d) {
+      if (m->nmesh[d] <= 0 || !(m->box[d] > 0)) return FPM_ERR_ARG;
+      P.lat[d] = 0;
+      P.ratio[d] = 1.0;
+      P.inv_cell[d] = (T)((double)m->nmesh[d] / m->box[d]);
+    }
+    return FPM_OK;
+  }
   P.q = static_cast<const T*>(part->q);
   P.s = static_cast<const T*>(part->s);
   P.n = part->n;
@@ -216,7 +231,7 @@ static int readout_impl(const fpm_particles* part, const fpm_mesh* m, int order,
   ReadArgs<T> R{};
   R.ncomp = ncomp;
   for (int c = 0; c < ncomp; ++c) {
-    if (!meshes[c] || !outs[c] || strides[c] < 1) return FPM_ERR_ARG;
+    if (!meshes[c] || (P.n > 0 && !outs[c]) || strides[c] < 1) return FPM_ERR_ARG;
     R.mesh[c] = static_cast<const T*>(meshes[c]);
     R.out[c] = static_cast<T*>(outs[c]);
     R.stride[c] = strides[c];

@@ -235,8 +235,8 @@ static int positions_impl(const fpm_particles* part, const fpm_mesh* m, void* ou
   PartDesc<T> P;
   int rc = make_desc<T>(part, m, P);
   if (rc) return rc;
-  if (!out) return FPM_ERR_ARG;
   if (P.n == 0) return FPM_OK;
+  if (!out) return FPM_ERR_ARG;
   double sp[3] = {1, 1, 1};
   if (!part->q)
     for (int d = 0; d < 3; ++d) sp[d] = m->box[d] / (double)part->lattice[d];

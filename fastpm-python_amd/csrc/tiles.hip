@@ -1252,7 +1252,7 @@ static int readout_tiles_impl(const fpm_particles* part, const fpm_mesh* m, int 
   if (ncomp < 1 || ncomp > 4 || !meshes || !outs || !strides) return FPM_ERR_ARG;
   TileRead<T> R{};
   for (int c = 0; c < ncomp; ++c) {
-    if (!meshes[c] || !outs[c] || strides[c] < 1) return FPM_ERR_ARG;
+    if (!meshes[c] || (P.n > 0 && !outs[c]) || strides[c] < 1) return FPM_ERR_ARG;
     R.mesh[c] = static_cast<const T*>(meshes[c]);
     R.out[c] = static_cast<T*>(outs[c]);
     R.stride[c] = strides[c];
@@ -1277,7 +1277,7 @@ static int paint_tiles_w_impl(const fpm_particles* part, const fpm_mesh* m, int 
   if (ncomp < 1 || ncomp > 3 || !weights || !wstrides || !outs) return FPM_ERR_ARG;
   TilePaintW<T> A{};
   for (int c = 0; c < ncomp; ++c) {
-    if (!weights[c] || !outs[c] || wstrides[c] < 1) return FPM_ERR_ARG;
+    if ((P.n > 0 && !weights[c]) || !outs[c] || wstrides[c] < 1) return FPM_ERR_ARG;
     A.w[c] = static_cast<const T*>(weights[c]);
     A.wstride[c] = wstrides[c];
     A.out[c] = static_cast<T*>(outs[c]);
@@ -1310,7 +1310,8 @@ static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int o
   Workspace w;
   int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
   if (rc) return rc;
-  if (!phi || !f_out || f_stride < 3 || (rho_out && (!rho || rho_stride < 1))) return FPM_ERR_ARG;
+  if (!phi || (P.n > 0 && !f_out) || f_stride < 3 || (rho_out && (!rho || rho_stride < 1)))
+    return FPM_ERR_ARG;
   // the potential box reaches two cells past the density box on each side
   for (int d = 0; d < 3; ++d)
     if (m->nmesh[d] < TILE + order + 3) return FPM_ERR_UNSUPPORTED;
@@ -1346,7 +1347,7 @@ static int pos_grad_impl(const fpm_particles* part, const fpm_mesh* m, int order
   Workspace w;
   int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
   if (rc) return rc;
-  if (!phi || !gf || !gx || gf_stride < 3) return FPM_ERR_ARG;
+  if (!phi || (P.n > 0 && (!gf || !gx)) || gf_stride < 3) return FPM_ERR_ARG;
   for (int d = 0; d < 3; ++d)
     if (m->nmesh[d] < TILE + order + 3) return FPM_ERR_UNSUPPORTED;
   PosGrad<T> G{};

@@ -46,7 +46,8 @@ typedef struct fpm_particles {
   const void* s;          /* displacement [n][3], or NULL for zero; positions are
                              wrapped periodically and must lie within 2^31 cells
                              of the box */
-  int64_t n;              /* particle count */
+  int64_t n;              /* particle count (0 is accepted everywhere: paints
+                             write a zero mesh, particle outputs may be NULL) */
   int32_t lattice[3];     /* lattice particles per side (used when q == NULL) */
   int32_t pad_;
   double lattice_shift;   /* lattice offset in lattice spacings */

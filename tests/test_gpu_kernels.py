@@ -599,3 +599,37 @@ def test_nbody_fused_steps_bitwise(gpu):
                 assert torch.equal(x, y), (dtype, key)
             else:
                 assert ((x - y).abs().max() / y.abs().max()).item() < 1e-11, (dtype, key)
+
+
+@pytest.mark.parametrize("dtype", ["f4", "f8"])
+def test_empty_particle_set(gpu, dtype):
+    """No particles (a rank that owns none, in the reference's decomposed
+    runs): paints give a zero mesh, readouts and gradients are no-ops, and
+    no entry point rejects the empty (null) arrays."""
+    from fastpm.pm import ParticleMesh, Particles
+    from fastpm.adjoint import window_grad, paint_weighted
+
+    pm = ParticleMesh(Nmesh=(32, 32, 32), BoxSize=50.0, dtype=dtype)
+    td = torch.float32 if dtype == "f4" else torch.float64
+    Q = torch.empty((0, 3), dtype=td, device=gpu)
+    parts = Particles(Q, torch.empty((0, 3), dtype=td, device=gpu))
+    field = pm.create("real", value=torch.ones(32, 32, 32, dtype=td, device=gpu))
+    rho = pm.paint(parts, mass=1.0)
+    assert float(rho.value.abs().max()) == 0.0
+    out = torch.empty(0, dtype=td, device=gpu)
+    pm.readout_many([field], parts, [out])
+    g = torch.empty((0, 3), dtype=td, device=gpu)
+    window_grad(pm, parts, [field], None, g, 1.0)
+    assert float(paint_weighted(pm, parts, out).value.abs().max()) == 0.0
+    assert pm.positions(parts).shape == (0, 3)
+    if pm.tiles_supported():
+        pm.bin_particles(parts)
+        assert float(pm.paint_tiled(parts, mass=1.0).value.abs().max()) == 0.0
+        pm.readout_tiled([field], parts, [out])
+        F = torch.empty((0, 3), dtype=td, device=gpu)
+        pm.readout_tiled_grad4(field, parts, F, rho=field, rho_out=out)
+        outs = [pm.create("real") for _ in range(3)]
+        pm.paint_tiled_weighted(parts, [F[:, 0], F[:, 1], F[:, 2]], outs)
+        assert all(float(o.value.abs().max()) == 0.0 for o in outs)
+        pm.window_grad_tiled_grad4(field, parts, F, g)
+    torch.cuda.synchronize()

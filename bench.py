@@ -7,9 +7,12 @@ the no-wiggle Eisenstein-Hu linear power, white-noise seed 7, 2LPT at a = 0.1
 and 40 kick-drift-kick steps linear in a to a = 1, CIC paint/readout with
 window compensation.  Inputs are synthetic (generated on the device).
 
-A step is one K, D, F, K cycle.  The float32 run (integer lattice index plus
-float32 displacement) is timed over --steps steps after --warmup untimed
-steps, then run on to a = 1; the float64 plain run (one transform and one
+A step is one K, D, F, K cycle; the timed steps run as Solver.nbody runs
+them without a monitor (a step boundary's closing kick, the next opening kick
+and the drift in one pass, bit for bit the separate updates; the Force's
+delta_k not stored since nobody receives it).  The float32 run (integer
+lattice index plus float32 displacement) is timed over --steps steps after
+--warmup untimed steps, then run on to a = 1; the float64 plain run (one transform and one
 readout per force component) is timed the same way.  The final states are
 compared: P(k) relative error for k < k_Nyq/2, r(k), and the largest periodic
 displacement difference.  Further legs (each can be skipped): the float64

@@ -7,10 +7,14 @@
 // from the matter-dominated start (a0, a0, -3/7 a0^2, -6/7 a0^2) at
 // a0 = min(1e-7, a) (fastpm/background.py:243-247), then scales D1 and D2
 // (with their derivatives) to 1 at a_normalize (fastpm/background.py:201-204).
-// Integration: adaptive Dormand-Prince 5(4), landing exactly on every
-// requested a.
+// Integration: Dormand-Prince 5 steps of fixed size 1/256 in ln a from
+// ln 1e-7, kept per cosmology, and one partial step from the grid point
+// below each requested a, so the value at a depends on a alone (not on which
+// other scale factors are requested with it) and a query costs one step once
+// the grid reaches a.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "../../include/fpm.h"
@@ -39,10 +43,9 @@ struct Cosmo {
   }
 };
 
-// Dormand-Prince 5(4) embedded Runge-Kutta (Dormand & Prince 1980) with
-// step-size control on a mixed absolute/relative error of 1e-13, landing
-// exactly on x1.
-void integrate(const Cosmo& c, double x0, double x1, double y[4]) {
+// One Dormand-Prince 5 step (Dormand & Prince 1980) of size h from (x, y),
+// without error control.
+void dp_step(const Cosmo& c, double x, double h, const double y[4], double yn[4]) {
   static const double c2 = 1.0 / 5, c3 = 3.0 / 10, c4 = 4.0 / 5, c5 = 8.0 / 9;
   static const double a21 = 1.0 / 5;
   static const double a31 = 3.0 / 40, a32 = 9.0 / 40;
@@ -53,46 +56,66 @@ void integrate(const Cosmo& c, double x0, double x1, double y[4]) {
                       a64 = 49.0 / 176, a65 = -5103.0 / 18656;
   static const double b1 = 35.0 / 384, b3 = 500.0 / 1113, b4 = 125.0 / 192,
                       b5 = -2187.0 / 6784, b6 = 11.0 / 84;
-  static const double e1 = 35.0 / 384 - 5179.0 / 57600, e3 = 500.0 / 1113 - 7571.0 / 16695,
-                      e4 = 125.0 / 192 - 393.0 / 640, e5 = -2187.0 / 6784 + 92097.0 / 339200,
-                      e6 = 11.0 / 84 - 187.0 / 2100, e7 = -1.0 / 40;
-  const double tol = 1e-13;
-  double x = x0;
-  double h = std::min(0.01, x1 - x0);
-  double k1[4], k2[4], k3[4], k4[4], k5[4], k6[4], k7[4], t[4], yn[4];
-  int guard = 0;
-  while (x < x1 && guard++ < 1000000) {
-    if (x + h > x1) h = x1 - x;
-    c.rhs(x, y, k1);
-    for (int i = 0; i < 4; ++i) t[i] = y[i] + h * a21 * k1[i];
-    c.rhs(x + c2 * h, t, k2);
-    for (int i = 0; i < 4; ++i) t[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
-    c.rhs(x + c3 * h, t, k3);
-    for (int i = 0; i < 4; ++i) t[i] = y[i] + h * (a41 * k1[i] + a42 * k2[i] + a43 * k3[i]);
-    c.rhs(x + c4 * h, t, k4);
-    for (int i = 0; i < 4; ++i)
-      t[i] = y[i] + h * (a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i]);
-    c.rhs(x + c5 * h, t, k5);
-    for (int i = 0; i < 4; ++i)
-      t[i] = y[i] + h * (a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i]);
-    c.rhs(x + h, t, k6);
-    for (int i = 0; i < 4; ++i)
-      yn[i] = y[i] + h * (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b5 * k5[i] + b6 * k6[i]);
-    c.rhs(x + h, yn, k7);
-    double err = 0.0;
-    for (int i = 0; i < 4; ++i) {
-      const double ei = h * (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i] +
-                             e7 * k7[i]);
-      const double sc = tol * (1e-30 + std::max(std::fabs(y[i]), std::fabs(yn[i])));
-      err = std::max(err, std::fabs(ei) / sc);
-    }
-    if (err <= 1.0) {
-      x += h;
-      for (int i = 0; i < 4; ++i) y[i] = yn[i];
-    }
-    const double fac = err > 0 ? 0.9 * std::pow(err, -0.2) : 5.0;
-    h *= std::min(5.0, std::max(0.2, fac));
+  double k1[4], k2[4], k3[4], k4[4], k5[4], k6[4], t[4];
+  c.rhs(x, y, k1);
+  for (int i = 0; i < 4; ++i) t[i] = y[i] + h * a21 * k1[i];
+  c.rhs(x + c2 * h, t, k2);
+  for (int i = 0; i < 4; ++i) t[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
+  c.rhs(x + c3 * h, t, k3);
+  for (int i = 0; i < 4; ++i) t[i] = y[i] + h * (a41 * k1[i] + a42 * k2[i] + a43 * k3[i]);
+  c.rhs(x + c4 * h, t, k4);
+  for (int i = 0; i < 4; ++i)
+    t[i] = y[i] + h * (a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i]);
+  c.rhs(x + c5 * h, t, k5);
+  for (int i = 0; i < 4; ++i)
+    t[i] = y[i] + h * (a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i]);
+  c.rhs(x + h, t, k6);
+  for (int i = 0; i < 4; ++i)
+    yn[i] = y[i] + h * (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b5 * k5[i] + b6 * k6[i]);
+}
+
+// The solution on the fixed grid x_k = X0 + k H, one track per cosmology,
+// extended on demand (guarded: Python may call from several threads).
+constexpr double kA0 = 1e-7;
+constexpr double kH = 1.0 / 256;
+struct Track {
+  double om, ol, ok;
+  std::vector<double> y;  // 4 per grid point
+};
+std::mutex g_mu;
+std::vector<Track> g_tracks;
+
+// State at ln a = x >= X0 (the caller holds g_mu).
+void state_at(Track& t, const Cosmo& c, double x, double out[4]) {
+  const double x0 = std::log(kA0);
+  if (t.y.empty()) {
+    t.y = {kA0, kA0, -3.0 / 7.0 * kA0 * kA0, -6.0 / 7.0 * kA0 * kA0};
   }
+  const int64_t k = (int64_t)std::floor((x - x0) / kH);
+  while ((int64_t)(t.y.size() / 4) <= k) {
+    const int64_t j = (int64_t)(t.y.size() / 4) - 1;
+    double yn[4];
+    dp_step(c, x0 + (double)j * kH, kH, &t.y[4 * j], yn);
+    t.y.insert(t.y.end(), yn, yn + 4);
+  }
+  const double xk = x0 + (double)k * kH;
+  if (x > xk) {
+    dp_step(c, xk, x - xk, &t.y[4 * k], out);
+  } else {
+    for (int i = 0; i < 4; ++i) out[i] = t.y[4 * k + i];
+  }
+}
+
+// D1, D1', D2, D2' (unnormalised) at a: the matter-dominated start below a0.
+void growth_at(Track& t, const Cosmo& c, double a, double y[4]) {
+  if (a <= kA0) {
+    y[0] = a;
+    y[1] = a;
+    y[2] = -3.0 / 7.0 * a * a;
+    y[3] = -6.0 / 7.0 * a * a;
+    return;
+  }
+  state_at(t, c, std::log(a), y);
 }
 
 }  // namespace
@@ -103,41 +126,29 @@ extern "C" int fpm_growth(double omega_m, double omega_lambda, double omega_k, d
   for (int i = 0; i < n; ++i)
     if (!(a[i] > 0)) return FPM_ERR_ARG;
   const Cosmo c{omega_m, omega_lambda, omega_k};
-  // points to visit: a_normalize and every request, in increasing order
-  std::vector<double> pts(a, a + n);
-  pts.push_back(a_normalize);
-  std::sort(pts.begin(), pts.end());
-  pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
-  const double a0 = std::min(1e-7, pts.front());
-  double y[4] = {a0, a0, -3.0 / 7.0 * a0 * a0, -6.0 / 7.0 * a0 * a0};
-  double x = std::log(a0);
-  std::vector<double> sol(pts.size() * 6);
-  for (size_t p = 0; p < pts.size(); ++p) {
-    const double xt = std::log(pts[p]);
-    integrate(c, x, xt, y);
-    x = xt;
-    double dy[4];
-    c.rhs(xt, y, dy);
-    double* s = &sol[p * 6];
-    s[0] = y[0];
-    s[1] = y[1];
-    s[2] = dy[1];
-    s[3] = y[2];
-    s[4] = y[3];
-    s[5] = dy[3];
+  std::lock_guard<std::mutex> lock(g_mu);
+  Track* t = nullptr;
+  for (auto& tr : g_tracks)
+    if (tr.om == omega_m && tr.ol == omega_lambda && tr.ok == omega_k) t = &tr;
+  if (!t) {
+    if (g_tracks.size() >= 64) g_tracks.clear();  // bounded: re-solved on demand
+    g_tracks.push_back(Track{omega_m, omega_lambda, omega_k, {}});
+    t = &g_tracks.back();
   }
-  const size_t inorm = std::lower_bound(pts.begin(), pts.end(), a_normalize) - pts.begin();
-  const double d1n = sol[inorm * 6 + 0], d2n = sol[inorm * 6 + 3];
+  double yn[4];
+  growth_at(*t, c, a_normalize, yn);
+  const double d1n = yn[0], d2n = yn[2];
   for (int i = 0; i < n; ++i) {
-    const size_t p = std::lower_bound(pts.begin(), pts.end(), a[i]) - pts.begin();
-    const double* s = &sol[p * 6];
+    double y[4], dy[4];
+    growth_at(*t, c, a[i], y);
+    c.rhs(std::log(a[i]), y, dy);
     double* o = out + 9 * (size_t)i;
-    o[0] = s[0] / d1n;
-    o[1] = s[1] / d1n;
-    o[2] = s[2] / d1n;
-    o[3] = s[3] / d2n;
-    o[4] = s[4] / d2n;
-    o[5] = s[5] / d2n;
+    o[0] = y[0] / d1n;
+    o[1] = y[1] / d1n;
+    o[2] = dy[1] / d1n;
+    o[3] = y[2] / d2n;
+    o[4] = y[3] / d2n;
+    o[5] = dy[3] / d2n;
     o[6] = c.efunc(a[i]);
     o[7] = c.eprime(a[i]);
     o[8] = c.omega_m(a[i]);

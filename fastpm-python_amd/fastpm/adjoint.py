@@ -244,10 +244,17 @@ class ForwardModel:
         self.step = self.solver.nbodystep
         self.actions = list(leapfrog(self.stages))
         self.checkpoints = []
+        pending = []
         for act in self.actions:
             if act[0] == "F":
+                # the kicks and drift since the last Force in one pass
+                self.step.run_actions(pending, state)
+                pending = []
                 self.checkpoints.append(state.S.clone())
-            self.step.run(*act, state, None)
+                self.step.run(*act, state, None)
+            else:
+                pending.append(act)
+        self.step.run_actions(pending, state)
         return state
 
     def backward(self, gS, gP=None, gF=None):

@@ -1,10 +1,12 @@
 """Growth functions and FastPM kick/drift factors (reference:
 fastpm/background.py:4-256).
 
-The ODE is solved by the native library (fpm_growth, fourth-order Runge-Kutta
-in ln a, double precision) exactly at every queried scale factor, so values at
-the grid points of the reference agree with its odeint solution to the ODE
-tolerance, and values off the grid are exact rather than interpolated.
+The ODE is solved by the native library (fpm_growth: Dormand-Prince 5 steps
+of 1/256 in ln a from a = 1e-7, kept per cosmology, and one partial step to
+each queried scale factor; double precision), so values at the grid points
+of the reference agree with its odeint solution to the ODE tolerance, values
+off the grid are solved rather than interpolated, and a value does not
+depend on which other scale factors are queried with it.
 """
 import numpy as np
 

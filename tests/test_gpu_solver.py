@@ -3,7 +3,7 @@ against the FP64 oracle Solver (GPU only).
 
 Float64 runs agree with the oracle to the tolerance of the growth ODE: the
 oracle restates the reference's odeint solve (relative tolerance ~1.5e-8)
-while the product integrates with Dormand-Prince 5(4), so end-to-end
+while the product integrates with fixed-step Dormand-Prince 5, so end-to-end
 agreement is checked at rtol 1e-6 of each array's scale.  Float32 runs are checked against the
 north-star gates at small sizes: P(k) relative error <= 1e-3 for
 k < 0.5 k_Nyq and r(k) >= 0.999."""

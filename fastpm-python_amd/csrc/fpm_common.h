@@ -83,25 +83,33 @@ __device__ __forceinline__ void lattice_of(const PartDesc<T>& P, int64_t p, int3
   }
 }
 
+// Cell and fraction of lattice particle p whose lattice coordinates li are
+// known.  No FMA contraction: every kernel that places a particle (binning,
+// paint, readout) must round its position identically.
+template <typename T>
+__device__ __forceinline__ void lattice_cell(const PartDesc<T>& P, int64_t p, const int32_t li[3],
+                                             int64_t cell[3], T frac[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double base = ((double)li[d] + P.lat_shift) * P.ratio[d];
+    const double fb = floor(base);
+    T t2 = (T)(base - fb);
+    if (P.s) t2 += P.s[3 * p + d] * P.inv_cell[d];
+    const T ft = floor(t2);
+    cell[d] = (int64_t)(int32_t)fb + (int64_t)(int32_t)ft;
+    frac[d] = t2 - ft;
+  }
+}
+
 // Cells are exact for positions within 2^31 cells of the box.
 template <typename T>
 __device__ __forceinline__ void particle_cell(const PartDesc<T>& P, int64_t p, int64_t cell[3], T frac[3]) {
-  // No FMA contraction: every kernel that places a particle (binning, paint,
-  // readout) must round its position identically.
 #pragma clang fp contract(off)
   if (P.q == nullptr) {
     int32_t li[3];
     lattice_of(P, p, li);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double base = ((double)li[d] + P.lat_shift) * P.ratio[d];
-      const double fb = floor(base);
-      T t2 = (T)(base - fb);
-      if (P.s) t2 += P.s[3 * p + d] * P.inv_cell[d];
-      const T ft = floor(t2);
-      cell[d] = (int64_t)(int32_t)fb + (int64_t)(int32_t)ft;
-      frac[d] = t2 - ft;
-    }
+    lattice_cell(P, p, li, cell, frac);
   } else {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {

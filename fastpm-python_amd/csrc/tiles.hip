@@ -116,11 +116,8 @@ static int64_t layout_ws(const TileGeom& g, int64_t n, char* base, Workspace* w)
 // Window origin (wrapped) and the offset t it is weighted by (CIC: the
 // fraction f; TSC: f, or f - 1 when the nearest point is the next cell).
 template <typename T, int ORDER>
-__device__ __forceinline__ void window_t(const PartDesc<T>& P, int64_t p, const int nm[3],
-                                         int i0[3], T t[3]) {
-  int64_t cell[3];
-  T f[3];
-  particle_cell(P, p, cell, f);
+__device__ __forceinline__ void window_cell(const int64_t cell[3], const T f[3], const int nm[3],
+                                            int i0[3], T t[3]) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     int64_t o;
@@ -134,6 +131,15 @@ __device__ __forceinline__ void window_t(const PartDesc<T>& P, int64_t p, const 
     }
     i0[d] = (int)wrapi(o, nm[d]);
   }
+}
+
+template <typename T, int ORDER>
+__device__ __forceinline__ void window_t(const PartDesc<T>& P, int64_t p, const int nm[3],
+                                         int i0[3], T t[3]) {
+  int64_t cell[3];
+  T f[3];
+  particle_cell(P, p, cell, f);
+  window_cell<T, ORDER>(cell, f, nm, i0, t);
 }
 
 // Window weights from t, identical to the plain paint/readout.
@@ -195,19 +201,34 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
 // sides are multiples of 8, wave w takes an 8 x 8 (y, z) patch of one x plane,
 // whose particles fall in a few sub-bins (a wave along one lattice line spans
 // eight tiles); otherwise threads take particles in index order.  key and
-// rank are stored by thread index; the scatter applies the same order.
+// rank are stored by thread index; the scatter applies the same order.  The
+// patch gives a blocked particle's lattice coordinates, so its placement
+// needs no division of the particle index.
 struct BinOrder {
   int blocked;
   uint32_t nz8, ny8, lat1, lat2;
 };
 
-__device__ __forceinline__ int64_t bin_particle(const BinOrder& o, int64_t i) {
-  if (!o.blocked) return i;
-  const uint32_t w = (uint32_t)(i >> 6);
-  const uint32_t lane = (uint32_t)(i & 63);
-  const uint32_t t = w / o.nz8, pz = w - t * o.nz8;
-  const uint32_t ix = t / o.ny8, py = t - ix * o.ny8;
-  return ((int64_t)ix * o.lat1 + py * 8 + (lane >> 3)) * o.lat2 + pz * 8 + (lane & 7);
+template <typename T, int ORDER>
+__device__ __forceinline__ int64_t bin_window(const BinOrder& o, const PartDesc<T>& P,
+                                              const int nm[3], int64_t i, int i0[3], T t[3]) {
+  int64_t cell[3];
+  T f[3];
+  int64_t p = i;
+  if (o.blocked) {
+    const uint32_t w = (uint32_t)(i >> 6);
+    const uint32_t lane = (uint32_t)(i & 63);
+    const uint32_t u = w / o.nz8, pz = w - u * o.nz8;
+    const uint32_t ix = u / o.ny8, py = u - ix * o.ny8;
+    const int32_t li[3] = {(int32_t)ix, (int32_t)(py * 8 + (lane >> 3)),
+                           (int32_t)(pz * 8 + (lane & 7))};
+    p = ((int64_t)li[0] * o.lat1 + li[1]) * o.lat2 + li[2];
+    lattice_cell(P, p, li, cell, f);
+  } else {
+    particle_cell(P, p, cell, f);
+  }
+  window_cell<T, ORDER>(cell, f, nm, i0, t);
+  return p;
 }
 
 // Lanes of a wave that fall in the same sub-bin are merged before touching
@@ -271,10 +292,9 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_count(PartDesc<T> P, TileGe
       const int64_t i = base + u * kBinThreads + threadIdx.x;
       t[u] = -1;
       if (i < P.n) {
-        const int64_t p = bin_particle(o, i);
         int i0[3];
         T tt[3];
-        window_t<T, ORDER>(P, p, g.nm, i0, tt);
+        bin_window<T, ORDER>(o, P, g.nm, i, i0, tt);
         t[u] = bin_of<ORDER>(i0, g);
       }
     }
@@ -358,11 +378,10 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_scatter(PartDesc<T> P, Tile
       const int64_t i = base + u * kBinThreads + threadIdx.x;
       slot[u] = -1;
       if (i < P.n) {
-        const int64_t p = bin_particle(o, i);
         const int32_t rank = w.rank[i];
         int i0[3];
         T t[3];
-        window_t<T, ORDER>(P, p, g.nm, i0, t);
+        const int64_t p = bin_window<T, ORDER>(o, P, g.nm, i, i0, t);
         slot[u] = w.offs[bin_of<ORDER>(i0, g)] + rank;
         pp[u] = (int32_t)p;
         r[u].t[0] = (float)t[0];

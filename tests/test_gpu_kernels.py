@@ -268,10 +268,15 @@ def test_errors(gpu):
 @pytest.mark.parametrize("order", [2, 3])
 @pytest.mark.parametrize("nmesh,kind", [((16, 16, 16), "lattice"), ((32, 48, 64), "random"),
                                         ((64, 64, 64), "lattice"), ((96, 32, 16), "cluster"),
-                                        ((32, 48, 48), "lattice4")])
+                                        ((32, 48, 48), "lattice4"), ((32, 48, 64), "lattice"),
+                                        ((64, 64, 64), "lattice_spread")])
 def test_tiled_paint_readout(gpu, order, nmesh, kind):
     """Binned paint/readout against the oracle; bitwise reproducible; no
-    particle found outside its binned tile."""
+    particle found outside its binned tile.  The lattices cover the three
+    binning orders: 8^3 (8 x 8 patches), 32^3 and 16 x 24 x 32 (4 x 8 x 16
+    blocks per workgroup), 8 x 12 x 12 (index order); lattice_spread moves
+    every particle anywhere within one box either way, so a block's 512
+    particles fall in up to 512 distinct sub-bins."""
     from fastpm.pm import Particles
 
     print("tiled", order, nmesh, kind, flush=True)
@@ -283,7 +288,10 @@ def test_tiled_paint_readout(gpu, order, nmesh, kind):
         nlat = tuple(n // (4 if kind == "lattice4" else 2) for n in nmesh)
         pm_p = _pm(nlat, box, "f4")
         Q = pm_p.generate_uniform_particle_grid(shift=0.0)
-        s = rng.normal(0, 3.0, size=(len(Q), 3))
+        if kind == "lattice_spread":
+            s = rng.uniform(-1, 1, size=(len(Q), 3)) * np.array(box)
+        else:
+            s = rng.normal(0, 3.0, size=(len(Q), 3))
         q = O.lattice(nlat, box)
         parts = Particles(Q, torch.from_numpy(s.astype("f4")).to(gpu))
     else:
@@ -303,9 +311,12 @@ def test_tiled_paint_readout(gpu, order, nmesh, kind):
     b = pm.paint_tiled(parts, mass=0.75).value.clone()
     assert torch.equal(a, b)
     ref = O.paint(X, nmesh, box, order, 0.75)
-    assert _rel(a.cpu().numpy(), ref) < 5e-6
+    # displacements of up to a box side (lattice_spread) are rounded in
+    # float32 to ~1.5e-5 cells, in the kernels and in X alike
+    tol = 2e-5 if kind == "lattice_spread" else 5e-6
+    assert _rel(a.cpu().numpy(), ref) < tol
     plain = pm.paint(parts, mass=0.75).value
-    assert _rel(a.cpu().numpy(), plain.cpu().numpy().astype("f8")) < 5e-6
+    assert _rel(a.cpu().numpy(), plain.cpu().numpy().astype("f8")) < tol
     meshes = [rng.standard_normal(nmesh) for _ in range(4)]
     fields = [pm.create("real", value=torch.from_numpy(m.astype("f4")).to(gpu)) for m in meshes]
     F = torch.empty((len(parts), 3), dtype=torch.float32, device=gpu)
@@ -315,7 +326,7 @@ def test_tiled_paint_readout(gpu, order, nmesh, kind):
     got = [R.cpu().numpy()] + [F[:, d].cpu().numpy() for d in range(3)]
     for g, m, c in zip(got, meshes, sc):
         r = O.readout(m.astype("f4").astype("f8"), X, box, order) * c
-        assert np.abs(g - r).max() < 2e-5 * max(1.0, np.abs(r).max())
+        assert np.abs(g - r).max() < 4 * tol * max(1.0, np.abs(r).max())
     assert pm.tile_misplaced(len(parts), order) == 0
 
 

@@ -210,6 +210,14 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
 // lattice coordinates, so its placement needs no division of the particle
 // index.
 enum { kBinIndex = 0, kBinPatch = 1, kBinBlock = 2 };
+#ifndef FPM_BIN_ILP
+#define FPM_BIN_ILP 2
+#endif
+constexpr int kBinILP = FPM_BIN_ILP;
+constexpr int kBinThreads = 256;
+constexpr int kBinSpan = kBinThreads * kBinILP;  // particles per workgroup step
+constexpr int kBlockX = kBinSpan / 128;          // block: kBlockX x 8 x 16 particles
+static_assert(kBinSpan % 128 == 0 && (kBinSpan & (kBinSpan - 1)) == 0, "block order spans");
 struct BinOrder {
   int blocked;
   uint32_t nz, ny;  // patches (8 x 8) or blocks (4 x 8 x 16) along z and y
@@ -225,10 +233,10 @@ __device__ __forceinline__ int64_t bin_window(const BinOrder& o, const PartDesc<
   if (o.blocked != kBinIndex) {
     int32_t li[3];
     if (o.blocked == kBinBlock) {
-      const uint32_t c = (uint32_t)(i >> 9), j = (uint32_t)(i & 511);
+      const uint32_t c = (uint32_t)(i / kBinSpan), j = (uint32_t)(i % kBinSpan);
       const uint32_t u = c / o.nz, cz = c - u * o.nz;
       const uint32_t cx = u / o.ny, cy = u - cx * o.ny;
-      li[0] = (int32_t)(cx * 4 + (j >> 7));
+      li[0] = (int32_t)(cx * kBlockX + (j >> 7));
       li[1] = (int32_t)(cy * 8 + ((j >> 4) & 7));
       li[2] = (int32_t)(cz * 16 + (j & 15));
     } else {
@@ -257,11 +265,6 @@ __device__ __forceinline__ int64_t bin_window(const BinOrder& o, const PartDesc<
 // thread takes kBinILP particles (one per 256-particle stripe of its
 // block's span), whose position loads and atomics overlap: the pass is
 // bound by their latency, not by bandwidth.
-#ifndef FPM_BIN_ILP
-#define FPM_BIN_ILP 2
-#endif
-constexpr int kBinILP = FPM_BIN_ILP;
-constexpr int kBinThreads = 256;
 
 // Merges a wave's sub-bin keys (t[u] < 0: no particle) and adds each group
 // to its counter; rank[i] = the counter before the add + the lane's rank in
@@ -1252,8 +1255,7 @@ static BinOrder bin_order(const fpm_particles* part) {
   BinOrder o{};
   if (!part->q && part->lattice[2] % 8 == 0 && part->lattice[1] % 8 == 0) {
     // block order needs whole blocks and a workgroup span of one block
-    const bool block = part->lattice[0] % 4 == 0 && part->lattice[2] % 16 == 0 &&
-                       kBinThreads * kBinILP == 512;
+    const bool block = part->lattice[0] % kBlockX == 0 && part->lattice[2] % 16 == 0;
     o.blocked = block ? kBinBlock : kBinPatch;
     o.nz = (uint32_t)(part->lattice[2] / (block ? 16 : 8));
     o.ny = (uint32_t)(part->lattice[1] / 8);

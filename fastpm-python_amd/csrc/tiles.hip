@@ -501,9 +501,11 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_scatter(PartDesc<T> P, Tile
 // to its source tile, which lies o_d tiles below the target along axis d
 // (o_d in {0, 1}); window points outside the target are skipped, and on an
 // axis with a single tile they wrap onto it.
+// A deposit w * mass in fixed point: round(w * mass_fs) * 2^sh, with
+// |mass_fs| < 2^30 so that the rounding is a single-precision conversion.
 template <typename T, int ORDER>
 __device__ __forceinline__ void deposit_rec(const PRec& rec, int o, const TileGeom& g,
-                                            unsigned long long* acc, double mass_fx) {
+                                            unsigned long long* acc, float mass_fs, int sh) {
   int l[3];
   T wt[3][ORDER];
   window_rec<T, ORDER>(rec, l, wt);
@@ -527,9 +529,9 @@ __device__ __forceinline__ void deposit_rec(const PRec& rec, int o, const TileGe
 #pragma unroll
       for (int d = 0; d < ORDER; ++d) {
         if ((unsigned)u[2][d] >= (unsigned)TILE) continue;
-        const double v = (double)(wac * wt[2][d]) * mass_fx;
-        const long long q = __double2ll_rn(v);
-        atomicAdd(&acc[(u[0][a] * TILE + u[1][c]) * TILE + u[2][d]], (unsigned long long)q);
+        const int qi = __float2int_rn((float)(wac * wt[2][d]) * mass_fs);
+        const unsigned long long q = (unsigned long long)(long long)qi << sh;
+        atomicAdd(&acc[(u[0][a] * TILE + u[1][c]) * TILE + u[2][d]], q);
       }
     }
   }
@@ -580,8 +582,8 @@ __device__ __forceinline__ void gather_ranges(const TileGeom& g, const Workspace
 // as one flat index space, so their loads are issued together.
 template <typename T, int ORDER>
 __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, TileGeom g,
-                                                               Workspace w, T* mesh, double mass_fx,
-                                                               double inv_fx) {
+                                                               Workspace w, T* mesh, float mass_fs,
+                                                               int sh, double inv_fx) {
   constexpr int NB = TILE * TILE * TILE;
   __shared__ unsigned long long acc[NB];
   __shared__ int32_t rstart[64];
@@ -600,7 +602,7 @@ __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, Til
 #pragma unroll
     for (int step = 32; step >= 1; step >>= 1)
       if (rpre[r + step] <= f) r += step;
-    deposit_rec<T, ORDER>(w.rec[rstart[r] + (f - rpre[r])], rdir[r], g, acc, mass_fx);
+    deposit_rec<T, ORDER>(w.rec[rstart[r] + (f - rpre[r])], rdir[r], g, acc, mass_fs, sh);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < NB; i += kTileThreads) {
@@ -1323,14 +1325,21 @@ static int paint_tiles_impl(const fpm_particles* part, const fpm_mesh* m, int or
   const double total = std::fabs(mass) * (double)(P.n > 0 ? P.n : 1);
   int S = 62 - (int)std::ceil(std::log2(total + 1.0));
   if (S > 60) S = 60;
-  const double mass_fx = mass * std::ldexp(1.0, S);
+  // a deposit is rounded to a multiple of 2^sh, the shift that keeps
+  // |mass| * 2^(S - sh) below 2^30 (a relative step of 2^-30 of the mass)
+  int e = 0;
+  std::frexp(std::fabs(mass) * std::ldexp(1.0, S), &e);  // |mass| 2^S < 2^e
+  const int sh = e > 30 ? e - 30 : 0;
+  const float mass_fs = (float)(mass * std::ldexp(1.0, S - sh));
   const double inv_fx = std::ldexp(1.0, -S);
   T* mesh = static_cast<T*>(out);
   const dim3 grid((unsigned)g.ntiles), blk(kTileThreads);
   if (order == 2)
-    hipLaunchKernelGGL((k_paint_tiles<T, 2>), grid, blk, 0, st, P, g, w, mesh, mass_fx, inv_fx);
+    hipLaunchKernelGGL((k_paint_tiles<T, 2>), grid, blk, 0, st, P, g, w, mesh, mass_fs, sh,
+                       inv_fx);
   else
-    hipLaunchKernelGGL((k_paint_tiles<T, 3>), grid, blk, 0, st, P, g, w, mesh, mass_fx, inv_fx);
+    hipLaunchKernelGGL((k_paint_tiles<T, 3>), grid, blk, 0, st, P, g, w, mesh, mass_fs, sh,
+                       inv_fx);
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }

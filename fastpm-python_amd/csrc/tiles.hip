@@ -198,12 +198,12 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
 }
 
 // Thread-to-particle order of the binning passes.  On a lattice whose sides
-// are multiples of (4, 8, 16), a workgroup's 512 particles are a 4 x 8 x 16
-// lattice block (a quarter of a 16^3 tile's cells at B = 2 before the
-// particles move, so they fall in few sub-bins) and each wave a 4 x 16 (y, z)
-// layer of it: rows of 16 particles keep the position loads on whole cache
-// lines (an 8^3 cube of 8-particle rows measured 24 us slower in the
-// scatter).  With only the y and z sides multiples of 8, wave w takes the
+// are multiples of (kBlockX, 8, 16), a workgroup step's kBinSpan particles
+// are a kBlockX x 8 x 16 lattice block (4 x 8 x 16 with two particles per
+// thread: a quarter of a 16^3 tile's cells at B = 2 before the particles
+// move, so they fall in few sub-bins) and each wave a 4 x 16 (y, z) layer of
+// it: rows of 16 particles keep the position loads on whole cache lines (an
+// 8^3 cube of 8-particle rows measured 24 us slower in the scatter).  With only the y and z sides multiples of 8, wave w takes the
 // w-th 8 x 8 (y, z) patch of the x planes; otherwise threads take particles
 // in index order.  key and rank are stored by thread index; the scatter
 // applies the same order.  The block or patch gives a blocked particle's
@@ -220,7 +220,7 @@ constexpr int kBlockX = kBinSpan / 128;          // block: kBlockX x 8 x 16 part
 static_assert(kBinSpan % 128 == 0 && (kBinSpan & (kBinSpan - 1)) == 0, "block order spans");
 struct BinOrder {
   int blocked;
-  uint32_t nz, ny;  // patches (8 x 8) or blocks (4 x 8 x 16) along z and y
+  uint32_t nz, ny;  // patches (8 x 8) or blocks (kBlockX x 8 x 16) along z and y
   uint32_t lat1, lat2;
 };
 

@@ -38,6 +38,27 @@ __global__ void k_kick_drift(T* S, T* P, const T* F, int64_t n, T k0, T k1, int 
   }
 }
 
+// Adjoint of a step boundary, traversed backwards: the drift's transpose
+// gP += d gS, then the kicks' transposes gF += k_i gP, each rounded as
+// k_axpy.  fresh = 1: gF holds no gradient yet (it was consumed by a force's
+// transpose), so it starts from zero and is not read.  With no kick gF is
+// left alone.
+template <typename T>
+__global__ void k_drift_kick_vjp(const T* gS, T* gP, T* gF, int64_t n, T d, T k0, T k1,
+                                 int nkick, int fresh) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T p = madd(d, gS[i], gP[i]);
+    gP[i] = p;
+    if (nkick > 0) {
+      T f = fresh ? (T)0 : gF[i];
+      f = madd(k0, p, f);
+      if (nkick > 1) f = madd(k1, p, f);
+      gF[i] = f;
+    }
+  }
+}
+
 // y += table[counter[0] * stride + slot] * x: the coefficient is read on the
 // device, so one captured graph can replay every step of a run.  A counter
 // outside [0, rows) leaves y unchanged.
@@ -163,6 +184,21 @@ static int kick_drift_impl(void* S, void* P, const void* F, int64_t n, const dou
 }
 
 template <typename T>
+static int drift_kick_vjp_impl(const void* gS, void* gP, void* gF, int64_t n, const double* kicks,
+                               int nkick, double drift, int fresh, hipStream_t st) {
+  if (n < 0 || nkick < 0 || nkick > 2 || (nkick > 0 && !kicks) ||
+      (n > 0 && (!gS || !gP || (nkick > 0 && !gF))))
+    return FPM_ERR_ARG;
+  if (n == 0) return FPM_OK;
+  const T k0 = nkick > 0 ? (T)kicks[0] : (T)0, k1 = nkick > 1 ? (T)kicks[1] : (T)0;
+  hipLaunchKernelGGL(k_drift_kick_vjp<T>, dim3(grid_elems(n)), dim3(256), 0, st,
+                     static_cast<const T*>(gS), static_cast<T*>(gP), static_cast<T*>(gF), n,
+                     (T)drift, k0, k1, nkick, fresh ? 1 : 0);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
 static int axpy_table_impl(void* y, const void* x, int64_t n, const double* table,
                            const int32_t* counter, int rows, int stride, int slot,
                            hipStream_t st) {
@@ -265,6 +301,14 @@ int fpm_kick_drift_f32(void* S, void* P, const void* F, int64_t n, const double*
 int fpm_kick_drift_f64(void* S, void* P, const void* F, int64_t n, const double* kicks,
                       int nkick, double drift, fpm_stream_t s) {
   return kick_drift_impl<double>(S, P, F, n, kicks, nkick, drift, (hipStream_t)s);
+}
+int fpm_drift_kick_vjp_f32(const void* gS, void* gP, void* gF, int64_t n, const double* kicks,
+                           int nkick, double drift, int fresh, fpm_stream_t s) {
+  return drift_kick_vjp_impl<float>(gS, gP, gF, n, kicks, nkick, drift, fresh, (hipStream_t)s);
+}
+int fpm_drift_kick_vjp_f64(const void* gS, void* gP, void* gF, int64_t n, const double* kicks,
+                           int nkick, double drift, int fresh, fpm_stream_t s) {
+  return drift_kick_vjp_impl<double>(gS, gP, gF, n, kicks, nkick, drift, fresh, (hipStream_t)s);
 }
 int fpm_axpy_table_f32(void* y, const void* x, int64_t n, const double* table,
                        const int32_t* counter, int rows, int stride, int slot, fpm_stream_t s) {

@@ -63,6 +63,7 @@ _TYPED = {
     "fpm_grad4_div": (_int, [_vp, _vp, _PM, _dbl, _int, _vp]),
     "fpm_drift_wrap": (_int, [_vp, _vp, _i64, _dbl, _vp, _vp]),
     "fpm_kick_drift": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _dbl, _vp]),
+    "fpm_drift_kick_vjp": (_int, [_vp, _vp, _vp, _i64, _vp, _int, _dbl, _int, _vp]),
     "fpm_fft_c2r": (_int, [_vp, _int, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
     "fpm_fft_r2c_c2r": (_int, [_vp, _vp, _int, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),
     "fpm_fft_c2r_grad3": (_int, [_vp, _PK, _vp, _vp, _PM, _i64, _vp, _vp, _vp, _vp]),

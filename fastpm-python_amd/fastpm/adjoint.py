@@ -24,6 +24,8 @@ checkpoint of the displacement taken before it (per-step checkpointing):
     D: S += c P     ->  gP += c gS
     F: F = f(S)     ->  gS += f'(S)^T gF,  gF = 0
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -257,6 +259,11 @@ class ForwardModel:
         self.step.run_actions(pending, state)
         return state
 
+    # the reverse pass applies a drift and the kicks before it (in the
+    # forward order) as one fpm_drift_kick_vjp pass; False: one fpm_axpy per
+    # action (the same result bit for bit)
+    fuse_vjp = True
+
     def backward(self, gS, gP=None, gF=None):
         """Returns G_w, the gradient with respect to the white-noise modes,
         given the gradients with respect to the final S, P and F."""
@@ -265,7 +272,27 @@ class ForwardModel:
         gF = torch.zeros_like(gS) if gF is None else gF.clone()
         pt = self.step._pt
         ck = list(self.checkpoints)
-        for action, ai, ac, af in reversed(self.actions):
+        acts = list(reversed(self.actions))
+        fresh = False  # gF was consumed by a force VJP and not yet cleared
+        i = 0
+        while i < len(acts):
+            action, ai, ac, af = acts[i]
+            if action == "D" and self.fuse_vjp:
+                kicks = []
+                j = i + 1
+                while j < len(acts) and acts[j][0] == "K" and len(kicks) < 2:
+                    kicks.append(pt.kick_factor(*acts[j][1:]))
+                    j += 1
+                kc = (ctypes.c_double * 2)(*(kicks + [0.0] * (2 - len(kicks))))
+                _lib.call("fpm_drift_kick_vjp", _np_dtype(gS), gS.data_ptr(), gP.data_ptr(),
+                          gF.data_ptr(), gS.numel(), kc, len(kicks),
+                          float(pt.drift_factor(ai, ac, af)), int(fresh), _stream())
+                fresh = fresh and not kicks
+                i = j
+                continue
+            if fresh:
+                gF.zero_()
+                fresh = False
             if action == "K":
                 _axpy(gF, gP, pt.kick_factor(ai, ac, af))
             elif action == "D":
@@ -273,7 +300,10 @@ class ForwardModel:
             else:
                 S = ck.pop()
                 force_vjp(self.step, S, self.Q, gF, gS)
-                gF.zero_()
+                fresh = True
+            i += 1
+        if fresh:
+            gF.zero_()
         G = lpt_vjp(self.solver, self.dlin, self.Q, self.stages[0], gS, gP, gF, self.order)
         _scale_by_table(G.pm, G, self._lin_tab)
         return G

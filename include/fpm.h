@@ -302,6 +302,20 @@ int fpm_drift_wrap_f32(void* S, const void* P, int64_t n, double a, const double
 int fpm_drift_wrap_f64(void* S, const void* P, int64_t n, double a, const double* box,
                        fpm_stream_t stream);
 
+/* The transpose of fpm_kick_drift for the reverse pass of the differentiable
+ * model: over n elements gP += drift * gS (the drift's VJP), then
+ * gF += kicks[i] * gP for i < nkick (the kicks' VJPs), each rounded as
+ * fpm_axpy, so the result equals the separate fpm_axpy calls bit for bit.
+ * fresh != 0 means gF holds no gradient yet (it was just consumed by a
+ * force's VJP): it then starts from zero and is not read.  With nkick = 0
+ * gF is not touched.  The VJPs of Kick (fastpm/core.py:172, P += fac F) and
+ * Drift (core.py:179, S += fac P) in the reverse order of a leapfrog step
+ * boundary (core.py:261-288). */
+int fpm_drift_kick_vjp_f32(const void* gS, void* gP, void* gF, int64_t n, const double* kicks,
+                           int nkick, double drift, int fresh, fpm_stream_t stream);
+int fpm_drift_kick_vjp_f64(const void* gS, void* gP, void* gF, int64_t n, const double* kicks,
+                           int nkick, double drift, int fresh, fpm_stream_t stream);
+
 /* y += table[counter[0] * stride + slot] * x, with the FP64 coefficient table
  * ([rows][stride]) and the step counter in device memory, so a captured graph
  * can replay the kick/drift updates of every step (fastpm/core.py:168-180

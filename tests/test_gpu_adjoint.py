@@ -199,6 +199,58 @@ def test_fp32_gradient_matches_fp64(gpu):
     assert np.abs(a - b).max() < 1e-2 * np.abs(b).max()
 
 
+@pytest.mark.parametrize("dtype", ["f4", "f8"])
+@pytest.mark.parametrize("nkick", [0, 1, 2])
+@pytest.mark.parametrize("fresh", [0, 1])
+def test_drift_kick_vjp_bitwise(gpu, dtype, nkick, fresh):
+    """fpm_drift_kick_vjp equals the separate fpm_axpy calls bit for bit
+    (gF cleared first when fresh, untouched without kicks)."""
+    import ctypes
+    from fastpm import _lib
+    from fastpm.pm import _stream
+
+    print("drift_kick_vjp", dtype, nkick, fresh, flush=True)
+    tdt = torch.float32 if dtype == "f4" else torch.float64
+    g = torch.Generator(device="cpu").manual_seed(5)
+    n = 3 * 10007
+    gS, gP, gF = (torch.randn(n, generator=g, dtype=tdt).to(gpu) for _ in range(3))
+    kicks, drift = [0.37, -1.3e-2][:nkick], 0.61
+    a = [gS.clone(), gP.clone(), gF.clone()]
+    _lib.call("fpm_axpy", dtype, a[1].data_ptr(), a[0].data_ptr(), n, drift, _stream())
+    if fresh and nkick:
+        a[2].zero_()
+    for k in kicks:
+        _lib.call("fpm_axpy", dtype, a[2].data_ptr(), a[1].data_ptr(), n, k, _stream())
+    b = [gS.clone(), gP.clone(), gF.clone()]
+    kc = (ctypes.c_double * 2)(*(kicks + [0.0] * (2 - nkick)))
+    _lib.call("fpm_drift_kick_vjp", dtype, b[0].data_ptr(), b[1].data_ptr(), b[2].data_ptr(), n,
+              kc, nkick, drift, fresh, _stream())
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("dtype", ["f4", "f8"])
+def test_fused_reverse_pass_bitwise(gpu, dtype):
+    """The reverse pass with fused drift/kick VJPs equals the pass with one
+    fpm_axpy per action: bit for bit in float32 (fixed-point tiled paints);
+    in float64 the force VJP's atomic paints sum in arrival order, so two
+    runs agree to rounding only."""
+    print("fused reverse pass", dtype, flush=True)
+    solver, model, loss, _ = _model(dtype, nc=16, box=100.0, nsteps=3)
+    st = model.forward(solver.whitenoise(7))
+    _, gS = loss(st)
+    out = {}
+    for fuse in (True, False):
+        model.fuse_vjp = fuse
+        out[fuse] = model.backward(gS).value.clone()
+    model.fuse_vjp = True
+    if dtype == "f4":
+        assert torch.equal(out[True], out[False])
+    else:
+        d = (out[True] - out[False]).abs().max() / out[False].abs().max()
+        assert float(d) < 1e-12
+
+
 def test_autograd_function(gpu):
     from fastpm.adjoint import SimulateFunction
 

@@ -128,7 +128,8 @@ int fpm_readout_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, 
  * fpm_paint_tiled and fpm_readout_tiled read the lists only -- the positions
  * as of the bin call are used -- so they must follow a bin call with the same
  * particles, mesh and order.  The tiled paint accumulates in 64-bit fixed
- * point, is bitwise reproducible, overwrites `out` (hold=False) and needs no
+ * point (each deposit rounded to a step of 2^-30 of |mass| or finer), is
+ * bitwise reproducible, overwrites `out` (hold=False) and needs no
  * clearing pass.  Same contracts as fpm_paint / fpm_readout
  * (fastpm/core.py:191,203; fastpm/force/gravity.py:24).
  * fpm_tile_misplaced_offset gives the byte offset in the workspace of an int32

@@ -40,6 +40,13 @@ def _axpy(y, x, a):
     _lib.call("fpm_axpy", _np_dtype(y), y.data_ptr(), x.data_ptr(), y.numel(), float(a), _stream())
 
 
+def _drift_kick_vjp(gS, gP, gF, drift, kicks, fresh):
+    """gP += drift gS, then gF (zero if fresh) += sum_i kicks[i] gP in one pass."""
+    kc = (ctypes.c_double * 2)(*(list(kicks) + [0.0] * (2 - len(kicks))))
+    _lib.call("fpm_drift_kick_vjp", _np_dtype(gS), gS.data_ptr(), gP.data_ptr(), gF.data_ptr(),
+              gS.numel(), kc, len(kicks), float(drift), int(fresh), _stream())
+
+
 def _np_dtype(t):
     return np.float32 if t.dtype == torch.float32 else np.float64
 
@@ -283,10 +290,7 @@ class ForwardModel:
                 while j < len(acts) and acts[j][0] == "K" and len(kicks) < 2:
                     kicks.append(pt.kick_factor(*acts[j][1:]))
                     j += 1
-                kc = (ctypes.c_double * 2)(*(kicks + [0.0] * (2 - len(kicks))))
-                _lib.call("fpm_drift_kick_vjp", _np_dtype(gS), gS.data_ptr(), gP.data_ptr(),
-                          gF.data_ptr(), gS.numel(), kc, len(kicks),
-                          float(pt.drift_factor(ai, ac, af)), int(fresh), _stream())
+                _drift_kick_vjp(gS, gP, gF, pt.drift_factor(ai, ac, af), kicks, fresh)
                 fresh = fresh and not kicks
                 i = j
                 continue

@@ -130,3 +130,67 @@ def test_run_actions_groups_kicks_and_drift():
     s = Rec()
     s.run_actions(acts, None, monitor=lambda *args: None)
     assert s.log == [a[0] for a in acts]
+
+
+def test_reverse_pass_fuses_drift_and_kicks():
+    """ForwardModel.backward applies each drift and the kicks before it as
+    one fused VJP (gF counted as zero after a force VJP instead of cleared),
+    with the same result as one axpy per action.  The device calls are
+    replaced by float64 CPU stand-ins, so no GPU is needed."""
+    import types
+    import torch
+    import fastpm.adjoint as A
+    from fastpm.core import leapfrog
+
+    log = []
+
+    def axpy(y, x, a):
+        log.append("axpy")
+        y.add_(a * x)
+
+    def dkv(gS, gP, gF, drift, kicks, fresh):
+        log.append("D" + "K" * len(kicks) + ("*" if fresh else ""))
+        gP.add_(drift * gS)
+        if kicks:
+            f = torch.zeros_like(gF) if fresh else gF.clone()
+            for k in kicks:
+                f = f + k * gP
+            gF.copy_(f)
+
+    def fvjp(step, S, Q, gF, gS):
+        log.append("F")
+        gS.add_(0.3 * gF + 0.01 * S)
+
+    def lvjp(solver, dlin, Q, a, gS, gP, gF, order):
+        return types.SimpleNamespace(pm=None, vals=(gS.clone(), gP.clone(), gF.clone()))
+
+    pt = types.SimpleNamespace(kick_factor=lambda ai, ac, af: 1.0 + ai + 2 * ac + 3 * af,
+                               drift_factor=lambda ai, ac, af: 0.5 - ai + ac * af)
+    stages = [0.1, 0.4, 0.7, 1.0]
+    saved = (A._axpy, A._drift_kick_vjp, A.force_vjp, A.lpt_vjp, A._scale_by_table)
+    A._axpy, A._drift_kick_vjp, A.force_vjp, A.lpt_vjp = axpy, dkv, fvjp, lvjp
+    A._scale_by_table = lambda pm, G, tab: None
+    try:
+        out, logs = {}, {}
+        for fuse in (True, False):
+            m = A.ForwardModel.__new__(A.ForwardModel)
+            m.fuse_vjp = fuse
+            m.actions = list(leapfrog(stages))
+            m.step = types.SimpleNamespace(_pt=pt)
+            g = torch.Generator().manual_seed(3)
+            m.checkpoints = [torch.randn(12, generator=g, dtype=torch.float64) for _ in range(4)]
+            m.solver = m.dlin = m.Q = m._lin_tab = None
+            m.stages, m.order = stages, 2
+            gS = torch.randn(12, generator=g, dtype=torch.float64)
+            gP = torch.randn(12, generator=g, dtype=torch.float64)
+            gF = torch.randn(12, generator=g, dtype=torch.float64)
+            log.clear()
+            out[fuse] = m.backward(gS, gP, gF).vals
+            logs[fuse] = list(log)
+    finally:
+        A._axpy, A._drift_kick_vjp, A.force_vjp, A.lpt_vjp, A._scale_by_table = saved
+    assert logs[True] == ["axpy", "F", "DKK*", "F", "DKK*", "F", "DK*", "F"]
+    assert logs[False] == ["axpy", "F"] + ["axpy"] * 3 + ["F"] + ["axpy"] * 3 + ["F"] + \
+        ["axpy"] * 2 + ["F"]
+    for a, b in zip(out[True], out[False]):
+        assert torch.equal(a, b)

@@ -14,7 +14,8 @@ its call sites rely on, and each function says how it is pinned:
   function           follows                                 pinned by (tests/test_oracle.py)
   ----------------   -------------------------------------   ------------------------------------
   philox4x32         Salmon et al. SC11 (Random123)           published known-answer vectors
-  whitenoise         core.py:113-114 (pmesh generate_...)     Hermitian symmetry, moments, |w|=1
+  whitenoise         core.py:113-114 (pmesh generate_...)     Hermitian symmetry, moments, |w|=1,
+                                                              real self-conjugate modes with <w^2>=1
   linear             core.py:116-118                          unitary noise -> P(k) exact per mode
   r2c / c2r          core.py:205 (pmesh r2c/c2r)              constant field, plane wave, round trip
   k_laplace/gradient force/kernels.py:3-35                    plane waves (closed forms), Hermiticity
@@ -23,8 +24,11 @@ its call sites rely on, and each function says how it is pinned:
                                                               adjoint dot-product identity
   lpt1               force/lpt.py:4-23                        ZA of a plane wave (closed form)
   lpt2source         force/lpt.py:25-64                       single wave -> 0; two waves closed form
-  Growth             background.py:4-256                      Einstein-de Sitter closed form,
-                                                              independent DOP853 solve
+  Growth             background.py:4-256                      Einstein-de Sitter closed forms of D1,
+                                                              D2, gf, gf2 and the kick/drift factors;
+                                                              independent DOP853 solve; the growth-ODE
+                                                              identities gf = 3/2 Om(a) a E D1 and
+                                                              gf2 = 3/2 Om(a) a E (D2 - c D1^2)
   lpt (Solver.lpt)   core.py:120-145                          kick identity of test_core.py:46-62
   leapfrog/autostages core.py:212-317                         counts of test_core.py:11-30
   kick/drift/force   core.py:168-210, force/gravity.py:10-29  JANUS reversibility (examples/janus.py),

@@ -49,6 +49,46 @@ def test_growth_api_matches_oracle():
                                    err_msg=name)
 
 
+def test_growth_eds_closed_forms_product():
+    """The product's growth and stepping factors in Einstein-de Sitter
+    (Omega_m = 1): gf = 3/2 a^(1/2), gf2 = 5 a^(3/2), Gf = a^(3/2),
+    kick = (af^(3/2) - ai^(3/2)) / (3/2 ac), drift = (af - ai) / ac^(3/2)."""
+    from fastpm.background import MatterDominated
+
+    a = np.array([0.1, 0.3, 0.7, 1.0])
+    p = MatterDominated(1.0, a_normalize=1.0)
+    np.testing.assert_allclose(p.D1(a), a, rtol=1e-9)
+    np.testing.assert_allclose(p.D2(a), a ** 2, rtol=1e-9)
+    np.testing.assert_allclose(p.Gf(a), a ** 1.5, rtol=1e-9)
+    np.testing.assert_allclose(p.gf(a), 1.5 * a ** 0.5, rtol=1e-9)
+    np.testing.assert_allclose(p.gf2(a), 5 * a ** 1.5, rtol=1e-9)
+    for ai, ac, af in [(0.1, 0.1, (0.1 * 0.2) ** 0.5), (0.2, 0.25, 0.3), (0.5, 0.7, 1.0)]:
+        assert p.kick_factor(ai, ac, af) == pytest.approx(
+            (af ** 1.5 - ai ** 1.5) / (1.5 * ac), rel=1e-9)
+        assert p.drift_factor(ai, ac, af) == pytest.approx((af - ai) / ac ** 1.5, rel=1e-9)
+
+
+@pytest.mark.parametrize("Om,a_norm", [(0.3, 1.0), (0.3075, 0.5)])
+def test_growth_lcdm_force_factor_identity_product(Om, a_norm):
+    """gf = 3/2 Om(a) a E D1 and gf2 = 3/2 Om(a) a E (D2 - c1^2/c2 D1^2)
+    (consequences of the growth ODE; c1, c2 the raw D1, D2 at a_normalize)
+    against an independent DOP853 solve, for the product's MatterDominated."""
+    from fastpm.background import MatterDominated
+    from test_oracle import independent_growth
+
+    a = np.array([0.05, 0.1, 0.4, 0.8, 1.0])
+    p = MatterDominated(Om, a_normalize=a_norm)
+    D1, D2, c1, c2 = independent_growth(Om, a, a_norm)
+    D1, D2 = D1 / c1, D2 / c2
+    E = np.sqrt(Om / a ** 3 + 1 - Om)
+    Oma = Om / a ** 3 / E ** 2
+    np.testing.assert_allclose(p.D1(a), D1, rtol=1e-9)
+    np.testing.assert_allclose(p.D2(a), D2, rtol=1e-9)
+    np.testing.assert_allclose(p.gf(a), 1.5 * Oma * a * E * D1, rtol=1e-9)
+    np.testing.assert_allclose(p.gf2(a), 1.5 * Oma * a * E * (D2 - c1 ** 2 / c2 * D1 ** 2),
+                               rtol=1e-9)
+
+
 def test_kernel_composition():
     from fastpm.force.kernels import laplace, gradient, longrange, compensation, KernelOp
 

@@ -51,6 +51,30 @@ def test_whitenoise_moments():
     np.testing.assert_array_equal(O.whitenoise((8, 8, 8), 4), O.whitenoise((8, 8, 8), 4))
 
 
+@pytest.mark.parametrize("unitary", [False, True])
+def test_whitenoise_self_conjugate_modes(unitary):
+    """The seven nonzero self-conjugate modes of an even mesh (every index 0 or
+    n/2) are their own conjugates, so they must be real; they must still carry
+    <w^2> = 1 like every other mode (a real Gaussian with the complex modes'
+    amplitude would give 1/2), be Gaussian (fourth moment 3) and symmetric in
+    sign.  Unitary noise puts +1 or -1 there, each half the time."""
+    vals = []
+    for seed in range(1500):
+        w = O.whitenoise((4, 4, 4), seed, unitary)
+        for i, j, k in itertools.product((0, 2), repeat=3):
+            if (i, j, k) != (0, 0, 0):
+                vals.append(w[i, j, k])
+    v = np.array(vals)
+    assert np.all(v.imag == 0)
+    x = v.real
+    assert abs((x > 0).mean() - 0.5) < 0.03
+    if unitary:
+        np.testing.assert_array_equal(np.abs(x), 1.0)
+    else:
+        assert abs((x ** 2).mean() - 1) < 0.05
+        assert abs((x ** 4).mean() - 3) < 0.4
+
+
 def test_linear_unitary_power_exact():
     """With unitary noise, V |delta_k|^2 equals P(|k|) on every mode."""
     nmesh, box = (16, 16, 16), (100.0,) * 3
@@ -246,6 +270,70 @@ def test_growth_lcdm_independent_solver():
     np.testing.assert_allclose(g.D1(a), s.y[0] / s.y[0][-1], rtol=1e-7)
     np.testing.assert_allclose(g.D2(a), s.y[2] / s.y[2][-1], rtol=1e-6)
     np.testing.assert_allclose(g.D1(a, 1), s.y[1] / s.y[0][-1], rtol=1e-7)
+
+
+def independent_growth(Om, a, a_normalize):
+    """Raw (unnormalised) D1, D2 of the growth ODE at a and at a_normalize from
+    a DOP853 solve that shares no code with the oracle (scipy.integrate.solve_ivp,
+    rtol 1e-13), with E and Omega_m(a) written out for a flat matter +
+    Lambda universe."""
+    from scipy.integrate import solve_ivp
+
+    def rhs(x, y):
+        A = np.exp(x)
+        E = np.sqrt(Om / A ** 3 + 1 - Om)
+        Ep = A * 0.5 / E * (-3 * Om / A ** 4)
+        h = -2 - Ep / E
+        o = 1.5 * Om / A ** 3 / E ** 2
+        return [y[1], h * y[1] + o * y[0], y[3], h * y[3] + o * y[2] - o * y[0] ** 2]
+
+    a0 = 1e-7
+    pts = np.unique(np.concatenate([np.atleast_1d(a), [a_normalize]]))
+    s = solve_ivp(rhs, [np.log(a0), np.log(pts[-1])], [a0, a0, -3 / 7 * a0 ** 2, -6 / 7 * a0 ** 2],
+                  method="DOP853", rtol=1e-13, atol=1e-300, t_eval=np.log(pts))
+    at = {float(p): (s.y[0][i], s.y[2][i]) for i, p in enumerate(pts)}
+    D1 = np.array([at[float(x)][0] for x in np.atleast_1d(a)])
+    D2 = np.array([at[float(x)][1] for x in np.atleast_1d(a)])
+    return D1, D2, at[float(a_normalize)][0], at[float(a_normalize)][1]
+
+
+def test_growth_eds_force_and_step_factors():
+    """Einstein-de Sitter closed forms of what the stepping uses.  With
+    Omega_m = 1: D1 = a, D2 = a^2 (each normalised at a = 1), E = a^(-3/2), so
+    Gf = D1' a^2 E = a^(3/2), gf = dGf/da = 3/2 a^(1/2), gf2 = 5 a^(3/2), and
+    the FastPM factors of core.py:168-180 are
+    kick = (af^(3/2) - ai^(3/2)) / (3/2 ac) and drift = (af - ai) / ac^(3/2).
+    Dropping any of gf's three terms (D1'' a^2 E, D1' a^2 E', 2 D1' a^2 E)
+    changes the 3/2 and fails here."""
+    a = np.array([0.1, 0.3, 0.7, 1.0])
+    g = O.Growth(1.0, a=a, a_normalize=1.0)
+    np.testing.assert_allclose(g.Gf(a), a ** 1.5, rtol=1e-7)
+    np.testing.assert_allclose(g.gf(a), 1.5 * a ** 0.5, rtol=1e-7)
+    np.testing.assert_allclose(g.gf2(a), 5 * a ** 1.5, rtol=1e-7)
+    s = O.Solver(8, 100.0, 1.0)
+    for ai, ac, af in [(0.1, 0.1, (0.1 * 0.2) ** 0.5), (0.2, 0.25, 0.3), (0.5, 0.7, 1.0)]:
+        assert s.kick_factor(ai, ac, af) == pytest.approx(
+            (af ** 1.5 - ai ** 1.5) / (1.5 * ac), rel=1e-7)
+        assert s.drift_factor(ai, ac, af) == pytest.approx((af - ai) / ac ** 1.5, rel=1e-7)
+
+
+@pytest.mark.parametrize("Om,a_norm", [(0.3, 1.0), (0.3075, 0.5)])
+def test_growth_lcdm_force_factor_identity(Om, a_norm):
+    """gf = dGf/da, which the growth ODE (D1'' = -(2 + E'/E) D1' + 3/2 Om(a) D1)
+    turns into gf(a) = 3/2 Om(a) a E(a) D1(a); with D1, D2 normalised
+    separately by their raw values c1, c2 at a_normalize the same algebra gives
+    gf2(a) = 3/2 Om(a) a E(a) (D2(a) - c1^2/c2 D1(a)^2).  The right-hand sides
+    use neither second derivatives nor E', and c1, c2, D1, D2 come from the
+    independent solve."""
+    a = np.array([0.05, 0.1, 0.4, 0.8, 1.0])
+    g = O.Growth(Om, a=a, a_normalize=a_norm)
+    D1, D2, c1, c2 = independent_growth(Om, a, a_norm)
+    D1, D2 = D1 / c1, D2 / c2
+    E = np.sqrt(Om / a ** 3 + 1 - Om)
+    Oma = Om / a ** 3 / E ** 2
+    np.testing.assert_allclose(g.gf(a), 1.5 * Oma * a * E * D1, rtol=1e-7)
+    np.testing.assert_allclose(g.gf2(a), 1.5 * Oma * a * E * (D2 - c1 ** 2 / c2 * D1 ** 2),
+                               rtol=1e-7)
 
 
 # ---------------------------------------------------------------- stepping

@@ -18,8 +18,8 @@ compared: P(k) relative error for k < k_Nyq/2, r(k), and the largest periodic
 displacement difference.  Further legs (each can be skipped): the float64
 oracle's KDK step on the host cores (cpu_baseline), 64 boxes of uneven sizes
 replayed from a captured graph (ensemble), the differentiable model's forward
-and adjoint pass at 128^3 / 256^3 (gradient) and, with --large, 512^3
-particles on a 1024^3 TSC mesh.  One JSON line is printed.
+and adjoint pass at 128^3 / 256^3 (gradient) and 512^3 particles on a
+1024^3 TSC mesh (large_box).  One JSON line is printed.
 """
 import argparse
 import json
@@ -50,8 +50,8 @@ def parse():
                     help="skip the 64-box batched ensemble (graph replay) measurement")
     ap.add_argument("--no-grad", action="store_true",
                     help="skip the differentiable-model (forward + adjoint) measurement")
-    ap.add_argument("--large", action="store_true",
-                    help="also run the large box: 512^3 particles on 1024^3, TSC, FP32 vs FP64")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the large box: 512^3 particles on 1024^3, TSC, FP32 vs FP64")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -377,7 +377,7 @@ def main():
     if not args.no_grad:
         print("bench: differentiable model", file=sys.stderr, flush=True)
         out["gradient"] = run_gradient()
-    if args.large:
+    if not args.no_large:
         print("bench: large box", file=sys.stderr, flush=True)
         out["large_box"] = run_large(args)
     print(json.dumps(out))

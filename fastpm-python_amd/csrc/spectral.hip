@@ -129,6 +129,25 @@ __global__ void k_linear_scale(C<T>* f, int n0, int n1, int n2, int64_t pitch2, 
   }
 }
 
+// The same scaling with the table entry of each mode named by an index array
+// ([n0][n1][n2/2+1] int32, one entry per half-spectrum mode): boxes whose
+// sides differ, where |k| takes too many values for an n^2 table.
+template <typename T>
+__global__ void k_linear_scale_indexed(C<T>* f, int n0, int n1, int n2, int64_t pitch2,
+                                       const double* tab, const int32_t* index) {
+  const int nh = n2 / 2 + 1;
+  const int64_t total = (int64_t)n0 * n1 * nh;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(t % nh);
+    const int64_t r = t / nh;
+    const double s = tab[index[t]];
+    C<T>* p = f + r * pitch2 + k;
+    const C<T> v = *p;
+    *p = cmk<C<T>>((T)((double)v.x * s), (T)((double)v.y * s));
+  }
+}
+
 template <typename T>
 __global__ void k_lpt2_source(const T* xx, const T* yy, const T* zz, const T* xy, const T* yz,
                               const T* zx, T* out, int64_t n) {
@@ -303,6 +322,20 @@ static int linear_impl(void* f, const fpm_mesh* m, int64_t pitch2, const double*
 }
 
 template <typename T>
+static int linear_indexed_impl(void* f, const fpm_mesh* m, int64_t pitch2, const double* tab,
+                               int64_t ntab, const int32_t* index, hipStream_t st) {
+  int rc = check_cmesh(m, pitch2);
+  if (rc) return rc;
+  if (!f || !tab || !index || ntab < 1) return FPM_ERR_ARG;
+  const int n2 = m->nmesh[2];
+  const int64_t total = (int64_t)m->nmesh[0] * m->nmesh[1] * (n2 / 2 + 1);
+  hipLaunchKernelGGL(k_linear_scale_indexed<T>, dim3(grid_elems(total)), dim3(256), 0, st,
+                     static_cast<C<T>*>(f), m->nmesh[0], m->nmesh[1], n2, pitch2, tab, index);
+  FPM_LAUNCH_CHECK();
+  return FPM_OK;
+}
+
+template <typename T>
 static int lpt2_impl(const void* const* phi, void* out, int64_t n, hipStream_t st) {
   if (!phi || !out || n < 0) return FPM_ERR_ARG;
   for (int i = 0; i < 6; ++i)
@@ -338,6 +371,14 @@ int fpm_linear_scale_f32(void* f, const fpm_mesh* m, int64_t p, const double* t,
 int fpm_linear_scale_f64(void* f, const fpm_mesh* m, int64_t p, const double* t, int64_t nt,
                          fpm_stream_t s) {
   return linear_impl<double>(f, m, p, t, nt, (hipStream_t)s);
+}
+int fpm_linear_scale_indexed_f32(void* f, const fpm_mesh* m, int64_t p, const double* t,
+                                 int64_t nt, const int32_t* idx, fpm_stream_t s) {
+  return linear_indexed_impl<float>(f, m, p, t, nt, idx, (hipStream_t)s);
+}
+int fpm_linear_scale_indexed_f64(void* f, const fpm_mesh* m, int64_t p, const double* t,
+                                 int64_t nt, const int32_t* idx, fpm_stream_t s) {
+  return linear_indexed_impl<double>(f, m, p, t, nt, idx, (hipStream_t)s);
 }
 int fpm_lpt2_source_f32(const void* const* phi, void* out, int64_t n, fpm_stream_t s) {
   return lpt2_impl<float>(phi, out, n, (hipStream_t)s);

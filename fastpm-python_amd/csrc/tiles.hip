@@ -71,7 +71,7 @@ struct Workspace {
   PRec* rec;           // particle record per slot
   int32_t* count;
   int32_t* offs;       // nbins + 1
-  int32_t* misplaced;  // particles found outside their binned tile
+  int32_t* misplaced;  // particles the scatter could not place in their sub-bin
   int32_t* btot;       // per-workgroup totals of the sub-bin count scan
   int64_t bytes;
 };
@@ -474,7 +474,15 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_scatter(PartDesc<T> P, Tile
         int i0[3];
         T t[3];
         const int64_t p = bin_window<T, ORDER>(o, P, g.nm, i, i0, t);
-        slot[u] = w.offs[bin_of<ORDER>(i0, g)] + rank;
+        const int32_t b = bin_of<ORDER>(i0, g);
+        const int32_t lo = w.offs[b];
+        // a slot outside the sub-bin's range from the count would leave some
+        // other slot unwritten: count it instead of storing it
+        if (rank >= 0 && rank < w.offs[b + 1] - lo && lo + rank < P.n) {
+          slot[u] = lo + rank;
+        } else {
+          atomicAdd(w.misplaced, 1);
+        }
         pp[u] = (int32_t)p;
         r[u].t[0] = (float)t[0];
         r[u].t[1] = (float)t[1];
@@ -484,11 +492,11 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_scatter(PartDesc<T> P, Tile
       }
     }
     // the count placed each particle with the same arithmetic (no FMA
-    // contraction in particle_cell), so its slot is in its sub-bin; the guard
-    // keeps the stores in bounds regardless
+    // contraction in particle_cell), so its slot is in its sub-bin; the range
+    // check above counts any that is not, and stores stay in bounds
 #pragma unroll
     for (int u = 0; u < kBinILP; ++u) {
-      if (slot[u] >= 0 && slot[u] < P.n) {
+      if (slot[u] >= 0) {
         w.list[slot[u]] = pp[u];
         w.rec[slot[u]] = r[u];
       }

@@ -70,6 +70,7 @@ _TYPED = {
     "fpm_kspace_apply": (_int, [_vp, _PK, _PM, _i64, _vp]),
     "fpm_whitenoise": (_int, [_vp, _PM, _i64, ctypes.c_uint64, _int, _vp]),
     "fpm_linear_scale": (_int, [_vp, _PM, _i64, _vp, _i64, _vp]),
+    "fpm_linear_scale_indexed": (_int, [_vp, _PM, _i64, _vp, _i64, _vp, _vp]),
     "fpm_lpt2_source": (_int, [_vp, _vp, _i64, _vp]),
     "fpm_axpy": (_int, [_vp, _vp, _i64, _dbl, _vp]),
     "fpm_lpt_combine": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),

@@ -172,28 +172,36 @@ class Solver:
         return self.pm.generate_whitenoise(seed, type="complex", unitary=unitary)
 
     def linear(self, whitenoise, Pk):
-        """delta_k = sqrt(P(|k|)/V) w_k (core.py:116-118).  P is evaluated on the
-        host once per distinct integer |n|^2 and applied by table lookup."""
+        """delta_k = sqrt(P(|k|)/V) w_k (core.py:116-118).  P, a host callable,
+        is evaluated once per distinct |k| of the mesh and applied by table
+        lookup: on a cubic box the table is indexed by the integer |n|^2; when
+        the sides differ the distinct k^2 are found on the device (sorted,
+        with each mode's index into them)."""
         pm = whitenoise.pm
         box = pm.BoxSize
-        if not np.allclose(box, box[0], rtol=0, atol=0):
-            raise NotImplementedError("linear() needs an isotropic box")
-        nmax = int(sum((int(n) // 2) ** 2 for n in pm.Nmesh))
-        n2 = np.arange(nmax + 1, dtype="f8")
-        k = 2 * np.pi * np.sqrt(n2) / box[0]
-        try:
-            pk = np.asarray(Pk(k), dtype="f8")
-        except Exception:
-            pk = np.array([float(Pk(x)) for x in k])
-        with np.errstate(invalid="ignore"):
-            tab = np.sqrt(pk / float(np.prod(box)))
-        tab[0] = 0.0
-        tab = np.nan_to_num(tab, nan=0.0, posinf=0.0)
-        tab_d = torch.from_numpy(tab).to(pm.device)
         out = whitenoise.copy()
-        _lib.call("fpm_linear_scale", pm.dtype, out._t.data_ptr(), pm._desc, pm.pitch2,
-                  tab_d.data_ptr(), len(tab), _stream())
-        out._table = tab_d  # the table lives as long as the field
+        if np.all(box == box[0]):
+            nmax = int(sum((int(n) // 2) ** 2 for n in pm.Nmesh))
+            k = 2 * np.pi * np.sqrt(np.arange(nmax + 1, dtype="f8")) / box[0]
+            tab_d = torch.from_numpy(_linear_table(Pk, k, box)).to(pm.device)
+            _lib.call("fpm_linear_scale", pm.dtype, out._t.data_ptr(), pm._desc, pm.pitch2,
+                      tab_d.data_ptr(), len(tab_d), _stream())
+            out._table = tab_d  # the table lives as long as the field
+            return out
+        kk = None
+        for d, k1 in enumerate(pm.k_axes()):
+            shape = [1, 1, 1]
+            shape[d] = len(k1)
+            t = torch.from_numpy(k1 ** 2).to(pm.device).reshape(shape)
+            kk = t if kk is None else kk + t
+        uniq, index = torch.unique(kk.reshape(-1), sorted=True, return_inverse=True)
+        del kk
+        k = np.sqrt(uniq.cpu().numpy())
+        tab_d = torch.from_numpy(_linear_table(Pk, k, box)).to(pm.device)
+        index = index.to(torch.int32).contiguous()
+        _lib.call("fpm_linear_scale_indexed", pm.dtype, out._t.data_ptr(), pm._desc, pm.pitch2,
+                  tab_d.data_ptr(), len(tab_d), index.data_ptr(), _stream())
+        out._table = (tab_d, index)
         return out
 
     def lpt(self, linear, Q, a, order=2):
@@ -226,6 +234,19 @@ class Solver:
         """Apply an action list; mutates and returns ``state`` (core.py:147-152)."""
         self.nbodystep.run_actions(stepping, state, monitor)
         return state
+
+
+def _linear_table(Pk, k, box):
+    """sqrt(P(k) / V) at the wave numbers k (0 at k = 0 and where P is not
+    finite)."""
+    try:
+        pk = np.asarray(Pk(k), dtype="f8")
+    except Exception:
+        pk = np.array([float(Pk(x)) for x in k])
+    with np.errstate(invalid="ignore"):
+        tab = np.sqrt(pk / float(np.prod(box)))
+    tab[k == 0] = 0.0
+    return np.ascontiguousarray(np.nan_to_num(tab, nan=0.0, posinf=0.0))
 
 
 class FastPMStep:
@@ -327,9 +348,9 @@ class FastPMStep:
 
     def workspace(self):
         """Force-mesh buffers, allocated once: the stencil path needs the
-        density, one real and one complex mesh (the spectrum buffer only when
-        a monitor receives delta_k, see delta_k_buffer); the k-space path
-        three of each and the spectrum."""
+        density, one real and one complex mesh; the k-space path three of
+        each and the spectrum (reused by forces whose delta_k no monitor
+        receives; a monitor gets a new field per force, see force_kernels)."""
         if self._ws is None:
             pm = self.pm
             n = 1 if self.stencil else 3
@@ -394,9 +415,11 @@ class FastPMStep:
 
     def force_kernels(self, state, keep_delta_k=True):
         """Enqueues the force computation (no host-side bookkeeping, so it can
-        be captured in a graph); returns the delta_k field, or None on the
-        stencil path when keep_delta_k is false (the spectrum is then not
-        stored)."""
+        be captured in a graph).  With keep_delta_k the spectrum goes to a new
+        ComplexField, returned, which a monitor may keep (the reference makes
+        a new field per Force, core.py:205); otherwise it is not stored on the
+        stencil path (None is returned) and goes to the reused workspace
+        buffer on the k-space path."""
         ws = self.workspace()
         layout, X1, rho = self.prepare_force(state, smoothing=None)
         F = state.F
@@ -404,14 +427,15 @@ class FastPMStep:
             # delta_k = r2c(rho) and phi = c2r(common * delta_k), sharing the x pass
             phi = ws["force"][0]
             with self._mark("fft"):
-                kout = self.delta_k_buffer() if keep_delta_k else False
+                kout = ComplexField(self.pm) if keep_delta_k else False
                 delta_k, _ = r2c_c2r_many(rho, [self.force_common()], kout=kout,
                                           outs=[phi], work=ws["work"][:1])
             with self._mark("readout"):
                 self.pm.readout_tiled_grad4(phi, X1, F, rho=rho, rho_out=state.RHO)
             return delta_k
         with self._mark("r2c"):
-            delta_k = rho.r2c(out=self.delta_k_buffer())
+            delta_k = rho.r2c(out=ComplexField(self.pm) if keep_delta_k
+                              else self.delta_k_buffer())
         ops = self.force_ops()
         if self.solver.fused:
             with self._mark("c2r"):

@@ -100,6 +100,6 @@ class GlassStep(core.FastPMStep):
         return [laplace * gradient(d, order=1) * -1.0 for d in range(3)]
 
     def Force(self, state, ai, ac, af):
-        self.force_kernels(state)
+        self.force_kernels(state, keep_delta_k=False)
         state.a["F"] = af
 

@@ -178,6 +178,17 @@ class ParticleMesh:
     def tdtype(self):
         return _TORCH[self.dtype]
 
+    def k_axes(self):
+        """Wave numbers of the half-spectrum axes in h/Mpc as float64 numpy
+        arrays: 2 pi fftfreq(n_d) n_d / L_d along x and y, 2 pi m / L_z with
+        m = 0 .. n_z/2 along z."""
+        out = []
+        for d in range(3):
+            n = int(self.Nmesh[d])
+            m = np.arange(n // 2 + 1) if d == 2 else np.fft.fftfreq(n, 1.0 / n)
+            out.append(2 * np.pi * m / float(self.BoxSize[d]))
+        return out
+
     def twiddles(self):
         if self._tw is None:
             self._tw = [torch.from_numpy(_lib.twiddles(int(n), self.dtype)).to(self.device)
@@ -265,8 +276,9 @@ class ParticleMesh:
         return ws
 
     def tile_misplaced(self, n, order):
-        """Particles that paint/readout found outside their binned tile since
-        the last bin call (synchronises)."""
+        """Particles the last bin call's scatter could not store in their
+        sub-bin (0 unless the count and scatter passes disagreed; then the
+        tile lists are incomplete).  Synchronises."""
         off = int(_lib.lib().fpm_tile_misplaced_offset(self._desc, int(order), int(n)))
         ws = self._tile_ws
         return int(ws[off:off + 4].view(torch.int32).item())
@@ -450,23 +462,43 @@ class ComplexField:
     def copy(self):
         return ComplexField(self.pm, self._t.clone())
 
-    def apply(self, kernel, out=None, kind=None):
-        """Multiply by one of the k-space kernels of ``fastpm.force.kernels``
-        (evaluated on the device).  ``out=Ellipsis`` applies in place."""
+    def apply(self, kernel, out=None, kind="wavenumber"):
+        """pmesh ComplexField.apply (fastpm/core.py:116-118,
+        fastpm/operators.py:5-22, fastpm/force/gravity.py:17-22):
+        ``out=Ellipsis`` applies in place, ``out=None`` returns a new field.
+
+        ``kernel`` is either an operator of ``fastpm.force.kernels`` (one
+        fused multiplier kernel in libfpm) or a callable ``f(k, v)`` as in
+        the reference.  The callable receives, on the device, ``k``: three
+        broadcastable wave-number tensors of the field's real dtype (h/Mpc;
+        fftfreq order along x and y, 0 .. n/2 along z; with kind='circular'
+        k h in [-pi, pi), with kind='index' the integer wave numbers), and
+        ``v``: the half-spectrum values, carrying ``v.i`` (broadcastable
+        integer indices 0 .. n-1), ``v.Nmesh`` and ``v.BoxSize``; it returns
+        the new values.  Arithmetic, comparisons, indexing and numpy ufuncs
+        (numpy.sin, numpy.exp, ...) act on these tensors on the device, and
+        ``nonzero()`` returns a tuple of index tensors as numpy's does."""
         from .force.kernels import KernelOp
 
-        if not isinstance(kernel, KernelOp):
-            raise TypeError("apply() takes a fastpm.force.kernels operator on the device path, "
-                            "got %r" % (kernel,))
         if out is Ellipsis:
             dst = self
         else:
             dst = self.copy() if out is None else out
             if out is not None:
                 out._t.copy_(self._t)
-        op = kernel.fpm_kop()
-        _lib.call("fpm_kspace_apply", self.pm.dtype, dst._t.data_ptr(), op, self.pm._desc,
-                  self.pm.pitch2, _stream())
+        if isinstance(kernel, KernelOp):
+            op = kernel.fpm_kop()
+            _lib.call("fpm_kspace_apply", self.pm.dtype, dst._t.data_ptr(), op, self.pm._desc,
+                      self.pm.pitch2, _stream())
+            return dst
+        if not callable(kernel):
+            raise TypeError("apply() takes a fastpm.force.kernels operator or a callable "
+                            "f(k, v), got %r" % (kernel,))
+        k, v = _mode_arguments(self.pm, dst.value, kind)
+        res = kernel(k, v)
+        if not isinstance(res, torch.Tensor):
+            raise TypeError("apply(): the callable must return a tensor, got %r" % type(res))
+        dst.value.copy_(res.as_subclass(torch.Tensor).broadcast_to(dst.value.shape))
         return dst
 
     def c2r(self, out=None, ops=None):
@@ -474,6 +506,75 @@ class ComplexField:
         k-space operators into the first pass, giving one output per operator."""
         return c2r_many(self, [None] if ops is None else ops, None if out in (None, Ellipsis)
                         else [out])[0]
+
+
+# numpy ufuncs on apply() arguments, by the torch function that computes them
+_UFUNCS = {
+    "add": torch.add, "subtract": torch.sub, "multiply": torch.mul, "divide": torch.div,
+    "true_divide": torch.div, "power": torch.pow, "negative": torch.neg, "absolute": torch.abs,
+    "sqrt": torch.sqrt, "square": torch.square, "exp": torch.exp, "expm1": torch.expm1,
+    "log": torch.log, "log1p": torch.log1p, "log10": torch.log10, "sin": torch.sin,
+    "cos": torch.cos, "tan": torch.tan, "arcsin": torch.asin, "arccos": torch.acos,
+    "arctan": torch.atan, "arctan2": torch.atan2, "sinh": torch.sinh, "cosh": torch.cosh,
+    "tanh": torch.tanh, "conjugate": torch.conj_physical, "sign": torch.sgn,
+    "floor": torch.floor, "ceil": torch.ceil, "rint": torch.round, "maximum": torch.maximum,
+    "minimum": torch.minimum, "equal": torch.eq, "not_equal": torch.ne, "less": torch.lt,
+    "less_equal": torch.le, "greater": torch.gt, "greater_equal": torch.ge,
+    "logical_and": torch.logical_and, "logical_or": torch.logical_or,
+    "logical_not": torch.logical_not, "isfinite": torch.isfinite, "isnan": torch.isnan,
+}
+
+
+class ModeTensor(torch.Tensor):
+    """A device tensor as handed to ``ComplexField.apply`` callables: a numpy
+    ufunc applied to it runs as the torch function of the same name on the
+    device (its result is again a ModeTensor), and ``nonzero()`` returns a
+    tuple of index tensors, as numpy's does, so kernels written against numpy
+    arrays (fastpm/operators.py:5-22, fastpm/force/kernels.py:3-35) run
+    unchanged.  Functions that are not ufuncs (numpy.interp, numpy.sinc) are
+    not translated."""
+
+    def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
+        fn = _UFUNCS.get(ufunc.__name__)
+        if method != "__call__" or fn is None or kwargs:
+            raise TypeError("numpy.%s on device mode arrays is not supported in apply()"
+                            % ufunc.__name__)
+        args = [x.as_subclass(torch.Tensor) if isinstance(x, torch.Tensor) else x
+                for x in inputs]
+        if not any(isinstance(x, torch.Tensor) for x in args[:1]):
+            # a scalar first operand: torch functions want a tensor there
+            t = next(x for x in args if isinstance(x, torch.Tensor))
+            args[0] = torch.as_tensor(args[0], device=t.device)
+        return fn(*args).as_subclass(ModeTensor)
+
+    def nonzero(self, *args, **kwargs):
+        return torch.Tensor.nonzero(self.as_subclass(torch.Tensor), as_tuple=True)
+
+
+def _mode_arguments(pm, values, kind):
+    """(k, v) of an apply() callable for the half-spectrum view `values`."""
+    if kind not in ("wavenumber", "circular", "index"):
+        raise ValueError("kind must be 'wavenumber', 'circular' or 'index', got %r" % (kind,))
+    rdt = pm.tdtype
+    k, idx = [], []
+    for d, kd in enumerate(pm.k_axes()):
+        n = int(pm.Nmesh[d])
+        shape = [1, 1, 1]
+        shape[d] = len(kd)
+        if kind == "circular":
+            kd = kd * float(pm.BoxSize[d]) / n
+        elif kind == "index":
+            kd = np.rint(kd * float(pm.BoxSize[d]) / (2 * np.pi)).astype("i8")
+        t = torch.from_numpy(np.ascontiguousarray(kd)).to(pm.device)
+        if kind != "index":
+            t = t.to(rdt)
+        k.append(t.reshape(shape).as_subclass(ModeTensor))
+        idx.append(torch.arange(len(kd), device=pm.device).reshape(shape).as_subclass(ModeTensor))
+    v = values.as_subclass(ModeTensor)
+    v.i = idx
+    v.Nmesh = np.array(pm.Nmesh)
+    v.BoxSize = np.array(pm.BoxSize)
+    return k, v
 
 
 def c2r_grad3(cfield, common, outs=None, work=None, grad_order=1):

@@ -133,8 +133,10 @@ int fpm_readout_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, 
  * clearing pass.  Same contracts as fpm_paint / fpm_readout
  * (fastpm/core.py:191,203; fastpm/force/gravity.py:24).
  * fpm_tile_misplaced_offset gives the byte offset in the workspace of an int32
- * counter of particles found outside their binned tile (zeroed by each bin
- * call; the record-based kernels cannot misplace a particle, so it stays 0). */
+ * counter, zeroed by each bin call, of particles the scatter could not store
+ * in their sub-bin (a slot outside the range the count pass gave that
+ * sub-bin); nonzero means the lists are incomplete and the tiled kernels
+ * must not be used with them.  0 is the only valid outcome. */
 int fpm_tiles_supported(const fpm_mesh* mesh);
 int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n);
 int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n);
@@ -267,6 +269,18 @@ int fpm_linear_scale_f32(void* field, const fpm_mesh* mesh, int64_t pitch2, cons
                          int64_t ntable, fpm_stream_t stream);
 int fpm_linear_scale_f64(void* field, const fpm_mesh* mesh, int64_t pitch2, const double* table,
                          int64_t ntable, fpm_stream_t stream);
+
+/* delta_k *= table[index[mode]] with index an int32 array of one entry per
+ * half-spectrum mode ([n0][n1][n2/2+1], C order; every entry in [0, ntable)
+ * is the caller's guarantee) and table holding sqrt(P(k)/V): Solver.linear
+ * (fastpm/core.py:116-118) for a box whose sides differ, where the caller
+ * tabulates P at the distinct |k| of the mesh. */
+int fpm_linear_scale_indexed_f32(void* field, const fpm_mesh* mesh, int64_t pitch2,
+                                 const double* table, int64_t ntable, const int32_t* index,
+                                 fpm_stream_t stream);
+int fpm_linear_scale_indexed_f64(void* field, const fpm_mesh* mesh, int64_t pitch2,
+                                 const double* table, int64_t ntable, const int32_t* index,
+                                 fpm_stream_t stream);
 
 /* 2LPT source in real space from the six second derivatives
  * phi = {xx, yy, zz, xy, yz, zx}:

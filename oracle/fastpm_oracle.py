@@ -33,6 +33,9 @@ its call sites rely on, and each function says how it is pinned:
   leapfrog/autostages core.py:212-317                         counts of test_core.py:11-30
   kick/drift/force   core.py:168-210, force/gravity.py:10-29  JANUS reversibility (examples/janus.py),
                                                               kick identity, plane-wave force
+  longrange          force/gravity.py:5-29                    plane-wave closed form with split and
+                                                              compensation; equals Solver.force at
+                                                              split 0
   power              utils.py:77-164 (estimator semantics)    brute-force binning loops
 
 Deliberate, documented choices where the reference is silent or unreachable:
@@ -318,6 +321,26 @@ def lpt2source(dlin_k, nmesh, box):
 
 
 # ----------------------------------------------------------------------------
+# long-range force (force/gravity.py:5-29)
+def longrange(x_list, delta_k, nmesh, box, split, factor, order=2, deconv=0):
+    """longrange_batch (gravity.py:11-29): pot = delta_k / k^2 (0 at k = 0)
+    * exp(-k^2 split^2) [/ W(k)^deconv, this build's compensation]; for each
+    axis d, c2r(i a_d(k) pot) with the four-point a_d, read out at every
+    position set; every force multiplied by factor.  One [N, 3] array per set
+    (longrange(x, ...) of gravity.py:5-8 is the one-set case)."""
+    pot = (delta_k * k_laplace(nmesh, box) * k_longrange(nmesh, box, split)
+           * k_deconv(nmesh, box, deconv))
+    out = [np.empty((len(x), 3)) for x in x_list]
+    for d in range(3):
+        fd = c2r(pot * k_gradient(d, nmesh, box, 1), nmesh)
+        for x, o in zip(x_list, out):
+            o[:, d] = readout(fd, x, box, order)
+    for o in out:
+        o *= factor
+    return out
+
+
+# ----------------------------------------------------------------------------
 # growth (background.py:4-256), restated with odeint exactly as the reference
 class Growth:
     """Matter-dominated growth: D1, D2 and derivatives in ln a at grid points
@@ -487,13 +510,15 @@ class State:
 
 
 class Solver:
-    """FP64 restatement of fastpm.core.Solver for a cubic particle mesh of
-    side ``nc`` and force mesh ``nc * B``."""
+    """FP64 restatement of fastpm.core.Solver for a particle mesh of ``nc``
+    cells per side (an int, or three sides) and force mesh ``nc * B``, in a
+    box of side ``box`` (a float, or three sides)."""
 
     def __init__(self, nc, box, Om0, B=1, a_linear=1.0, resampler="cic", deconv=0):
-        self.nmesh = (nc, nc, nc)
-        self.fnmesh = (nc * B, nc * B, nc * B)
-        self.box = (float(box),) * 3
+        nc = tuple(int(n) for n in np.broadcast_to(nc, (3,)))
+        self.nmesh = nc
+        self.fnmesh = tuple(n * B for n in nc)
+        self.box = tuple(float(b) for b in np.broadcast_to(box, (3,)))
         self.Om0 = Om0
         self.a_linear = a_linear
         self.order = {"cic": 2, "tsc": 3}[resampler]

@@ -243,3 +243,32 @@ def test_monitor_sees_delta_k_and_unmonitored_run_matches(gpu):
         assert torch.equal(getattr(a, key), getattr(b, key)), key
     assert len(seen) == len(stages)
     assert all(abs(z - 1) < 1e-5 for z in seen), seen
+
+
+@pytest.mark.parametrize("dtype", ["f4", "f8"])
+def test_monitor_may_keep_delta_k(gpu, dtype):
+    """A monitor that keeps the event objects past the callback still holds
+    each Force's own delta_k after the run: every Force makes a new field
+    (core.py:205), none is overwritten by a later Force.  Float32 takes the
+    stencil path, float64 the k-space path."""
+    from fastpm.core import leapfrog
+
+    solver, _, Pk, pm = _setup(16, 100.0, 2, dtype)
+    assert solver.nbodystep.stencil == (dtype == "f4")
+    st = solver.lpt(solver.linear(solver.whitenoise(8), Pk), pm.generate_uniform_particle_grid(),
+                    0.1)
+    kept, copies = [], []
+
+    def monitor(action, ai, ac, af, state, event):
+        if action == "F":
+            kept.append(event)
+            copies.append(event["delta_k"].value.clone())
+
+    solver.nbody(st, leapfrog([0.1, 0.4, 0.7, 1.0]), monitor=monitor)
+    assert len(kept) == 4
+    fields = [e["delta_k"] for e in kept]
+    assert len({id(f._t) for f in fields}) == 4
+    for f, c in zip(fields, copies):
+        assert torch.equal(f.value, c)
+    for i in range(3):
+        assert not torch.equal(copies[i], copies[i + 1])

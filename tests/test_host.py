@@ -234,3 +234,66 @@ def test_reverse_pass_fuses_drift_and_kicks():
         ["axpy"] * 2 + ["F"]
     for a, b in zip(out[True], out[False]):
         assert torch.equal(a, b)
+
+
+def test_mode_tensor_runs_numpy_style_kernels():
+    """apply() callables written against numpy arrays (the reference's
+    laplace_kernel and diff_kernel, operators.py:5-22, and the order-1
+    gradient of force/kernels.py:17-24) give numpy's results on ModeTensor
+    arguments (CPU tensors here; the device path is the same code)."""
+    import torch
+    from fastpm.pm import ModeTensor
+
+    def laplace_kernel(k, v):
+        kk = sum(ki ** 2 for ki in k)
+        mask = (kk == 0).nonzero()
+        kk[mask] = 1
+        b = v / kk
+        b[mask] = 0
+        return b
+
+    def diff_kernel(dir, k, v):
+        mask = (v.i[dir] != v.Nmesh[dir] // 2)
+        return v * (1j * k[dir] * mask)
+
+    def grad4(dir, k, v):
+        cellsize = v.BoxSize[dir] / v.Nmesh[dir]
+        w = k[dir] * cellsize
+        a = 1 / (6.0 * cellsize) * (8 * np.sin(w) - np.sin(2 * w))
+        return v * (1j * a)
+
+    rng = np.random.default_rng(0)
+    n, L = (8, 6, 4), np.array([10.0, 12.0, 8.0])
+    kn = [2 * np.pi * np.fft.fftfreq(n[0], 1.0 / n[0]) / L[0],
+          2 * np.pi * np.fft.fftfreq(n[1], 1.0 / n[1]) / L[1],
+          2 * np.pi * np.arange(n[2] // 2 + 1) / L[2]]
+    kb = [kn[0][:, None, None], kn[1][None, :, None], kn[2][None, None, :]]
+    vals = rng.standard_normal((8, 6, 3)) + 1j * rng.standard_normal((8, 6, 3))
+
+    class V(np.ndarray):
+        pass
+
+    def numpy_args():
+        v = vals.copy().view(V)
+        v.i = [np.arange(8)[:, None, None], np.arange(6)[None, :, None],
+               np.arange(3)[None, None, :]]
+        v.Nmesh, v.BoxSize = np.array(n), L
+        return [x.copy() for x in kb], v
+
+    def torch_args():
+        v = torch.from_numpy(vals.copy()).as_subclass(ModeTensor)
+        v.i = [torch.from_numpy(x).as_subclass(ModeTensor) for x in numpy_args()[1].i]
+        v.Nmesh, v.BoxSize = np.array(n), L
+        return [torch.from_numpy(x.copy()).as_subclass(ModeTensor) for x in kb], v
+
+    cases = [laplace_kernel, lambda k, v: diff_kernel(0, k, v),
+             lambda k, v: diff_kernel(1, k, v), lambda k, v: grad4(2, k, v),
+             lambda k, v: v * np.exp(-sum(ki ** 2 for ki in k) * 0.3)]
+    for f in cases:
+        want = np.asarray(f(*numpy_args()))
+        got = f(*torch_args())
+        assert isinstance(got, ModeTensor)
+        np.testing.assert_allclose(got.as_subclass(torch.Tensor).numpy(), want, rtol=1e-14,
+                                   atol=1e-14)
+    with pytest.raises(TypeError):
+        np.add.reduce(torch_args()[0][0])

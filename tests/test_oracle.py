@@ -466,3 +466,44 @@ def test_glass_oracle():
     assert len(st) == 27 and st[0] == 0 and abs(st[-1] - 6.5 * np.pi) < 1e-12
     X0 = O.glass(8, 128.0, 123, spread=0.0)
     assert np.array_equal(X0, O.lattice((8,) * 3, (128.0,) * 3) % 128.0)
+
+
+@pytest.mark.parametrize("split,deconv", [(0.0, 0), (3.0, 0), (2.5, 2)])
+def test_longrange_plane_wave(split, deconv):
+    """gravity.py:11-29 on a density cosine A cos(k0 x): pot = delta_k / k^2
+    (no sign), so the x force on the mesh is
+    -factor a(k0) A / k0^2 sin(k0 x) exp(-k0^2 r^2) / sinc(k0 h / 2)^deconv
+    with the four-point a(k0) = (8 sin k0 h - sin 2 k0 h) / (6 h), and zero
+    along y and z.  Read out at grid points (exact) and half a cell along x
+    (CIC: the mean of the two neighbours), as two position sets."""
+    N, L, factor = 16, 64.0, 0.45
+    h = L / N
+    k0 = 2 * np.pi * 3 / L
+    x = np.arange(N) * h
+    A = 0.3
+    rho = A * np.cos(k0 * x)[:, None, None] * np.ones((N, N, N))
+    dk = O.r2c(rho)
+    a = (8 * np.sin(k0 * h) - np.sin(2 * k0 * h)) / (6 * h)
+    amp = -factor * a * A / k0 ** 2 * np.exp(-k0 ** 2 * split ** 2)
+    if deconv:
+        amp /= (np.sin(k0 * h / 2) / (k0 * h / 2)) ** deconv
+    q0 = O.lattice((N,) * 3, (L,) * 3)
+    q1 = q0 + np.array([0.5 * h, 0.25 * h, 0.0])
+    f0, f1 = O.longrange([q0, q1], dk, (N,) * 3, (L,) * 3, split, factor, 2, deconv)
+    np.testing.assert_allclose(f0[:, 0], amp * np.sin(k0 * q0[:, 0]), atol=1e-13)
+    mid = 0.5 * (np.sin(k0 * q0[:, 0]) + np.sin(k0 * (q0[:, 0] + h)))
+    np.testing.assert_allclose(f1[:, 0], amp * mid, atol=1e-13)
+    assert np.abs(f0[:, 1:]).max() < 1e-13 and np.abs(f1[:, 1:]).max() < 1e-13
+
+
+def test_longrange_is_solver_force():
+    """At split 0 with factor 3/2 Om0 and the painted 1 + delta, longrange is
+    the Solver's Force (core.py:195-210)."""
+    s = O.Solver(8, 64.0, 0.3, B=2)
+    st = s.lpt(s.linear(s.whitenoise(5), _pk), O.lattice((8,) * 3, (64.0,) * 3), 0.5)
+    X = st.X
+    nm = (16,) * 3
+    rho = O.paint(X, nm, (64.0,) * 3, 2) / (len(X) / 16 ** 3)
+    f = O.longrange([X], O.r2c(rho), nm, (64.0,) * 3, 0.0, 1.5 * 0.3)[0]
+    s.force(st, 0.5, 0.5, 0.5)
+    np.testing.assert_allclose(f, st.F, rtol=1e-12, atol=1e-14 * np.abs(st.F).max())

@@ -165,6 +165,7 @@ extern "C" const char* fpm_strerror(int status) {
     case FPM_ERR_UNSUPPORTED: return "unsupported size (FFT length or mesh side)";
     case FPM_ERR_HIP: return "HIP launch failed";
     case FPM_ERR_WORKSPACE: return "workspace too small";
+    case FPM_ERR_SCRATCH: return "request exceeds the fixed scratch of an allocation-free call";
     default: return "unknown status";
   }
 }

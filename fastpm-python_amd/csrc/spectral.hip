@@ -1,5 +1,7 @@
 // Spectral-side kernels: white noise, linear-power scaling, the 2LPT source
 // product, and double-precision shell-binned spectra.
+#include <mutex>
+
 #include "fpm_common.h"
 
 namespace fpm {
@@ -289,6 +291,50 @@ static int power_impl(const void* f1, const void* f2, const fpm_mesh* m, int64_t
   return FPM_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free P(k)
+// fpm_power_bin_noalloc keeps the block partials in this array instead of a
+// caller workspace: 2048 blocks x 128 bins x 5 sums (10 MiB), held by every
+// device the library runs on until the process ends.  Host code never passes
+// the array's name to a kernel (on the host it is only a stand-in for the
+// symbol): each device's address comes from hipGetSymbolAddress, queried once
+// per device with that device current, on the entry point's first call there.
+constexpr int64_t kPowerScratchDoubles = (int64_t)2048 * 128 * kPowerSums;
+static __device__ __align__(256) double scratch[kPowerScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices] = {};
+
+static int scratch_address(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return FPM_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FPM_ERR_HIP;
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return FPM_OK;
+}
+
+// The same kernels as power_impl, with the partials in `scratch`.  Every
+// partial the final sum reads is written by the partial pass of the same
+// call, so nothing needs clearing; calls that overlap on one device share the
+// array and race.
+template <typename T>
+static int power_noalloc_impl(const void* f1, const void* f2, const fpm_mesh* m, int64_t pitch2,
+                              int nbins, double kmin, double dk, double* out, hipStream_t st) {
+  int rc = check_cmesh(m, pitch2);
+  if (rc) return rc;
+  if (nbins < 1) return FPM_ERR_ARG;
+  if ((int64_t)power_blocks(m) * nbins * kPowerSums > kPowerScratchDoubles) return FPM_ERR_SCRATCH;
+  double* work = nullptr;
+  rc = scratch_address(&work);
+  if (rc) return rc;
+  return power_impl<T>(f1, f2, m, pitch2, nbins, kmin, dk, work, kPowerScratchDoubles, out, st);
+}
+
 template <typename T>
 static int whitenoise_impl(void* out, const fpm_mesh* m, int64_t pitch2, uint64_t seed,
                            int unitary, hipStream_t st) {
@@ -397,6 +443,14 @@ int fpm_power_bin_f32(const void* f1, const void* f2, const fpm_mesh* m, int64_t
 int fpm_power_bin_f64(const void* f1, const void* f2, const fpm_mesh* m, int64_t p, int nbins,
                       double kmin, double dk, double* w, int64_t wl, double* out, fpm_stream_t s) {
   return power_impl<double>(f1, f2, m, p, nbins, kmin, dk, w, wl, out, (hipStream_t)s);
+}
+int fpm_power_bin_noalloc_f32(const void* f1, const void* f2, const fpm_mesh* m, int64_t p,
+                              int nbins, double kmin, double dk, double* out, fpm_stream_t s) {
+  return power_noalloc_impl<float>(f1, f2, m, p, nbins, kmin, dk, out, (hipStream_t)s);
+}
+int fpm_power_bin_noalloc_f64(const void* f1, const void* f2, const fpm_mesh* m, int64_t p,
+                              int nbins, double kmin, double dk, double* out, fpm_stream_t s) {
+  return power_noalloc_impl<double>(f1, f2, m, p, nbins, kmin, dk, out, (hipStream_t)s);
 }
 
 }  // extern "C"

@@ -76,6 +76,7 @@ _TYPED = {
     "fpm_lpt_combine": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "fpm_positions": (_int, [_PP, _PM, _vp, _vp]),
     "fpm_power_bin": (_int, [_vp, _vp, _PM, _i64, _int, _dbl, _dbl, _vp, _i64, _vp, _vp]),
+    "fpm_power_bin_noalloc": (_int, [_vp, _vp, _PM, _i64, _int, _dbl, _dbl, _vp, _vp]),
     "fpm_twiddles": (_int, [_int, _vp]),
     "fpm_paint_weighted": (_int, [_PP, _PM, _int, _vp, _i64, _dbl, _vp, _int, _vp]),
     "fpm_window_grad": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _dbl, _vp, _int, _vp]),

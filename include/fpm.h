@@ -38,6 +38,7 @@ extern "C" {
 #define FPM_ERR_UNSUPPORTED (-2)  /* FFT length without a compiled plan */
 #define FPM_ERR_HIP (-3)          /* a HIP launch failed */
 #define FPM_ERR_WORKSPACE (-4)    /* workspace too small */
+#define FPM_ERR_SCRATCH (-5)      /* beyond the fixed scratch of an allocation-free call */
 
 typedef struct hipStream_t_opaque* fpm_stream_t; /* ABI-compatible with hipStream_t */
 
@@ -387,6 +388,25 @@ int fpm_power_bin_f32(const void* f1, const void* f2, const fpm_mesh* mesh, int6
 int fpm_power_bin_f64(const void* f1, const void* f2, const fpm_mesh* mesh, int64_t pitch2,
                       int nbins, double kmin, double dk, double* work, int64_t work_len,
                       double* out, fpm_stream_t stream);
+
+/* fpm_power_bin without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving or
+ * monitoring loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The block partials live in a fixed device array of the library
+ * (10 MiB, 1310720 doubles, held on every device the library runs on until
+ * the process ends); a call needs fpm_power_work_len(mesh, nbins) doubles of
+ * it, so up to 128 bins on any mesh (more on meshes below 2048 x 8192 modes),
+ * and returns FPM_ERR_SCRATCH beyond that.  Same results as fpm_power_bin,
+ * bit for bit.  It allocates nothing; its first call on a device looks the
+ * array's address up (no allocation).  The stream must belong to the current
+ * device.  All calls on one device share the array: calls that overlap on a
+ * device (other streams or threads) race. */
+int fpm_power_bin_noalloc_f32(const void* f1, const void* f2, const fpm_mesh* mesh,
+                              int64_t pitch2, int nbins, double kmin, double dk, double* out,
+                              fpm_stream_t stream);
+int fpm_power_bin_noalloc_f64(const void* f1, const void* f2, const fpm_mesh* mesh,
+                              int64_t pitch2, int nbins, double kmin, double dk, double* out,
+                              fpm_stream_t stream);
 
 /* ---- Reverse-mode (adjoint) kernels.  They replace the pmesh gradient
  * operators the reference's hand-written adjoints call

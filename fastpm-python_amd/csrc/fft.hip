@@ -230,6 +230,13 @@ static int col_wide(int elem_bytes, int n) {
   return (n >= 0 && n <= 1024) ? g_col_wide[elem_bytes == 8 ? 1 : 0][n] : 0;
 }
 
+// The tiling of a strided pass over an axis of length n of mesh m: the mesh's
+// FPM_VAR_FFT_COLS field, or the process-wide table when that is 0.
+static int col_choice(const fpm_mesh* m, int elem_bytes, int n) {
+  const int v = FPM_VAR_FIELD(m->variant, FPM_VAR_FFT_COLS);
+  return v == 1 ? 0 : (v == 2 ? 1 : col_wide(elem_bytes, n));
+}
+
 #ifndef FPM_Z_ELEMS
 #define FPM_Z_ELEMS 2048  // complex elements per z-pass workgroup
 #endif
@@ -237,10 +244,12 @@ static int col_wide(int elem_bytes, int n) {
 #define FPM_Z_EPT 8  // complex elements per thread in the z pass
 #endif
 
-// Lines per workgroup for the z pass: about FPM_Z_ELEMS complex elements.
-constexpr int z_lines(int M) {
+// Lines per workgroup for the z pass: about FPM_Z_ELEMS complex elements
+// (times 2^ZV / 2 for the FPM_VAR_Z_LINES choices 1 and 2: half and double).
+constexpr int z_elems(int ZV) { return ZV == 1 ? FPM_Z_ELEMS / 2 : (ZV == 2 ? 2 * FPM_Z_ELEMS : FPM_Z_ELEMS); }
+constexpr int z_lines(int M, int ZV = 0) {
   int nl = 1;
-  while (nl * 2 * M <= FPM_Z_ELEMS) nl *= 2;
+  while (nl * 2 * M <= z_elems(ZV)) nl *= 2;
   return nl;
 }
 
@@ -907,8 +916,8 @@ static void launch_col_nw(ColArgs<T> a, hipStream_t st) {
 }
 
 template <typename T, int N, bool INV, bool KOP>
-static void launch_col_n(ColArgs<T> a, hipStream_t st) {
-  if (col_wide((int)sizeof(T), N)) launch_col_nw<T, N, INV, KOP, 1>(a, st);
+static void launch_col_n(ColArgs<T> a, const fpm_mesh* m, hipStream_t st) {
+  if (col_choice(m, (int)sizeof(T), N)) launch_col_nw<T, N, INV, KOP, 1>(a, st);
   else launch_col_nw<T, N, INV, KOP, 0>(a, st);
 }
 
@@ -928,21 +937,21 @@ static void launch_col_fused_nw(ColArgs<T> a, hipStream_t st) {
 }
 
 template <typename T, int N>
-static void launch_col_fused_n(ColArgs<T> a, hipStream_t st) {
-  if (col_wide((int)sizeof(T), N)) launch_col_fused_nw<T, N, 1>(a, st);
+static void launch_col_fused_n(ColArgs<T> a, const fpm_mesh* m, hipStream_t st) {
+  if (col_choice(m, (int)sizeof(T), N)) launch_col_fused_nw<T, N, 1>(a, st);
   else launch_col_fused_nw<T, N, 0>(a, st);
 }
 
 // Forward transform along the e axis (stored to a.kout, times a.kscale) fused
 // with the multiplier and inverse transform of every output.
 template <typename T>
-static int launch_col_fused(int n, ColArgs<T> a, hipStream_t st) {
+static int launch_col_fused(int n, ColArgs<T> a, const fpm_mesh* m, hipStream_t st) {
   if (a.ncomp < 1 || a.ncomp > 6) return FPM_ERR_ARG;
   for (int c = 0; c < a.ncomp; ++c)
     if (!a.out[c]) return FPM_ERR_ARG;
   switch (n) {
 #define FPM_CASE(N_) \
-  case N_: launch_col_fused_n<T, N_>(a, st); break;
+  case N_: launch_col_fused_n<T, N_>(a, m, st); break;
     FPM_LENGTHS(FPM_CASE)
 #undef FPM_CASE
     default: return FPM_ERR_UNSUPPORTED;
@@ -951,13 +960,13 @@ static int launch_col_fused(int n, ColArgs<T> a, hipStream_t st) {
 }
 
 template <typename T, bool INV, bool KOP>
-static int launch_col(int n, ColArgs<T> a, hipStream_t st) {
+static int launch_col(int n, ColArgs<T> a, const fpm_mesh* m, hipStream_t st) {
   if (a.ncomp < 1 || a.ncomp > 6 || (!KOP && a.ncomp != 1)) return FPM_ERR_ARG;
   for (int c = 0; c < a.ncomp; ++c)
     if (!a.out[c]) return FPM_ERR_ARG;
   switch (n) {
 #define FPM_CASE(N_) \
-  case N_: launch_col_n<T, N_, INV, KOP>(a, st); break;
+  case N_: launch_col_n<T, N_, INV, KOP>(a, m, st); break;
     FPM_LENGTHS(FPM_CASE)
 #undef FPM_CASE
     default: return FPM_ERR_UNSUPPORTED;
@@ -965,9 +974,9 @@ static int launch_col(int n, ColArgs<T> a, hipStream_t st) {
   return FPM_OK;
 }
 
-template <typename T, int M, bool INV>
-static void launch_z_m(const ZArgs<T>& a, hipStream_t st) {
-  constexpr int NL = z_lines(M);
+template <typename T, int M, bool INV, int ZV>
+static void launch_z_mv(const ZArgs<T>& a, hipStream_t st) {
+  constexpr int NL = z_lines(M, ZV);
   constexpr int NT = clampi(round64(NL * M / FPM_Z_EPT), 64, 1024);
   const unsigned nblk = (unsigned)((a.nrows + NL - 1) / NL);
   if (INV)
@@ -976,13 +985,22 @@ static void launch_z_m(const ZArgs<T>& a, hipStream_t st) {
     hipLaunchKernelGGL((k_fft_z_r2c<T, M, NL, NT>), dim3(nblk), dim3(NT), 0, st, a);
 }
 
+template <typename T, int M, bool INV>
+static void launch_z_m(const ZArgs<T>& a, const fpm_mesh* m, hipStream_t st) {
+  switch (FPM_VAR_FIELD(m->variant, FPM_VAR_Z_LINES)) {
+    case 1: launch_z_mv<T, M, INV, 1>(a, st); break;
+    case 2: launch_z_mv<T, M, INV, 2>(a, st); break;
+    default: launch_z_mv<T, M, INV, 0>(a, st); break;
+  }
+}
+
 template <typename T, bool INV>
-static int launch_z(int n2, const ZArgs<T>& a, hipStream_t st) {
+static int launch_z(int n2, const ZArgs<T>& a, const fpm_mesh* m, hipStream_t st) {
   if (n2 % 2) return FPM_ERR_UNSUPPORTED;
   if (a.nrows <= 0) return FPM_OK;
   switch (n2 / 2) {
 #define FPM_CASE(N_) \
-  case N_: launch_z_m<T, N_, INV>(a, st); break;
+  case N_: launch_z_m<T, N_, INV>(a, m, st); break;
     FPM_LENGTHS(FPM_CASE)
 #undef FPM_CASE
     default: return FPM_ERR_UNSUPPORTED;
@@ -1015,7 +1033,7 @@ static int r2c_impl(const void* in, void* out, const fpm_mesh* m, int64_t pitch2
   z.nrows = (int64_t)n0 * n1;
   z.pitch2 = pitch2;
   z.tw = static_cast<const C<T>*>(tw2);
-  if ((rc = launch_z<T, false>(n2, z, st))) return rc;
+  if ((rc = launch_z<T, false>(n2, z, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   // y pass, in place
   ColArgs<T> y{};
@@ -1028,7 +1046,7 @@ static int r2c_impl(const void* in, void* out, const fpm_mesh* m, int64_t pitch2
   y.ncols = nh;
   y.scale = (T)1;
   y.tw = static_cast<const C<T>*>(tw1);
-  if ((rc = launch_col<T, false, false>(n1, y, st))) return rc;
+  if ((rc = launch_col<T, false, false>(n1, y, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   // x pass, in place, with 1/N
   ColArgs<T> x{};
@@ -1041,7 +1059,7 @@ static int r2c_impl(const void* in, void* out, const fpm_mesh* m, int64_t pitch2
   x.ncols = nh;
   x.scale = (T)(1.0 / ((double)n0 * n1 * n2));
   x.tw = static_cast<const C<T>*>(tw0);
-  if ((rc = launch_col<T, false, false>(n0, x, st))) return rc;
+  if ((rc = launch_col<T, false, false>(n0, x, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }
@@ -1102,7 +1120,7 @@ static int c2r_impl(const void* in, int ncomp, const fpm_kop* ops, void* const* 
   x.eaxis = 0;
   x.oaxis = 1;
   if ((rc = fill_kop(x.kop, ncomp, ops, m))) return rc;
-  if ((rc = launch_col<T, true, true>(n0, x, st))) return rc;
+  if ((rc = launch_col<T, true, true>(n0, x, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   for (int c = 0; c < ncomp; ++c) {
     C<T>* wc = static_cast<C<T>*>(work[c]);
@@ -1116,7 +1134,7 @@ static int c2r_impl(const void* in, int ncomp, const fpm_kop* ops, void* const* 
     y.ncols = nh;
     y.scale = (T)1;
     y.tw = static_cast<const C<T>*>(tw1);
-    if ((rc = launch_col<T, true, false>(n1, y, st))) return rc;
+    if ((rc = launch_col<T, true, false>(n1, y, m, st))) return rc;
     FPM_LAUNCH_CHECK();
     ZArgs<T> z{};
     z.cin = wc;
@@ -1124,7 +1142,7 @@ static int c2r_impl(const void* in, int ncomp, const fpm_kop* ops, void* const* 
     z.nrows = (int64_t)n0 * n1;
     z.pitch2 = pitch2;
     z.tw = static_cast<const C<T>*>(tw2);
-    if ((rc = launch_z<T, true>(n2, z, st))) return rc;
+    if ((rc = launch_z<T, true>(n2, z, m, st))) return rc;
     FPM_LAUNCH_CHECK();
   }
   return FPM_OK;
@@ -1159,7 +1177,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   z.nrows = (int64_t)n0 * n1;
   z.pitch2 = pitch2;
   z.tw = static_cast<const C<T>*>(tw2);
-  if ((rc = launch_z<T, false>(n2, z, st))) return rc;
+  if ((rc = launch_z<T, false>(n2, z, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   ColArgs<T> y{};
   y.in = ck;
@@ -1171,7 +1189,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   y.ncols = nh;
   y.scale = (T)1;
   y.tw = static_cast<const C<T>*>(tw1);
-  if ((rc = launch_col<T, false, false>(n1, y, st))) return rc;
+  if ((rc = launch_col<T, false, false>(n1, y, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   ColArgs<T> x{};
   x.in = ck;
@@ -1188,14 +1206,14 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   x.eaxis = 0;
   x.oaxis = 1;
   if ((rc = fill_kop(x.kop, ncomp, ops, m))) return rc;
-  if ((rc = launch_col_fused<T>(n0, x, st))) return rc;
+  if ((rc = launch_col_fused<T>(n0, x, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   for (int c = 0; c < ncomp; ++c) {
     C<T>* wc = static_cast<C<T>*>(work[c]);
     ColArgs<T> yi = y;
     yi.in = wc;
     yi.out[0] = wc;
-    if ((rc = launch_col<T, true, false>(n1, yi, st))) return rc;
+    if ((rc = launch_col<T, true, false>(n1, yi, m, st))) return rc;
     FPM_LAUNCH_CHECK();
     ZArgs<T> zi{};
     zi.cin = wc;
@@ -1203,7 +1221,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
     zi.nrows = (int64_t)n0 * n1;
     zi.pitch2 = pitch2;
     zi.tw = static_cast<const C<T>*>(tw2);
-    if ((rc = launch_z<T, true>(n2, zi, st))) return rc;
+    if ((rc = launch_z<T, true>(n2, zi, m, st))) return rc;
     FPM_LAUNCH_CHECK();
   }
   return FPM_OK;
@@ -1249,7 +1267,7 @@ static int c2r_grad3_impl(const void* in, const fpm_kop* common, void* const* ou
   x.eaxis = 0;
   x.oaxis = 1;
   if ((rc = fill_kop(x.kop, 2, xo, m))) return rc;
-  if ((rc = launch_col<T, true, true>(n0, x, st))) return rc;
+  if ((rc = launch_col<T, true, true>(n0, x, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   // y pass on A: identity (in place) and the y gradient (into C)
   fpm_kop yo[2] = {};
@@ -1272,14 +1290,14 @@ static int c2r_grad3_impl(const void* in, const fpm_kop* common, void* const* ou
   y.eaxis = 1;
   y.oaxis = 0;
   if ((rc = fill_kop(y.kop, 2, yo, m))) return rc;
-  if ((rc = launch_col<T, true, true>(n1, y, st))) return rc;
+  if ((rc = launch_col<T, true, true>(n1, y, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   // y pass on B, in place
   ColArgs<T> yb = y;
   yb.in = B;
   yb.out[0] = B;
   yb.ncomp = 1;
-  if ((rc = launch_col<T, true, false>(n1, yb, st))) return rc;
+  if ((rc = launch_col<T, true, false>(n1, yb, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   // z passes: B -> F_x, C -> F_y, A with the z gradient -> F_z
   const C<T>* src[3] = {B, Cw, A};
@@ -1293,7 +1311,7 @@ static int c2r_grad3_impl(const void* in, const fpm_kop* common, void* const* ou
     z.zgrad = d == 2 ? 1 : 0;
     z.zorder = common->grad_order;
     z.L2 = m->box[2];
-    if ((rc = launch_z<T, true>(n2, z, st))) return rc;
+    if ((rc = launch_z<T, true>(n2, z, m, st))) return rc;
     FPM_LAUNCH_CHECK();
   }
   return FPM_OK;

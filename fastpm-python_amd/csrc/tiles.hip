@@ -588,10 +588,10 @@ __device__ __forceinline__ void gather_ranges(const TileGeom& g, const Workspace
 // The up to 64 (source tile, sub-bin) ranges feeding a tile are gathered
 // first (one lane each) and scanned, then all their particles are processed
 // as one flat index space, so their loads are issued together.
-template <typename T, int ORDER>
-__global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, TileGeom g,
-                                                               Workspace w, T* mesh, float mass_fs,
-                                                               int sh, double inv_fx) {
+template <typename T, int ORDER, int NT = kTileThreads>
+__global__ __launch_bounds__(NT) void k_paint_tiles(PartDesc<T> P, TileGeom g, Workspace w,
+                                                     T* mesh, float mass_fs, int sh,
+                                                     double inv_fx) {
   constexpr int NB = TILE * TILE * TILE;
   __shared__ unsigned long long acc[NB];
   __shared__ int32_t rstart[64];
@@ -600,11 +600,11 @@ __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, Til
   const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
   int lo[3];
   tile_coords(t, g, lo);
-  for (int i = threadIdx.x; i < NB; i += kTileThreads) acc[i] = 0ull;
+  for (int i = threadIdx.x; i < NB; i += NT) acc[i] = 0ull;
   gather_ranges(g, w, lo, rstart, rpre, rdir);
   __syncthreads();
   const int32_t total = rpre[64];
-  for (int32_t f = threadIdx.x; f < total; f += kTileThreads) {
+  for (int32_t f = threadIdx.x; f < total; f += NT) {
     // range r with rpre[r] <= f < rpre[r + 1]
     int r = 0;
 #pragma unroll
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(kTileThreads) void k_paint_tiles(PartDesc<T> P, Til
     deposit_rec<T, ORDER>(w.rec[rstart[r] + (f - rpre[r])], rdir[r], g, acc, mass_fs, sh);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < NB; i += kTileThreads) {
+  for (int i = threadIdx.x; i < NB; i += NT) {
     const int z = i % TILE, y = (i / TILE) % TILE, x = i / (TILE * TILE);
     mesh[((int64_t)(lo[0] + x) * g.nm[1] + (lo[1] + y)) * g.nm[2] + (lo[2] + z)] =
         (T)((double)(long long)acc[i] * inv_fx);
@@ -1261,11 +1261,18 @@ static int prep(const fpm_particles* part, const fpm_mesh* m, int order, void* w
   return FPM_OK;
 }
 
-static BinOrder bin_order(const fpm_particles* part) {
+// The walk over lattice particles of the count and scatter passes: 4 x 8 x 16
+// blocks where the lattice divides into them, else 8 x 8 patches, else index
+// order (the mesh's FPM_VAR_BIN field can ask for patches or index order).
+// Every walk gives the same sub-bins; only the order within a sub-bin differs,
+// which the fixed-point paint and the readouts do not see.
+static BinOrder bin_order(const fpm_particles* part, const fpm_mesh* m) {
   BinOrder o{};
-  if (!part->q && part->lattice[2] % 8 == 0 && part->lattice[1] % 8 == 0) {
+  const int bv = FPM_VAR_FIELD(m->variant, FPM_VAR_BIN);
+  if (bv != 2 && !part->q && part->lattice[2] % 8 == 0 && part->lattice[1] % 8 == 0) {
     // block order needs whole blocks and a workgroup span of one block
-    const bool block = part->lattice[0] % kBlockX == 0 && part->lattice[2] % 16 == 0;
+    const bool block =
+        bv == 0 && part->lattice[0] % kBlockX == 0 && part->lattice[2] % 16 == 0;
     o.blocked = block ? kBinBlock : kBinPatch;
     o.nz = (uint32_t)(part->lattice[2] / (block ? 16 : 8));
     o.ny = (uint32_t)(part->lattice[1] / 8);
@@ -1307,7 +1314,7 @@ static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, voi
   Workspace w;
   int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
   if (rc) return rc;
-  const BinOrder o = bin_order(part);
+  const BinOrder o = bin_order(part, m);
   bin_clear(g, w, st);
   if (P.n > 0) {
     if (order == 2)
@@ -1341,13 +1348,18 @@ static int paint_tiles_impl(const fpm_particles* part, const fpm_mesh* m, int or
   const float mass_fs = (float)(mass * std::ldexp(1.0, S - sh));
   const double inv_fx = std::ldexp(1.0, -S);
   T* mesh = static_cast<T*>(out);
-  const dim3 grid((unsigned)g.ntiles), blk(kTileThreads);
-  if (order == 2)
-    hipLaunchKernelGGL((k_paint_tiles<T, 2>), grid, blk, 0, st, P, g, w, mesh, mass_fs, sh,
-                       inv_fx);
-  else
-    hipLaunchKernelGGL((k_paint_tiles<T, 3>), grid, blk, 0, st, P, g, w, mesh, mass_fs, sh,
-                       inv_fx);
+  const dim3 grid((unsigned)g.ntiles);
+  switch (FPM_VAR_FIELD(m->variant, FPM_VAR_PAINT) * 4 + order) {
+#define FPM_PAINT(V_, O_, NT_)                                                                  \
+  case V_ * 4 + O_:                                                                             \
+    hipLaunchKernelGGL((k_paint_tiles<T, O_, NT_>), grid, dim3(NT_), 0, st, P, g, w, mesh,       \
+                       mass_fs, sh, inv_fx);                                                    \
+    break;
+    FPM_PAINT(1, 2, 256) FPM_PAINT(1, 3, 256) FPM_PAINT(2, 2, 1024) FPM_PAINT(2, 3, 1024)
+    FPM_PAINT(0, 2, kTileThreads) FPM_PAINT(0, 3, kTileThreads)
+#undef FPM_PAINT
+    default: return FPM_ERR_ARG;
+  }
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }
@@ -1457,12 +1469,23 @@ static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int o
     G.f_scale[d] = (T)(f_scale * (double)m->nmesh[d] / (12.0 * m->box[d]));
   if (P.n == 0) return FPM_OK;
   const dim3 grid((unsigned)g.ntiles);
-  if (order == 2)
-    hipLaunchKernelGGL((k_readout_grad_tiles<T, 2>), grid, dim3(grad_threads<2>()), 0, st, P, g,
-                       w, G);
-  else
-    hipLaunchKernelGGL((k_readout_grad_tiles<T, 3>), grid, dim3(grad_threads<3>()), 0, st, P, g,
-                       w, G);
+  int nt = 0;
+  switch (FPM_VAR_FIELD(m->variant, FPM_VAR_READOUT)) {
+    case 1: nt = 256; break;
+    case 2: nt = 512; break;
+    case 3: nt = 1024; break;
+    default: nt = order == 2 ? grad_threads<2>() : grad_threads<3>(); break;
+  }
+  switch (nt * 4 + order) {
+#define FPM_GRAD(O_, NT_)                                                                       \
+  case NT_ * 4 + O_:                                                                            \
+    hipLaunchKernelGGL((k_readout_grad_tiles<T, O_, NT_>), grid, dim3(NT_), 0, st, P, g, w, G); \
+    break;
+    FPM_GRAD(2, 256) FPM_GRAD(2, 512) FPM_GRAD(2, 1024)
+    FPM_GRAD(3, 256) FPM_GRAD(3, 512) FPM_GRAD(3, 1024)
+#undef FPM_GRAD
+    default: return FPM_ERR_ARG;
+  }
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }

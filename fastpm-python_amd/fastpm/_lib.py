@@ -28,9 +28,34 @@ class FpmParticles(ctypes.Structure):
 class FpmMesh(ctypes.Structure):
     _fields_ = [
         ("nmesh", ctypes.c_int32 * 3),
-        ("pad_", ctypes.c_int32),
+        ("variant", ctypes.c_int32),
         ("box", ctypes.c_double * 3),
     ]
+
+
+# fields of FpmMesh.variant (fpm.h FPM_VAR_*): name -> (bit shift, choices)
+VARIANT_FIELDS = {
+    "fft_cols": (0, (0, 1, 2)),
+    "z_lines": (2, (0, 1, 2)),
+    "bin": (4, (0, 1, 2)),
+    "readout": (6, (0, 1, 2, 3)),
+    "paint": (8, (0, 1, 2)),
+}
+
+
+def pack_variant(choices):
+    """FpmMesh.variant from {field: choice}; unnamed fields stay 0 (default)."""
+    v = 0
+    for name, c in choices.items():
+        shift, allowed = VARIANT_FIELDS[name]
+        if int(c) not in allowed:
+            raise ValueError("variant %s=%r not in %s" % (name, c, allowed))
+        v |= int(c) << shift
+    return v
+
+
+def unpack_variant(v):
+    return {name: (int(v) >> shift) & 3 for name, (shift, _) in VARIANT_FIELDS.items()}
 
 
 class FpmKop(ctypes.Structure):

@@ -141,8 +141,9 @@ class Solver:
         in nbody without a monitor, each step boundary's kicks and drift as
         one pass; False runs the plain sequence of the reference (one inverse
         transform and one readout per component, separate updates).
-    autotune : time the transform tilings on the force mesh first and keep
-        the faster (fastpm.tune.autotune_fft).
+    autotune : choose the force mesh's kernel variants by timing them
+        (fastpm.tune): the transform variants here, the binning, paint and
+        readout variants on the particles of the first force.
     """
 
     def __init__(self, pm, cosmology, B=1, a_linear=1.0, compensate=False, fused=True,
@@ -158,8 +159,10 @@ class Solver:
         self.a_linear = a_linear
         self.compensate = bool(compensate)
         self.fused = bool(fused)
+        self.autotune = bool(autotune)
+        self._particles_tuned = False
         if autotune:
-            # transform tiling of the force mesh (fastpm.tune)
+            # transform variants of the force mesh (fastpm.tune)
             from .tune import autotune_fft
 
             autotune_fft(fpm)
@@ -374,6 +377,12 @@ class FastPMStep:
         X1 = state.particles
         layout = self.pm.decompose(X1, smoothing)
         rho = self.workspace()["rho"]
+        if self.tiled and self.solver.autotune and not self.solver._particles_tuned:
+            # binning, paint and readout variants on these particles (fastpm.tune)
+            from .tune import autotune_particles
+
+            autotune_particles(self.pm, X1)
+            self.solver._particles_tuned = True
         if self.tiled:
             with self._mark("bin"):
                 self.pm.bin_particles(X1)

@@ -45,8 +45,8 @@ class Ensemble:
         over `nstreams` streams (largest first, each to the least loaded
         stream) so small boxes run concurrently; inside a captured graph the
         streams become parallel branches (64 boxes, 20 steps: 8.49 / 8.21 /
-        8.12 / 8.02 ms per step with 2 / 4 / 8 / 16 streams).  autotune selects the transform
-        tiling per force-mesh shape before the first step (fastpm.tune)."""
+        8.12 / 8.02 ms per step with 2 / 4 / 8 / 16 streams).  autotune selects the kernel
+        variants per force-mesh shape before the first step (fastpm.tune)."""
         self.stages = np.asarray(stages, dtype="f8")
         self.autotune = bool(autotune)
         self.boxes = [Box(s["nc"], s["box"], s["cosmo"], s["seed"], dtype, B, resampler,
@@ -74,21 +74,18 @@ class Ensemble:
         Pk_cache = {} if Pk_cache is None else Pk_cache
         a0 = self.stages[0]
         if self.autotune:
-            # one tiling choice per distinct force-mesh shape (fastpm.tune)
-            from .tune import autotune_fft
-
-            seen = set()
-            for b in self.boxes:
-                shape = tuple(int(n) for n in b.solver.fpm.Nmesh)
-                if shape not in seen:
-                    seen.add(shape)
-                    autotune_fft(b.solver.fpm)
+            from .tune import autotune_fft, autotune_particles
         for b in self.boxes:
             key = (b.cosmo.sigma8, b.cosmo.Om0, b.cosmo.h, b.cosmo.n_s)
             if key not in Pk_cache:
                 Pk_cache[key] = LinearPower(b.cosmo)
             dlin = b.solver.linear(b.solver.whitenoise(b.seed), Pk_cache[key])
             b.state = b.solver.lpt(dlin, b.pm.generate_uniform_particle_grid(), a=a0)
+            if self.autotune:
+                # kernel variants per force-mesh shape and lattice (fastpm.tune):
+                # timed on the first box of each, cached for the others
+                autotune_fft(b.solver.fpm)
+                autotune_particles(b.solver.fpm, b.state.particles)
             b.step.run("F", a0, a0, a0, b.state, None)
             pt = b.step._pt
             rows = []

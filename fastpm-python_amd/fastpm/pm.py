@@ -171,6 +171,19 @@ class ParticleMesh:
 
     # ------------------------------------------------------------------ helpers
     @property
+    def variant(self):
+        """Kernel variants of this mesh ({field: choice}, fpm.h FPM_VAR_*;
+        0 = default), passed with every library call on the mesh; set by
+        fastpm.tune."""
+        return _lib.unpack_variant(self._desc.variant)
+
+    def set_variant(self, **choices):
+        """Updates the named variant fields, e.g. set_variant(fft_cols=2)."""
+        cur = self.variant
+        cur.update(choices)
+        self._desc.variant = _lib.pack_variant(cur)
+
+    @property
     def order(self):
         return resampler_order(self.resampler)
 

@@ -1,14 +1,26 @@
-"""Autotuner for the transform tiling.
+"""Autotuner for the kernel variants of a mesh.
 
-The strided (y, x) passes of the 3D transforms come in two compiled column
-tile widths (64- or 128-byte row segments, `fpm_fft_set_tiling`).  Which is
-faster depends on the axis length and on what else runs (measured on gfx950:
-128-byte tiles win at 512^3, 64-byte tiles at 192^3).  `autotune_fft` times
-the Force's transform (forward transform of the density and inverse
-transform of the potential) for each choice on scratch fields of the mesh
-and keeps the faster one for the mesh's axis lengths.  Choices are cached
-per (precision, shape) for the process and, when FASTPM_AMD_TUNE_CACHE names
-a JSON file, across processes.  The results agree to rounding either way.
+Every mesh descriptor carries a variant word (fpm.h FPM_VAR_*) that selects,
+per mesh, among compiled variants of the hot kernels:
+
+  fft_cols  the strided (y, x) transform passes: column tiles of 64- or
+            128-byte row segments (measured on gfx950: 128-byte tiles win at
+            512^3, 64-byte tiles at 192^3);
+  z_lines   the z transform pass: 1024, 2048 or 4096 complex elements per
+            workgroup (fewer lines per workgroup fill the chip on small
+            meshes);
+  bin       the binning walk over lattice particles: 4 x 8 x 16 blocks,
+            8 x 8 patches or index order;
+  paint     threads per tile of the tiled paint (256, 512, 1024);
+  readout   threads per tile of the stencil force readout (256, 512, 1024).
+
+Every choice gives the same results (bitwise for the particle kernels, to
+rounding for the transform tilings).  `autotune` times them on the mesh by
+coordinate descent (the transform fields together on the Force's transform,
+then each particle-kernel field on the binning, paint and readout of a
+particle set), keeps the fastest, and writes it into the mesh descriptor.
+Choices are cached per (precision, mesh shape, window order, lattice) for the
+process and, when FASTPM_AMD_TUNE_CACHE names a JSON file, across processes.
 """
 import json
 import os
@@ -21,8 +33,17 @@ from . import _lib
 _CACHE = {}
 
 
-def _key(pm):
-    return "%s:%s" % (np.dtype(pm.dtype).str, "x".join(str(int(n)) for n in pm.Nmesh))
+def _key(pm, order=None, parts=None):
+    k = "%s:%s" % (np.dtype(pm.dtype).str, "x".join(str(int(n)) for n in pm.Nmesh))
+    if order is not None:
+        k += ":o%d" % order
+    if parts is not None:
+        from .pm import LatticeGrid
+
+        q = parts.Q
+        k += ":lat%s" % ("x".join(str(n) for n in q.nlat) if isinstance(q, LatticeGrid)
+                         else "explicit%d" % len(q))
+    return k
 
 
 def _load_file():
@@ -30,7 +51,8 @@ def _load_file():
     if not path or not os.path.exists(path):
         return {}
     with open(path) as f:
-        return json.load(f)
+        data = json.load(f)
+    return {k: v for k, v in data.items() if isinstance(v, dict)}
 
 
 def _save_file(entries):
@@ -43,56 +65,141 @@ def _save_file(entries):
         json.dump(data, f, indent=1, sort_keys=True)
 
 
+def _lookup(key):
+    if key not in _CACHE:
+        _CACHE.update({k: v for k, v in _load_file().items() if k == key})
+    return _CACHE.get(key)
+
+
 def set_tiling(pm, wide):
-    """Selects the column tile width for the strided axes of pm's mesh."""
-    eb = np.dtype(pm.dtype).itemsize
-    for n in {int(pm.Nmesh[0]), int(pm.Nmesh[1])}:
-        _lib.call_plain("fpm_fft_set_tiling", eb, n, int(wide))
+    """Selects the column tile width of pm's strided transform passes
+    (wide = 1: 128-byte, 0: 64-byte row segments) for this mesh only."""
+    pm.set_variant(fft_cols=2 if wide else 1)
 
 
 def get_tiling(pm):
+    """{axis length: 1 if 128-byte tiles else 0} for pm's strided axes."""
+    v = pm.variant["fft_cols"]
     eb = np.dtype(pm.dtype).itemsize
-    return {int(n): int(_lib.lib().fpm_fft_get_tiling(eb, int(n)))
-            for n in {int(pm.Nmesh[0]), int(pm.Nmesh[1])}}
+    out = {}
+    for n in {int(pm.Nmesh[0]), int(pm.Nmesh[1])}:
+        out[n] = (v - 1) if v else int(_lib.lib().fpm_fft_get_tiling(eb, int(n)))
+    return out
+
+
+def _time(fn, reps):
+    for _ in range(2):
+        fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def _reps(pm):
+    return max(3, min(50, int(2 ** 27 // max(int(np.prod(pm.Nmesh)), 1))))
 
 
 def autotune_fft(pm, reps=None, verbose=False):
-    """Times both column tile widths on pm's mesh and selects the faster;
-    returns the chosen width flag (0: 64-byte, 1: 128-byte segments)."""
+    """Times the transform fields (fft_cols x z_lines) on the Force's fused
+    transform of pm and keeps the fastest; returns the chosen fields."""
     from .pm import ComplexField, r2c_c2r_many
     from .force.kernels import laplace
 
-    key = _key(pm)
-    if key not in _CACHE:
-        _CACHE.update({k: v for k, v in _load_file().items() if k == key})
-    if key in _CACHE:
-        set_tiling(pm, _CACHE[key])
-        return _CACHE[key]
-    ncell = int(np.prod(pm.Nmesh))
-    if reps is None:
-        reps = max(3, min(50, int(2 ** 27 // max(ncell, 1))))
+    key = _key(pm) + ":fft"
+    hit = _lookup(key)
+    if hit is not None:
+        pm.set_variant(**hit)
+        return hit
+    reps = _reps(pm) if reps is None else reps
     x = pm.create("real", value=torch.randn(tuple(int(n) for n in pm.Nmesh), device=pm.device,
                                             dtype=pm.tdtype))
-    kout, outs, work = ComplexField(pm), [pm.create("real")], [pm._complex_tensor()]
+    outs, work = [pm.create("real")], [pm._complex_tensor()]
     ops = [laplace * 1.0]
     times = {}
-    for wide in (0, 1):
-        set_tiling(pm, wide)
-        for _ in range(2):
-            r2c_c2r_many(x, ops, kout=kout, outs=outs, work=work)
-        torch.cuda.synchronize(pm.device)
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record()
-        for _ in range(reps):
-            r2c_c2r_many(x, ops, kout=kout, outs=outs, work=work)
-        b.record()
-        b.synchronize()
-        times[wide] = a.elapsed_time(b) / reps
-    best = min(times, key=times.get)
-    set_tiling(pm, best)
+    for cols in (1, 2):
+        for zl in (0, 1, 2):
+            pm.set_variant(fft_cols=cols, z_lines=zl)
+            times[(cols, zl)] = _time(lambda: r2c_c2r_many(x, ops, kout=False, outs=outs,
+                                                           work=work), reps)
+    cols, zl = min(times, key=times.get)
+    best = dict(fft_cols=cols, z_lines=zl)
+    pm.set_variant(**best)
     _CACHE[key] = best
     _save_file({key: best})
     if verbose:
-        print("autotune_fft %s: %s -> wide=%d" % (key, times, best))
+        print("autotune_fft %s: %s -> %s" % (key, times, best))
     return best
+
+
+def autotune_particles(pm, parts, order=None, reps=None, verbose=False):
+    """Times the particle-kernel fields (bin, paint, readout) of pm on the
+    binning, tiled paint and stencil force readout of `parts` (a Particles
+    or position set; nothing of it is modified) and keeps the fastest of
+    each; returns the chosen fields.  Meshes without the tiled path are left
+    as they are."""
+    from .pm import resampler_order
+
+    order = pm.order if order is None else order
+    if not pm.tiles_supported():
+        return {}
+    parts = pm._particles(parts)
+    key = _key(pm, order, parts)
+    hit = _lookup(key)
+    if hit is not None:
+        pm.set_variant(**hit)
+        return hit
+    rs = {2: "cic", 3: "tsc"}[order]
+    assert resampler_order(rs) == order
+    reps = max(3, min(20, int(2 ** 25 // max(len(parts), 1)))) if reps is None else reps
+    rho = pm.create("real")
+    phi = pm.create("real")
+    phi.value.normal_()
+    F = torch.empty((len(parts), 3), dtype=pm.tdtype, device=pm.device)
+    R = torch.empty((len(parts),), dtype=pm.tdtype, device=pm.device)
+    stencil = pm.grad4_supported(rs)
+    choices = {}
+    times = {}
+
+    def bin_():
+        pm.bin_particles(parts, resampler=rs)
+
+    def paint():
+        pm.paint_tiled(parts, mass=1.0, resampler=rs, out=rho)
+
+    def readout():
+        pm.readout_tiled_grad4(phi, parts, F, rho=rho, rho_out=R, resampler=rs)
+
+    fields = [("bin", (0, 1, 2), bin_), ("paint", (0, 1, 2), paint)]
+    if stencil:
+        fields.append(("readout", (1, 2, 3), readout))
+    for name, opts, fn in fields:
+        t = {}
+        for c in opts:
+            pm.set_variant(**{name: c})
+            bin_()
+            t[c] = _time(fn, reps)
+        c = min(t, key=t.get)
+        pm.set_variant(**{name: c})
+        choices[name] = c
+        times[name] = t
+    bin_()
+    _CACHE[key] = choices
+    _save_file({key: choices})
+    if verbose:
+        print("autotune_particles %s: %s -> %s" % (key, times, choices))
+    return choices
+
+
+def autotune(pm, parts=None, order=None, verbose=False):
+    """Transform fields, then (given a particle set) the particle fields of
+    pm; returns pm.variant."""
+    autotune_fft(pm, verbose=verbose)
+    if parts is not None:
+        autotune_particles(pm, parts, order=order, verbose=verbose)
+    return pm.variant

@@ -56,9 +56,34 @@ typedef struct fpm_particles {
 
 typedef struct fpm_mesh {
   int32_t nmesh[3];
-  int32_t pad_;
+  int32_t variant;        /* kernel variants for this mesh, FPM_VAR_* fields; 0 = defaults */
   double box[3];
 } fpm_mesh;
+
+/* Fields of fpm_mesh.variant (2 bits each).  Every choice computes the same
+ * results (bitwise for the particle kernels; to rounding for the transform
+ * tilings, whose per-column arithmetic is the same up to multiply-add
+ * contraction); fastpm.tune times them per mesh and keeps the fastest.  No
+ * counterpart in the reference (pmesh plans its own transforms).
+ *   FPM_VAR_FFT_COLS : strided transform passes, 0 = the process-wide choice
+ *                      of fpm_fft_set_tiling, 1 = 64-byte, 2 = 128-byte row
+ *                      segments per column tile
+ *   FPM_VAR_Z_LINES  : z transform pass, 0 = 2048, 1 = 1024, 2 = 4096 complex
+ *                      elements (rows x length) per workgroup
+ *   FPM_VAR_BIN      : binning walk over lattice particles, 0 = automatic
+ *                      (4 x 8 x 16 blocks where the lattice divides, else
+ *                      8 x 8 patches), 1 = patches, 2 = index order
+ *   FPM_VAR_READOUT  : threads per tile of the stencil force readout
+ *                      (fpm_readout_tiled_grad4_f32), 0 = default (1024 CIC,
+ *                      512 TSC), 1 = 256, 2 = 512, 3 = 1024
+ *   FPM_VAR_PAINT    : threads per tile of the tiled paint, 0 = default
+ *                      (512), 1 = 256, 2 = 1024 */
+#define FPM_VAR_FFT_COLS 0
+#define FPM_VAR_Z_LINES 2
+#define FPM_VAR_BIN 4
+#define FPM_VAR_READOUT 6
+#define FPM_VAR_PAINT 8
+#define FPM_VAR_FIELD(variant, shift) (((variant) >> (shift)) & 3)
 
 /* A k-space multiplier, evaluated per mode and fused into the first pass of the
  * inverse transform:  factor * [1/k^2 if laplace, 0 at k=0]
@@ -90,14 +115,14 @@ int fpm_fft_supported(int n);
 int fpm_twiddles_f32(int n, float* host_out);
 int fpm_twiddles_f64(int n, double* host_out);
 
-/* Run-time tiling of the strided (y, x) transform passes: for axis length n
- * and element size elem_bytes (4: float32, 8: float64), wide = 1 selects
- * column tiles of twice the default row segment (128 instead of 64 bytes),
- * wide = 0 the default.  Results agree to rounding (each column is
- * transformed with the same arithmetic, up to multiply-add contraction).  The setting is
- * process-wide: set it before launching transforms from other threads.
- * fastpm.tune.autotune_fft times both and keeps the faster per mesh.  No
- * counterpart in the reference (pmesh plans its FFTs through pfft). */
+/* Process-wide default tiling of the strided (y, x) transform passes, used
+ * by meshes whose FPM_VAR_FFT_COLS field is 0: for axis length n and element
+ * size elem_bytes (4: float32, 8: float64), wide = 1 selects column tiles of
+ * twice the default row segment (128 instead of 64 bytes), wide = 0 the
+ * default.  Results agree to rounding (each column is transformed with the
+ * same arithmetic, up to multiply-add contraction).  Set it before launching
+ * transforms from other threads; a mesh's own variant field overrides it.
+ * No counterpart in the reference (pmesh plans its FFTs through pfft). */
 int fpm_fft_set_tiling(int elem_bytes, int n, int wide);
 int fpm_fft_get_tiling(int elem_bytes, int n);
 

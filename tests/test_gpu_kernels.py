@@ -498,7 +498,9 @@ def test_fft_tiling_bitwise(gpu, dtype, nmesh):
         assert (a - b).abs().max().item() <= tol * b.abs().max().item()
     tune._CACHE.clear()
     best = tune.autotune_fft(pm, reps=3)
-    assert best in (0, 1) and set(tune.get_tiling(pm).values()) == {best}
+    assert best["fft_cols"] in (1, 2) and best["z_lines"] in (0, 1, 2)
+    assert set(tune.get_tiling(pm).values()) == {best["fft_cols"] - 1}
+    assert pm.variant["fft_cols"] == best["fft_cols"]
 
 
 @pytest.mark.parametrize("order", [2, 3])

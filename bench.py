@@ -151,30 +151,37 @@ def run_ensemble():
     from fastpm.ensemble import Ensemble, config_specs
 
     stages = np.linspace(0.1, 1.0, 21)
-    ens = Ensemble(config_specs(64), stages)
-    t0 = time.perf_counter()
-    ens.initialize()
-    torch.cuda.synchronize()
-    t_init = time.perf_counter() - t0
-    ens.step_eager()
-    torch.cuda.synchronize()
-    ens.capture()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(ens.nsteps - 1):
-        ens.graph.replay()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    ens.finish()  # the last closing kick (each replay began with the previous one)
-    n = ens.nsteps - 1
-    out = dict(workload="64 boxes, particle grids {32,48,64,96,128}^3, meshes 2x, 20 steps, "
-                        "FP32, graph replay per step",
-               particles=int(ens.npart),
-               cells=int(sum(int(b.solver.fpm.Nmesh.prod()) for b in ens.boxes)),
-               value=ens.npart * n / el, unit="particle-steps/s", ms_per_step=el / n * 1e3,
-               timed_steps=n, init_seconds=t_init)
-    del ens
-    torch.cuda.empty_cache()
+    out = None
+    for batched in (True, False):
+        ens = Ensemble(config_specs(64), stages, batched=batched)
+        t0 = time.perf_counter()
+        ens.initialize()
+        torch.cuda.synchronize()
+        t_init = time.perf_counter() - t0
+        ens.step_eager()
+        torch.cuda.synchronize()
+        ens.capture()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(ens.nsteps - 1):
+            ens.graph.replay()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ens.finish()  # the last closing kick (each replay began with the previous one)
+        n = ens.nsteps - 1
+        if batched:
+            out = dict(workload="64 boxes, particle grids {32,48,64,96,128}^3, meshes 2x, 20 steps, "
+                                "FP32, graph replay per step, batched launches (%d plans)"
+                                % len(ens.plans),
+                       particles=int(ens.npart),
+                       cells=int(sum(int(b.solver.fpm.Nmesh.prod()) for b in ens.boxes)),
+                       value=ens.npart * n / el, unit="particle-steps/s",
+                       ms_per_step=el / n * 1e3, timed_steps=n, init_seconds=t_init)
+        else:
+            # the same step with each box's own launches on 16 streams
+            out["per_box_streams_ms_per_step"] = el / n * 1e3
+        del ens
+        torch.cuda.empty_cache()
     return out
 
 

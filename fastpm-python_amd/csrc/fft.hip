@@ -15,7 +15,7 @@
 // transforms on graphics processors" (SC08): input j + r N/R, twiddle
 // w_N^{r (j mod Ns) N/(Ns R)}, output (j/Ns) Ns R + j mod Ns + r Ns.
 // Twiddles come from device tables computed in double on the host.
-#include "fpm_common.h"
+#include "batch.h"
 
 #include <cmath>
 #include <vector>
@@ -438,8 +438,8 @@ struct ColArgs {
 // multiplier to a copy.  Measured at 512^3 on
 // gfx950 this beats one workgroup per (tile, output), whose repeated tile
 // reads mostly miss L2.
-template <typename T, int N, int BZ, int NT, bool INV, bool KOP, bool MC, bool FWD = false>
-__global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
+template <typename T, int N, int BZ, int NT, bool INV, bool KOP, bool MC, bool FWD>
+__device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
   using V = C<T>;
   constexpr int E = N * BZ;
   constexpr int MAXC = MC ? 6 : 1;
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   __shared__ T kz[KOP ? BZ : 1];
   __shared__ T fo[KOP ? MAXC : 1];
 
-  const int64_t bid = xcd_remap(blockIdx.x, a.nblk);
+  const int64_t bid = xcd_remap(blk, a.nblk);
   const int outer = (int)(bid / a.ntz);
   const int z0 = (int)(bid % a.ntz) * BZ;
 
@@ -708,6 +708,11 @@ __global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
   }
 }
 
+template <typename T, int N, int BZ, int NT, bool INV, bool KOP, bool MC, bool FWD = false>
+__global__ __launch_bounds__(NT) void k_fft_col(ColArgs<T> a) {
+  fft_col_body<T, N, BZ, NT, INV, KOP, MC, FWD>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------- z passes
 // Real rows of length 2M are transformed as length-M complex rows
 // (z_m = x_{2m} + i x_{2m+1}) and unpacked to M+1 half-spectrum modes:
@@ -743,7 +748,7 @@ template <int M> constexpr int z_pitch() {
 // Forward: the first Stockham stage reads its inputs straight from the real
 // rows; the unpacked half spectrum is stored straight from registers.
 template <typename T, int M, int NL, int NT>
-__global__ __launch_bounds__(NT) void k_fft_z_r2c(ZArgs<T> a) {
+__device__ __forceinline__ void fft_z_r2c_body(const ZArgs<T>& a, int64_t blk) {
   using V = C<T>;
   constexpr int PAD = FPM_Z_PAD;
   constexpr int LP = z_pitch<M>();
@@ -754,7 +759,7 @@ __global__ __launch_bounds__(NT) void k_fft_z_r2c(ZArgs<T> a) {
   constexpr int PERP = (NL * NP + NT - 1) / NT;
   __shared__ V s[NL * LP];
   __shared__ V tw[2 * M];
-  const int64_t row0 = (int64_t)blockIdx.x * NL;
+  const int64_t row0 = blk * NL;
   for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = a.tw[i];
   const V* in = reinterpret_cast<const V*>(a.rin);
 #pragma unroll
@@ -802,10 +807,15 @@ __global__ __launch_bounds__(NT) void k_fft_z_r2c(ZArgs<T> a) {
   }
 }
 
+template <typename T, int M, int NL, int NT>
+__global__ __launch_bounds__(NT) void k_fft_z_r2c(ZArgs<T> a) {
+  fft_z_r2c_body<T, M, NL, NT>(a, blockIdx.x);
+}
+
 // Inverse: the packed rows are built straight from the global loads (with the
 // optional z-gradient factor) and the last stage stores the real rows.
 template <typename T, int M, int NL, int NT>
-__global__ __launch_bounds__(NT) void k_fft_z_c2r(ZArgs<T> a) {
+__device__ __forceinline__ void fft_z_c2r_body(const ZArgs<T>& a, int64_t blk) {
   using V = C<T>;
   constexpr int PAD = FPM_Z_PAD;
   constexpr int LP = z_pitch<M>();
@@ -816,7 +826,7 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(ZArgs<T> a) {
   constexpr int PERP = (NL * NP + NT - 1) / NT;
   __shared__ V s[NL * LP];
   __shared__ V tw[2 * M];
-  const int64_t row0 = (int64_t)blockIdx.x * NL;
+  const int64_t row0 = blk * NL;
   for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = a.tw[i];
   T two_pi_l = (T)0, inv6h = (T)0;
   if (a.zgrad) {
@@ -883,6 +893,11 @@ __global__ __launch_bounds__(NT) void k_fft_z_c2r(ZArgs<T> a) {
       }
     }
   }
+}
+
+template <typename T, int M, int NL, int NT>
+__global__ __launch_bounds__(NT) void k_fft_z_c2r(ZArgs<T> a) {
+  fft_z_c2r_body<T, M, NL, NT>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -1364,6 +1379,238 @@ static int twiddles_impl(int n, T* out) {
     if (4 * m == 3 * n) { c = 0.0; s = 1.0; }
     out[2 * m] = (T)c;
     out[2 * m + 1] = (T)s;
+  }
+  return FPM_OK;
+}
+
+// ---------------------------------------------------------------- batched boxes
+// The Force's transform (fpm_fft_r2c_c2r with one multiplier and no stored
+// spectrum) of many boxes: each pass is one launch per compiled kernel (axis
+// length and tiling), one workgroup per column tile or row group of every
+// box, through the pass bodies above, each box's arguments read from a
+// device table.  Float32, axis lengths that are multiples of 16.
+enum { kFbZr2c = 1, kFbYfwd, kFbXfused, kFbYinv, kFbZc2r };
+
+#define FPM_BCOL_LENGTHS(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+#define FPM_BZ_LENGTHS(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+template <typename T, int N, int BZ, int NT, bool INV, bool KOP, bool MC, bool FWD>
+__global__ __launch_bounds__(NT) void k_batch_fft_col(const ColArgs<T>* __restrict__ tab,
+                                                       const int64_t* __restrict__ pre,
+                                                       const int64_t* __restrict__ cnt, int nb,
+                                                       int64_t total) {
+  const int64_t it = blockIdx.x;
+  if (it >= total) return;
+  const int b = batch_box(pre, nb, it);
+  const int64_t j = it - pre[b];
+  if (j >= cnt[b]) return;
+  fft_col_body<T, N, BZ, NT, INV, KOP, MC, FWD>(tab[b], j);
+}
+
+template <typename T, int M, int NL, int NT, bool INV>
+__global__ __launch_bounds__(NT) void k_batch_fft_z(const ZArgs<T>* __restrict__ tab,
+                                                     const int64_t* __restrict__ pre,
+                                                     const int64_t* __restrict__ cnt, int nb,
+                                                     int64_t total) {
+  const int64_t it = blockIdx.x;
+  if (it >= total) return;
+  const int b = batch_box(pre, nb, it);
+  const int64_t j = it - pre[b];
+  if (j >= cnt[b]) return;
+  const ZArgs<T> a = tab[b];  // a local copy: the passes' stores cannot alias it
+  if constexpr (INV) fft_z_c2r_body<T, M, NL, NT>(a, j);
+  else fft_z_r2c_body<T, M, NL, NT>(a, j);
+}
+
+template <typename T, int N, bool INV, bool KOP, bool FWD, int WIDE>
+static void batch_col_nw(const BatchLaunch& L, hipStream_t st) {
+  constexpr int BZ = ColW<T, N, WIDE>::BZ;
+  constexpr int NT = threads_for<T>(N * BZ);
+  hipLaunchKernelGGL((k_batch_fft_col<T, N, BZ, NT, INV, KOP, false, FWD>),
+                     dim3(batch_grid(L.total)), dim3(NT), 0, st,
+                     static_cast<const ColArgs<T>*>(L.tab), L.pre, L.cnt, L.nb, L.total);
+}
+
+template <typename T, int M, bool INV, int ZV>
+static void batch_z_mv(const BatchLaunch& L, hipStream_t st) {
+  constexpr int NL = z_lines(M, ZV);
+  constexpr int NT = clampi(round64(NL * M / FPM_Z_EPT), 64, 1024);
+  hipLaunchKernelGGL((k_batch_fft_z<T, M, NL, NT, INV>), dim3(batch_grid(L.total)), dim3(NT), 0,
+                     st, static_cast<const ZArgs<T>*>(L.tab), L.pre, L.cnt, L.nb, L.total);
+}
+
+// column tile width (z columns) of an axis of length n with tiling `wide`
+template <typename T>
+static int col_bz(int n, int wide) {
+  switch (n) {
+#define FPM_CASE(N_) \
+  case N_: return wide ? ColW<T, N_, 1>::BZ : ColW<T, N_, 0>::BZ;
+    FPM_BCOL_LENGTHS(FPM_CASE)
+#undef FPM_CASE
+    default: return 0;
+  }
+}
+
+static bool batch_col_len(int n) {
+  switch (n) {
+#define FPM_CASE(N_) case N_:
+    FPM_BCOL_LENGTHS(FPM_CASE)
+#undef FPM_CASE
+    return true;
+    default: return false;
+  }
+}
+
+static bool batch_z_len(int m) {
+  switch (m) {
+#define FPM_CASE(N_) case N_:
+    FPM_BZ_LENGTHS(FPM_CASE)
+#undef FPM_CASE
+    return true;
+    default: return false;
+  }
+}
+
+int fft_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
+                    std::vector<BatchLaunch>& out) {
+  using T = float;
+  std::vector<ZArgs<T>> zf(nbox), zi(nbox);
+  std::vector<ColArgs<T>> yf(nbox), xf(nbox), yi(nbox);
+  std::vector<BatchKey> kz(nbox), ky(nbox), kx(nbox);
+  for (int i = 0; i < nbox; ++i) {
+    const fpm_batch_box& bx = boxes[i];
+    const fpm_mesh* m = &bx.mesh;
+    int rc = check_mesh(m, bx.pitch2);
+    if (rc) return rc;
+    if (!bx.rho || !bx.phi || !bx.work || !bx.tw[0] || !bx.tw[1] || !bx.tw[2]) return FPM_ERR_ARG;
+    const int n0 = m->nmesh[0], n1 = m->nmesh[1], n2 = m->nmesh[2];
+    if (!batch_col_len(n0) || !batch_col_len(n1) || !batch_z_len(n2 / 2))
+      return FPM_ERR_UNSUPPORTED;
+    const int nh = n2 / 2 + 1;
+    C<T>* wk = static_cast<C<T>*>(bx.work);
+    const int zv = FPM_VAR_FIELD(m->variant, FPM_VAR_Z_LINES);
+    ZArgs<T> z{};
+    z.rin = static_cast<const T*>(bx.rho);
+    z.cout = wk;
+    z.nrows = (int64_t)n0 * n1;
+    z.pitch2 = bx.pitch2;
+    z.tw = static_cast<const C<T>*>(bx.tw[2]);
+    zf[i] = z;
+    ZArgs<T> zc{};
+    zc.cin = wk;
+    zc.rout = static_cast<T*>(bx.phi);
+    zc.nrows = (int64_t)n0 * n1;
+    zc.pitch2 = bx.pitch2;
+    zc.tw = static_cast<const C<T>*>(bx.tw[2]);
+    zi[i] = zc;
+    kz[i] = BatchKey{n2 / 2, zv, 0, 0};
+    const int wy = col_choice(m, (int)sizeof(T), n1), wx = col_choice(m, (int)sizeof(T), n0);
+    ColArgs<T> y{};
+    y.in = wk;
+    y.out[0] = wk;
+    y.ncomp = 1;
+    y.estride = bx.pitch2;
+    y.ostride = (int64_t)n1 * bx.pitch2;
+    y.nouter = n0;
+    y.ncols = nh;
+    y.scale = (T)1;
+    y.tw = static_cast<const C<T>*>(bx.tw[1]);
+    const int bzy = col_bz<T>(n1, wy);
+    y.ntz = (nh + bzy - 1) / bzy;
+    y.nblk = (int64_t)y.ntz * y.nouter;
+    yf[i] = y;
+    yi[i] = y;
+    ky[i] = BatchKey{n1, wy, 0, 0};
+    ColArgs<T> x{};
+    x.in = wk;
+    x.kout = nullptr;
+    x.kscale = (T)(1.0 / ((double)n0 * n1 * n2));
+    x.out[0] = wk;
+    x.ncomp = 1;
+    x.estride = (int64_t)n1 * bx.pitch2;
+    x.ostride = bx.pitch2;
+    x.nouter = n1;
+    x.ncols = nh;
+    x.scale = (T)1;
+    x.tw = static_cast<const C<T>*>(bx.tw[0]);
+    x.eaxis = 0;
+    x.oaxis = 1;
+    if ((rc = fill_kop(x.kop, 1, &bx.common, m))) return rc;
+    const int bzx = col_bz<T>(n0, wx);
+    x.ntz = (nh + bzx - 1) / bzx;
+    x.nblk = (int64_t)x.ntz * x.nouter;
+    xf[i] = x;
+    kx[i] = BatchKey{n0, wx, 0, 0};
+  }
+  // per-box keys and item counts, found by the element's place in its vector
+  auto zfkey = [&](const ZArgs<T>& a) { return kz[&a - zf.data()]; };
+  auto zikey = [&](const ZArgs<T>& a) { return kz[&a - zi.data()]; };
+  auto zitems = [](const BatchKey& k, const ZArgs<T>& a) {
+    const int nl = z_lines(k.a, k.b);
+    return (a.nrows + nl - 1) / nl;
+  };
+  auto zfitems = [&](const ZArgs<T>& a) { return zitems(zfkey(a), a); };
+  auto ziitems = [&](const ZArgs<T>& a) { return zitems(zikey(a), a); };
+  auto citems = [](const ColArgs<T>& a) { return a.nblk; };
+  int rc;
+  if ((rc = batch_groups(zf, kFbZr2c, zfkey, zfitems, arena, out))) return rc;
+  if ((rc = batch_groups(yf, kFbYfwd, [&](const ColArgs<T>& a) { return ky[&a - yf.data()]; },
+                         citems, arena, out)))
+    return rc;
+  if ((rc = batch_groups(xf, kFbXfused, [&](const ColArgs<T>& a) { return kx[&a - xf.data()]; },
+                         citems, arena, out)))
+    return rc;
+  if ((rc = batch_groups(yi, kFbYinv, [&](const ColArgs<T>& a) { return ky[&a - yi.data()]; },
+                         citems, arena, out)))
+    return rc;
+  if ((rc = batch_groups(zi, kFbZc2r, zikey, ziitems, arena, out))) return rc;
+  return FPM_OK;
+}
+
+int fft_batch_run(const std::vector<BatchLaunch>& ls, hipStream_t st) {
+  using T = float;
+  for (const BatchLaunch& L : ls) {
+    bool ok = true;
+    switch (L.kind) {
+      case kFbZr2c:
+      case kFbZc2r: {
+        const bool inv = L.kind == kFbZc2r;
+        switch (L.a * 8 + L.b * 2 + (inv ? 1 : 0)) {
+#define FPM_CASE(M_)                                                             \
+  case M_ * 8 + 0: batch_z_mv<T, M_, false, 0>(L, st); break;                     \
+  case M_ * 8 + 1: batch_z_mv<T, M_, true, 0>(L, st); break;                      \
+  case M_ * 8 + 2: batch_z_mv<T, M_, false, 1>(L, st); break;                     \
+  case M_ * 8 + 3: batch_z_mv<T, M_, true, 1>(L, st); break;                      \
+  case M_ * 8 + 4: batch_z_mv<T, M_, false, 2>(L, st); break;                     \
+  case M_ * 8 + 5: batch_z_mv<T, M_, true, 2>(L, st); break;
+          FPM_BZ_LENGTHS(FPM_CASE)
+#undef FPM_CASE
+          default: ok = false;
+        }
+        break;
+      }
+      case kFbYfwd:
+      case kFbXfused:
+      case kFbYinv: {
+        const int k = L.kind == kFbYfwd ? 0 : (L.kind == kFbXfused ? 1 : 2);
+        switch (L.a * 8 + L.b * 4 + k) {
+#define FPM_CASE(N_)                                                                     \
+  case N_ * 8 + 0: batch_col_nw<T, N_, false, false, false, 0>(L, st); break;            \
+  case N_ * 8 + 1: batch_col_nw<T, N_, true, true, true, 0>(L, st); break;               \
+  case N_ * 8 + 2: batch_col_nw<T, N_, true, false, false, 0>(L, st); break;             \
+  case N_ * 8 + 4: batch_col_nw<T, N_, false, false, false, 1>(L, st); break;            \
+  case N_ * 8 + 5: batch_col_nw<T, N_, true, true, true, 1>(L, st); break;               \
+  case N_ * 8 + 6: batch_col_nw<T, N_, true, false, false, 1>(L, st); break;
+          FPM_BCOL_LENGTHS(FPM_CASE)
+#undef FPM_CASE
+          default: ok = false;
+        }
+        break;
+      }
+      default: ok = false;
+    }
+    if (!ok) return FPM_ERR_UNSUPPORTED;
+    FPM_LAUNCH_CHECK();
   }
   return FPM_OK;
 }

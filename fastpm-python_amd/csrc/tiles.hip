@@ -20,7 +20,7 @@
 //   readout  : one workgroup per tile stages the (TILE + order - 1)^3 box of
 //              every requested mesh in LDS (each cell read once plus the
 //              1/TILE overhang) and interpolates for the tile's particles.
-#include "fpm_common.h"
+#include "batch.h"
 
 #include <cmath>
 
@@ -369,12 +369,16 @@ __device__ __forceinline__ void count_ranks_wg(const int32_t t[kBinILP], int64_t
   }
 }
 
+// Kernel bodies take their block index and count explicitly, so one body
+// serves a box's own launch (blockIdx.x of gridDim.x) and a batched launch
+// over many boxes (the box's local block of its share of the items).
 template <typename T, int ORDER>
-__global__ __launch_bounds__(kBinThreads) void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w,
-                                                           BinOrder o) {
+__device__ __forceinline__ void bin_count_body(const PartDesc<T>& P, const TileGeom& g,
+                                               const Workspace& w, const BinOrder& o, int64_t blk,
+                                               int64_t nblk) {
   __shared__ int32_t tkey[kBinTable], tval[kBinTable];
   const int64_t span = (int64_t)kBinThreads * kBinILP;
-  for (int64_t base = blockIdx.x * span; base < P.n; base += (int64_t)gridDim.x * span) {
+  for (int64_t base = blk * span; base < P.n; base += nblk * span) {
     int32_t t[kBinILP];
 #pragma unroll
     for (int u = 0; u < kBinILP; ++u) {
@@ -394,6 +398,12 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_count(PartDesc<T> P, TileGe
   }
 }
 
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kBinThreads) void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w,
+                                                           BinOrder o) {
+  bin_count_body<T, ORDER>(P, g, w, o, blockIdx.x, gridDim.x);
+}
+
 // Exclusive scan of the sub-bin counts in two passes over kScanBlock-bin
 // chunks.  Pass one scans each chunk locally (into offs) and records the
 // chunk total; pass two adds the sum of the earlier chunk totals, which every
@@ -407,10 +417,10 @@ __device__ __forceinline__ int32_t wave_incl_scan(int32_t v, int lane) {
   return v;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_bin_scan_local(Workspace w, int64_t nbins) {
+__device__ __forceinline__ void scan_local_body(const Workspace& w, int64_t nbins, int64_t blk) {
   __shared__ int32_t wsum[kScanThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int64_t i0 = (int64_t)blockIdx.x * kScanBlock + (int64_t)tid * kScanPer;
+  const int64_t i0 = blk * kScanBlock + (int64_t)tid * kScanPer;
   int32_t v[kScanPer];
   int32_t s = 0;
 #pragma unroll
@@ -428,14 +438,19 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan_local(Workspace w, in
     if (i0 + i < nbins) w.offs[i0 + i] = run;
     run += v[i];
   }
-  if (tid == kScanThreads - 1) w.btot[blockIdx.x] = run;
+  if (tid == kScanThreads - 1) w.btot[blk] = run;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int64_t nbins) {
+__global__ __launch_bounds__(kScanThreads) void k_bin_scan_local(Workspace w, int64_t nbins) {
+  scan_local_body(w, nbins, blockIdx.x);
+}
+
+__device__ __forceinline__ void scan_add_body(const Workspace& w, int64_t nbins, int64_t blk,
+                                              int64_t nblk) {
   __shared__ int32_t wsum[kScanThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int32_t s = 0;
-  for (int b = tid; b < (int)blockIdx.x; b += kScanThreads) s += w.btot[b];
+  for (int b = tid; b < (int)blk; b += kScanThreads) s += w.btot[b];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
   if (lane == 0) wsum[wid] = s;
@@ -443,14 +458,18 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int6
   int32_t base = 0;
 #pragma unroll
   for (int k = 0; k < kScanThreads / 64; ++k) base += wsum[k];
-  const int64_t c0 = (int64_t)blockIdx.x * kScanBlock;
+  const int64_t c0 = blk * kScanBlock;
   for (int i = tid; i < kScanBlock; i += kScanThreads) {
     const int64_t b = c0 + i;
     if (b < nbins) {
       w.offs[b] += base;
     }
   }
-  if (blockIdx.x == gridDim.x - 1 && tid == 0) w.offs[nbins] = base + w.btot[blockIdx.x];
+  if (blk == nblk - 1 && tid == 0) w.offs[nbins] = base + w.btot[blk];
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int64_t nbins) {
+  scan_add_body(w, nbins, blockIdx.x, gridDim.x);
 }
 
 // Each particle goes to the start of its sub-bin (recomputed from its
@@ -458,10 +477,11 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int6
 // record (window offsets and tile-local origin), so paint and readout read
 // their tile's particles contiguously and need no positions.
 template <typename T, int ORDER>
-__global__ __launch_bounds__(kBinThreads) void k_bin_scatter(PartDesc<T> P, TileGeom g,
-                                                             Workspace w, BinOrder o) {
+__device__ __forceinline__ void bin_scatter_body(const PartDesc<T>& P, const TileGeom& g,
+                                                 const Workspace& w, const BinOrder& o,
+                                                 int64_t blk, int64_t nblk) {
   const int64_t span = (int64_t)kBinThreads * kBinILP;
-  for (int64_t base = blockIdx.x * span; base < P.n; base += (int64_t)gridDim.x * span) {
+  for (int64_t base = blk * span; base < P.n; base += nblk * span) {
     int32_t slot[kBinILP];
     int32_t pp[kBinILP];
     PRec r[kBinILP];
@@ -502,6 +522,12 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_scatter(PartDesc<T> P, Tile
       }
     }
   }
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kBinThreads) void k_bin_scatter(PartDesc<T> P, TileGeom g,
+                                                             Workspace w, BinOrder o) {
+  bin_scatter_body<T, ORDER>(P, g, w, o, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- paint
@@ -588,16 +614,17 @@ __device__ __forceinline__ void gather_ranges(const TileGeom& g, const Workspace
 // The up to 64 (source tile, sub-bin) ranges feeding a tile are gathered
 // first (one lane each) and scanned, then all their particles are processed
 // as one flat index space, so their loads are issued together.
-template <typename T, int ORDER, int NT = kTileThreads>
-__global__ __launch_bounds__(NT) void k_paint_tiles(PartDesc<T> P, TileGeom g, Workspace w,
-                                                     T* mesh, float mass_fs, int sh,
-                                                     double inv_fx) {
+template <typename T, int ORDER, int NT>
+__device__ __forceinline__ void paint_tiles_body(const PartDesc<T>& P, const TileGeom& g,
+                                                 const Workspace& w, T* mesh, float mass_fs,
+                                                 int sh, double inv_fx, int64_t blk,
+                                                 int64_t nblk) {
   constexpr int NB = TILE * TILE * TILE;
   __shared__ unsigned long long acc[NB];
   __shared__ int32_t rstart[64];
   __shared__ int32_t rpre[65];
   __shared__ int32_t rdir[64];
-  const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t t = xcd_remap(blk, nblk);
   int lo[3];
   tile_coords(t, g, lo);
   for (int i = threadIdx.x; i < NB; i += NT) acc[i] = 0ull;
@@ -618,6 +645,13 @@ __global__ __launch_bounds__(NT) void k_paint_tiles(PartDesc<T> P, TileGeom g, W
     mesh[((int64_t)(lo[0] + x) * g.nm[1] + (lo[1] + y)) * g.nm[2] + (lo[2] + z)] =
         (T)((double)(long long)acc[i] * inv_fx);
   }
+}
+
+template <typename T, int ORDER, int NT = kTileThreads>
+__global__ __launch_bounds__(NT) void k_paint_tiles(PartDesc<T> P, TileGeom g, Workspace w,
+                                                     T* mesh, float mass_fs, int sh,
+                                                     double inv_fx) {
+  paint_tiles_body<T, ORDER, NT>(P, g, w, mesh, mass_fs, sh, inv_fx, blockIdx.x, gridDim.x);
 }
 
 // Weighted paint of NC per-particle weight arrays (the adjoint of a
@@ -1080,14 +1114,15 @@ __device__ __forceinline__ void grad_one(const T* rbox, const T* pbox, const Gra
   for (int d = 0; d < 3; ++d) G.f_out[p * G.f_stride + d] = f[d] * G.f_scale[d];
 }
 
-template <typename T, int ORDER, int NT = grad_threads<ORDER>()>
-__global__ __launch_bounds__(NT) void k_readout_grad_tiles(PartDesc<T> P, TileGeom g,
-                                                                      Workspace w, GradRead<T> G) {
+template <typename T, int ORDER, int NT>
+__device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const TileGeom& g,
+                                                  const Workspace& w, const GradRead<T>& G,
+                                                  int64_t blk, int64_t nblk) {
   constexpr int TR = TILE + ORDER - 1;
   constexpr int TP = TILE + ORDER + 3;
   __shared__ T rbox[TR * TR * TR];
   __shared__ T pbox[TP * TP * TP];
-  const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t t = xcd_remap(blk, nblk);
   int lo[3];
   tile_coords(t, g, lo);
   const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
@@ -1118,6 +1153,12 @@ __global__ __launch_bounds__(NT) void k_readout_grad_tiles(PartDesc<T> P, TileGe
     window_rec<T, ORDER>(w.rec[s], l, wt);
     grad_one<T, ORDER>(rbox, pbox, G, (int64_t)w.list[s], l, wt);
   }
+}
+
+template <typename T, int ORDER, int NT = grad_threads<ORDER>()>
+__global__ __launch_bounds__(NT) void k_readout_grad_tiles(PartDesc<T> P, TileGeom g,
+                                                                      Workspace w, GradRead<T> G) {
+  readout_grad_body<T, ORDER, NT>(P, g, w, G, blockIdx.x, gridDim.x);
 }
 
 // Position gradient of the stencil force readout (the window-derivative
@@ -1327,6 +1368,33 @@ static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, voi
   return bin_finish<T>(P, g, w, o, order, st);
 }
 
+// Threads per tile of the tiled paint on mesh m (FPM_VAR_PAINT).
+static int paint_nt(const fpm_mesh* m) {
+  switch (FPM_VAR_FIELD(m->variant, FPM_VAR_PAINT)) {
+    case 1: return 256;
+    case 2: return 1024;
+    default: return kTileThreads;
+  }
+}
+
+// Fixed-point scale of the tiled paint of n particles of `mass`: deposits are
+// summed as 64-bit integers in units of 2^-S, with the total |mass| of all
+// particles below 2^62, and each deposit is rounded to a multiple of 2^sh,
+// the shift that keeps |mass| 2^(S - sh) below 2^30 (a relative step of
+// 2^-30 of the mass), so that the rounding is a single-precision conversion.
+static int paint_scale(int64_t n, double mass, float& mass_fs, int& sh, double& inv_fx) {
+  if (!(std::fabs(mass) > 0) || !std::isfinite(mass)) return FPM_ERR_ARG;
+  const double total = std::fabs(mass) * (double)(n > 0 ? n : 1);
+  int S = 62 - (int)std::ceil(std::log2(total + 1.0));
+  if (S > 60) S = 60;
+  int e = 0;
+  std::frexp(std::fabs(mass) * std::ldexp(1.0, S), &e);  // |mass| 2^S < 2^e
+  sh = e > 30 ? e - 30 : 0;
+  mass_fs = (float)(mass * std::ldexp(1.0, S - sh));
+  inv_fx = std::ldexp(1.0, -S);
+  return FPM_OK;
+}
+
 template <typename T>
 static int paint_tiles_impl(const fpm_particles* part, const fpm_mesh* m, int order, double mass,
                             void* out, void* ws, int64_t ws_bytes, hipStream_t st) {
@@ -1335,28 +1403,21 @@ static int paint_tiles_impl(const fpm_particles* part, const fpm_mesh* m, int or
   Workspace w;
   int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
   if (rc) return rc;
-  if (!out || !(std::fabs(mass) > 0) || !std::isfinite(mass)) return FPM_ERR_ARG;
-  // 64-bit fixed point: the total |mass| of all particles stays below 2^62.
-  const double total = std::fabs(mass) * (double)(P.n > 0 ? P.n : 1);
-  int S = 62 - (int)std::ceil(std::log2(total + 1.0));
-  if (S > 60) S = 60;
-  // a deposit is rounded to a multiple of 2^sh, the shift that keeps
-  // |mass| * 2^(S - sh) below 2^30 (a relative step of 2^-30 of the mass)
-  int e = 0;
-  std::frexp(std::fabs(mass) * std::ldexp(1.0, S), &e);  // |mass| 2^S < 2^e
-  const int sh = e > 30 ? e - 30 : 0;
-  const float mass_fs = (float)(mass * std::ldexp(1.0, S - sh));
-  const double inv_fx = std::ldexp(1.0, -S);
+  if (!out) return FPM_ERR_ARG;
+  float mass_fs;
+  int sh;
+  double inv_fx;
+  if ((rc = paint_scale(P.n, mass, mass_fs, sh, inv_fx))) return rc;
   T* mesh = static_cast<T*>(out);
   const dim3 grid((unsigned)g.ntiles);
-  switch (FPM_VAR_FIELD(m->variant, FPM_VAR_PAINT) * 4 + order) {
-#define FPM_PAINT(V_, O_, NT_)                                                                  \
-  case V_ * 4 + O_:                                                                             \
+  switch (paint_nt(m) * 4 + order) {
+#define FPM_PAINT(O_, NT_)                                                                      \
+  case NT_ * 4 + O_:                                                                            \
     hipLaunchKernelGGL((k_paint_tiles<T, O_, NT_>), grid, dim3(NT_), 0, st, P, g, w, mesh,       \
                        mass_fs, sh, inv_fx);                                                    \
     break;
-    FPM_PAINT(1, 2, 256) FPM_PAINT(1, 3, 256) FPM_PAINT(2, 2, 1024) FPM_PAINT(2, 3, 1024)
-    FPM_PAINT(0, 2, kTileThreads) FPM_PAINT(0, 3, kTileThreads)
+    FPM_PAINT(2, 256) FPM_PAINT(3, 256) FPM_PAINT(2, 512) FPM_PAINT(3, 512)
+    FPM_PAINT(2, 1024) FPM_PAINT(3, 1024)
 #undef FPM_PAINT
     default: return FPM_ERR_ARG;
   }
@@ -1442,6 +1503,40 @@ static int paint_tiles_w_impl(const fpm_particles* part, const fpm_mesh* m, int 
   return FPM_OK;
 }
 
+// Arguments of the stencil force readout (the force factor carries the
+// stencil's 1 / (12 h_d)).
+template <typename T>
+static int grad_setup(const fpm_mesh* m, int order, int64_t n, const void* rho, const void* phi,
+                      void* rho_out, int64_t rho_stride, void* f_out, int64_t f_stride,
+                      double rho_scale, double f_scale, GradRead<T>& G) {
+  if (!phi || (n > 0 && !f_out) || f_stride < 3 || (rho_out && (!rho || rho_stride < 1)))
+    return FPM_ERR_ARG;
+  // the potential box reaches two cells past the density box on each side
+  for (int d = 0; d < 3; ++d)
+    if (m->nmesh[d] < TILE + order + 3) return FPM_ERR_UNSUPPORTED;
+  G = GradRead<T>{};
+  G.rho = static_cast<const T*>(rho);
+  G.phi = static_cast<const T*>(phi);
+  G.rho_out = static_cast<T*>(rho_out);
+  G.f_out = static_cast<T*>(f_out);
+  G.rho_stride = rho_stride;
+  G.f_stride = f_stride;
+  G.rho_scale = (T)rho_scale;
+  for (int d = 0; d < 3; ++d)
+    G.f_scale[d] = (T)(f_scale * (double)m->nmesh[d] / (12.0 * m->box[d]));
+  return FPM_OK;
+}
+
+// Threads per tile of the stencil readout on mesh m (FPM_VAR_READOUT).
+static int grad_nt(const fpm_mesh* m, int order) {
+  switch (FPM_VAR_FIELD(m->variant, FPM_VAR_READOUT)) {
+    case 1: return 256;
+    case 2: return 512;
+    case 3: return 1024;
+    default: return order == 2 ? grad_threads<2>() : grad_threads<3>();
+  }
+}
+
 template <typename T>
 static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int order,
                              const void* rho, const void* phi, void* rho_out, int64_t rho_stride,
@@ -1452,30 +1547,13 @@ static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int o
   Workspace w;
   int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
   if (rc) return rc;
-  if (!phi || (P.n > 0 && !f_out) || f_stride < 3 || (rho_out && (!rho || rho_stride < 1)))
-    return FPM_ERR_ARG;
-  // the potential box reaches two cells past the density box on each side
-  for (int d = 0; d < 3; ++d)
-    if (m->nmesh[d] < TILE + order + 3) return FPM_ERR_UNSUPPORTED;
   GradRead<T> G{};
-  G.rho = static_cast<const T*>(rho);
-  G.phi = static_cast<const T*>(phi);
-  G.rho_out = static_cast<T*>(rho_out);
-  G.f_out = static_cast<T*>(f_out);
-  G.rho_stride = rho_stride;
-  G.f_stride = f_stride;
-  G.rho_scale = (T)rho_scale;
-  for (int d = 0; d < 3; ++d)
-    G.f_scale[d] = (T)(f_scale * (double)m->nmesh[d] / (12.0 * m->box[d]));
+  if ((rc = grad_setup<T>(m, order, P.n, rho, phi, rho_out, rho_stride, f_out, f_stride,
+                          rho_scale, f_scale, G)))
+    return rc;
   if (P.n == 0) return FPM_OK;
   const dim3 grid((unsigned)g.ntiles);
-  int nt = 0;
-  switch (FPM_VAR_FIELD(m->variant, FPM_VAR_READOUT)) {
-    case 1: nt = 256; break;
-    case 2: nt = 512; break;
-    case 3: nt = 1024; break;
-    default: nt = order == 2 ? grad_threads<2>() : grad_threads<3>(); break;
-  }
+  const int nt = grad_nt(m, order);
   switch (nt * 4 + order) {
 #define FPM_GRAD(O_, NT_)                                                                       \
   case NT_ * 4 + O_:                                                                            \
@@ -1522,6 +1600,226 @@ static int pos_grad_impl(const fpm_particles* part, const fpm_mesh* m, int order
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }
+// ---------------------------------------------------------------- batched boxes
+// The binning, paint and stencil readout of many boxes, one launch per
+// compiled kernel: each box's arguments (as its own calls would set them up)
+// sit in a device table, and the workgroups of a launch stride the boxes'
+// items (count-array chunks, particle spans, scan blocks or tiles), one
+// workgroup each, through the bodies above.
+template <typename T>
+struct TileBox {
+  PartDesc<T> P;
+  TileGeom g;
+  Workspace w;
+  BinOrder o;
+  T* rho;
+  float mass_fs;
+  int sh;
+  double inv_fx;
+  GradRead<T> G;
+};
+
+enum { kTbClear = 1, kTbCount, kTbScanLocal, kTbScanAdd, kTbScatter, kTbPaint, kTbReadout };
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_batch_bin_clear(const TileBox<T>* __restrict__ tab,
+                                                         const int64_t* __restrict__ pre,
+                                                         const int64_t* __restrict__ cnt, int nb,
+                                                         int64_t total) {
+  const int64_t it = blockIdx.x;
+  if (it >= total) return;
+  const int b = batch_box(pre, nb, it);
+  const int64_t j = it - pre[b];
+  if (j >= cnt[b]) return;
+  const TileBox<T>& x = tab[b];
+  const int64_t i = j * 256 + threadIdx.x;
+  if (i < x.g.nbins) x.w.count[i] = 0;
+  if (j == 0 && threadIdx.x == 0) x.w.misplaced[0] = 0;
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kBinThreads) void k_batch_bin_count(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total) {
+  const int64_t it = blockIdx.x;
+  if (it >= total) return;
+  const int b = batch_box(pre, nb, it);
+  const int64_t j = it - pre[b];
+  if (j >= cnt[b]) return;
+  const TileBox<T>& x = tab[b];
+  bin_count_body<T, ORDER>(x.P, x.g, x.w, x.o, j, cnt[b]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_batch_scan_local(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total) {
+  const int64_t it = blockIdx.x;
+  if (it >= total) return;
+  const int b = batch_box(pre, nb, it);
+  const int64_t j = it - pre[b];
+  if (j >= cnt[b]) return;
+  scan_local_body(tab[b].w, tab[b].g.nbins, j);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_batch_scan_add(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total) {
+  const int64_t it = blockIdx.x;
+  if (it >= total) return;
+  const int b = batch_box(pre, nb, it);
+  const int64_t j = it - pre[b];
+  if (j >= cnt[b]) return;
+  scan_add_body(tab[b].w, tab[b].g.nbins, j, cnt[b]);
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kBinThreads) void k_batch_bin_scatter(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total) {
+  const int64_t it = blockIdx.x;
+  if (it >= total) return;
+  const int b = batch_box(pre, nb, it);
+  const int64_t j = it - pre[b];
+  if (j >= cnt[b]) return;
+  const TileBox<T>& x = tab[b];
+  bin_scatter_body<T, ORDER>(x.P, x.g, x.w, x.o, j, cnt[b]);
+}
+
+template <typename T, int ORDER, int NT>
+__global__ __launch_bounds__(NT) void k_batch_paint(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total) {
+  const int64_t it = blockIdx.x;
+  if (it >= total) return;
+  const int b = batch_box(pre, nb, it);
+  const int64_t j = it - pre[b];
+  if (j >= cnt[b]) return;
+  const TileBox<T>& x = tab[b];
+  paint_tiles_body<T, ORDER, NT>(x.P, x.g, x.w, x.rho, x.mass_fs, x.sh, x.inv_fx, j, cnt[b]);
+}
+
+template <typename T, int ORDER, int NT>
+__global__ __launch_bounds__(NT) void k_batch_readout(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total) {
+  const int64_t it = blockIdx.x;
+  if (it >= total) return;
+  const int b = batch_box(pre, nb, it);
+  const int64_t j = it - pre[b];
+  if (j >= cnt[b]) return;
+  const TileBox<T>& x = tab[b];
+  readout_grad_body<T, ORDER, NT>(x.P, x.g, x.w, x.G, j, cnt[b]);
+}
+
+int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
+                     std::vector<BatchLaunch>& bin, std::vector<BatchLaunch>& paint,
+                     std::vector<BatchLaunch>& readout) {
+  std::vector<TileBox<float>> args(nbox);
+  std::vector<int> order(nbox), pnt(nbox), rnt(nbox);
+  for (int i = 0; i < nbox; ++i) {
+    const fpm_batch_box& bx = boxes[i];
+    TileBox<float>& x = args[i];
+    x = TileBox<float>{};
+    int rc = prep<float>(&bx.part, &bx.mesh, bx.order, bx.ws, bx.ws_bytes, x.P, x.g, x.w);
+    if (rc) return rc;
+    if (!bx.rho) return FPM_ERR_ARG;
+    x.o = bin_order(&bx.part, &bx.mesh);
+    x.rho = static_cast<float*>(bx.rho);
+    if ((rc = paint_scale(x.P.n, bx.mass, x.mass_fs, x.sh, x.inv_fx))) return rc;
+    if ((rc = grad_setup<float>(&bx.mesh, bx.order, x.P.n, bx.rho, bx.phi, bx.rho_out,
+                                bx.rho_stride, bx.force_out, bx.force_stride, bx.rho_scale,
+                                bx.force_scale, x.G)))
+      return rc;
+    order[i] = bx.order;
+    pnt[i] = paint_nt(&bx.mesh);
+    rnt[i] = grad_nt(&bx.mesh, bx.order);
+  }
+  // keys carry the box index through the grouping lambdas
+  auto idx = [&](const TileBox<float>& x) { return (int)(&x - args.data()); };
+  const int64_t span = (int64_t)kBinThreads * kBinILP;
+  int rc;
+  auto one = [&](const TileBox<float>&) { return BatchKey{0, 0, 0, 0}; };
+  auto by_order = [&](const TileBox<float>& x) { return BatchKey{order[idx(x)], 0, 0, 0}; };
+  if ((rc = batch_groups(args, kTbClear, one,
+                         [&](const TileBox<float>& x) { return (x.g.nbins + 255) / 256; }, arena,
+                         bin)))
+    return rc;
+  auto spans = [&](const TileBox<float>& x) { return (x.P.n + span - 1) / span; };
+  if ((rc = batch_groups(args, kTbCount, by_order, spans, arena, bin))) return rc;
+  auto sblk = [&](const TileBox<float>& x) { return scan_blocks(x.g); };
+  if ((rc = batch_groups(args, kTbScanLocal, one, sblk, arena, bin))) return rc;
+  if ((rc = batch_groups(args, kTbScanAdd, one, sblk, arena, bin))) return rc;
+  if ((rc = batch_groups(args, kTbScatter, by_order, spans, arena, bin))) return rc;
+  if ((rc = batch_groups(
+           args, kTbPaint,
+           [&](const TileBox<float>& x) { return BatchKey{order[idx(x)], pnt[idx(x)], 0, 0}; },
+           [&](const TileBox<float>& x) { return x.g.ntiles; }, arena, paint)))
+    return rc;
+  if ((rc = batch_groups(
+           args, kTbReadout,
+           [&](const TileBox<float>& x) { return BatchKey{order[idx(x)], rnt[idx(x)], 0, 0}; },
+           [&](const TileBox<float>& x) { return x.P.n > 0 ? x.g.ntiles : (int64_t)0; }, arena,
+           readout)))
+    return rc;
+  return FPM_OK;
+}
+
+int tile_batch_run(const std::vector<BatchLaunch>& ls, hipStream_t st) {
+  for (const BatchLaunch& L : ls) {
+    const auto* tab = static_cast<const TileBox<float>*>(L.tab);
+    const dim3 grid(batch_grid(L.total));
+    switch (L.kind) {
+      case kTbClear:
+        hipLaunchKernelGGL(k_batch_bin_clear<float>, grid, dim3(256), 0, st, tab, L.pre, L.cnt,
+                           L.nb, L.total);
+        break;
+      case kTbCount:
+        if (L.a == 2)
+          hipLaunchKernelGGL((k_batch_bin_count<float, 2>), grid, dim3(kBinThreads), 0, st, tab,
+                             L.pre, L.cnt, L.nb, L.total);
+        else
+          hipLaunchKernelGGL((k_batch_bin_count<float, 3>), grid, dim3(kBinThreads), 0, st, tab,
+                             L.pre, L.cnt, L.nb, L.total);
+        break;
+      case kTbScanLocal:
+        hipLaunchKernelGGL(k_batch_scan_local<float>, grid, dim3(kScanThreads), 0, st, tab, L.pre,
+                           L.cnt, L.nb, L.total);
+        break;
+      case kTbScanAdd:
+        hipLaunchKernelGGL(k_batch_scan_add<float>, grid, dim3(kScanThreads), 0, st, tab, L.pre,
+                           L.cnt, L.nb, L.total);
+        break;
+      case kTbScatter:
+        if (L.a == 2)
+          hipLaunchKernelGGL((k_batch_bin_scatter<float, 2>), grid, dim3(kBinThreads), 0, st, tab,
+                             L.pre, L.cnt, L.nb, L.total);
+        else
+          hipLaunchKernelGGL((k_batch_bin_scatter<float, 3>), grid, dim3(kBinThreads), 0, st, tab,
+                             L.pre, L.cnt, L.nb, L.total);
+        break;
+      case kTbPaint:
+        switch (L.b * 4 + L.a) {
+#define FPM_BPAINT(O_, NT_)                                                                    \
+  case NT_ * 4 + O_:                                                                           \
+    hipLaunchKernelGGL((k_batch_paint<float, O_, NT_>), grid, dim3(NT_), 0, st, tab, L.pre, L.cnt, \
+                       L.nb, L.total);                                                               \
+    break;
+          FPM_BPAINT(2, 256) FPM_BPAINT(3, 256) FPM_BPAINT(2, 512) FPM_BPAINT(3, 512)
+          FPM_BPAINT(2, 1024) FPM_BPAINT(3, 1024)
+#undef FPM_BPAINT
+          default: return FPM_ERR_ARG;
+        }
+        break;
+      case kTbReadout:
+        switch (L.b * 4 + L.a) {
+#define FPM_BREAD(O_, NT_)                                                                     \
+  case NT_ * 4 + O_:                                                                           \
+    hipLaunchKernelGGL((k_batch_readout<float, O_, NT_>), grid, dim3(NT_), 0, st, tab, L.pre,    \
+                       L.cnt, L.nb, L.total);                                                  \
+    break;
+          FPM_BREAD(2, 256) FPM_BREAD(3, 256) FPM_BREAD(2, 512) FPM_BREAD(3, 512)
+          FPM_BREAD(2, 1024) FPM_BREAD(3, 1024)
+#undef FPM_BREAD
+          default: return FPM_ERR_ARG;
+        }
+        break;
+      default: return FPM_ERR_ARG;
+    }
+    FPM_LAUNCH_CHECK();
+  }
+  return FPM_OK;
+}
+
 }  // namespace fpm
 
 using namespace fpm;

@@ -70,6 +70,39 @@ class FpmKop(ctypes.Structure):
     ]
 
 
+class FpmBatchBox(ctypes.Structure):
+    """fpm.h fpm_batch_box: one box of a batched step."""
+    _fields_ = [
+        ("part", FpmParticles),
+        ("mesh", FpmMesh),
+        ("order", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("mass", ctypes.c_double),
+        ("common", FpmKop),
+        ("pitch2", ctypes.c_int64),
+        ("rho", ctypes.c_void_p),
+        ("phi", ctypes.c_void_p),
+        ("work", ctypes.c_void_p),
+        ("tw", ctypes.c_void_p * 3),
+        ("ws", ctypes.c_void_p),
+        ("ws_bytes", ctypes.c_int64),
+        ("rho_out", ctypes.c_void_p),
+        ("rho_stride", ctypes.c_int64),
+        ("force_out", ctypes.c_void_p),
+        ("force_stride", ctypes.c_int64),
+        ("rho_scale", ctypes.c_double),
+        ("force_scale", ctypes.c_double),
+        ("P", ctypes.c_void_p),
+        ("table", ctypes.c_void_p),
+        ("rows", ctypes.c_int32),
+        ("stride", ctypes.c_int32),
+        ("close", ctypes.c_int32),
+        ("open", ctypes.c_int32),
+        ("drift", ctypes.c_int32),
+        ("pad2_", ctypes.c_int32),
+    ]
+
+
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
@@ -133,6 +166,10 @@ _PLAIN = {
     "fpm_window_grad_tiled_grad4_f32": (_int, [_PP, _PM, _int, _vp, _vp, _i64, _vp, _dbl, _int,
                                                _vp, _i64, _vp]),
     "fpm_counter_add": (_int, [_vp, ctypes.c_int32, _vp]),
+    "fpm_batch_device_bytes": (_i64, [_int, _vp]),
+    "fpm_batch_create": (_vp, [_int, _vp, _vp, _i64, ctypes.POINTER(ctypes.c_int)]),
+    "fpm_batch_step": (_int, [_vp, _vp, _vp]),
+    "fpm_batch_destroy": (None, [_vp]),
 }
 
 

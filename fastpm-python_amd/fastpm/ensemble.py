@@ -6,12 +6,21 @@ kick/drift coefficients of every step are precomputed in FP64 into a device
 table per box and read through a device step counter, so one step of all
 boxes is captured once as a graph and replayed for every step.  The replayed
 unit is a step boundary, as in Solver.nbody: the previous step's closing
-kick, this step's opening kick and drift in one pass
-(`fpm_kick_drift_table`), then the force; the last closing kick follows the
-final replay.  Paint, transforms and readout are the same kernels
-the single-box Solver runs (the tiled, deterministic paint on meshes that are
-multiples of 16), so a replayed ensemble reproduces the per-box eager runs
-bit for bit.
+kick, this step's opening kick and drift in one pass, then the force; the
+last closing kick follows the final replay.
+
+With `batched` (the default when every box takes the float32 stencil force)
+the boxes are split into `nplans` groups of about equal mesh volume, and one
+step of a group is a handful of launches (`fpm_batch_step`): one per
+compiled kernel, each covering every box of the group that uses it, one
+workgroup per tile, span or row group of each box, through a device table of
+per-box arguments built once (`fpm_batch_create`).  The groups run on
+parallel streams (branches of the captured graph), so one group's
+latency-bound passes overlap another's bandwidth-bound ones.  Otherwise each
+box enqueues its own kernels on one of `nstreams` streams.  Either way the per-box arithmetic
+is the single-box Solver's (the tiled, deterministic paint, the fused
+transform, the stencil readout), so a replayed ensemble reproduces the
+per-box eager runs bit for bit.
 
 The reference has no ensemble driver; this is the batched configuration of
 the benchmark (64 boxes of {32, 48, 64, 96, 128}^3 particles, 20 steps).
@@ -40,7 +49,7 @@ class Box:
 
 class Ensemble:
     def __init__(self, specs, stages, dtype="f4", B=2, resampler="cic", compensate=True,
-                 nstreams=16, autotune=True):
+                 nstreams=16, autotune=True, batched=True, nplans=2):
         """specs: iterable of dict(nc=, box=, cosmo=, seed=).  Boxes are spread
         over `nstreams` streams (largest first, each to the least loaded
         stream) so small boxes run concurrently; inside a captured graph the
@@ -49,6 +58,9 @@ class Ensemble:
         variants per force-mesh shape before the first step (fastpm.tune)."""
         self.stages = np.asarray(stages, dtype="f8")
         self.autotune = bool(autotune)
+        self.batched = bool(batched)
+        self.nplans = max(1, int(nplans))
+        self.plans = []
         self.boxes = [Box(s["nc"], s["box"], s["cosmo"], s["seed"], dtype, B, resampler,
                           compensate) for s in specs]
         self.nsteps = len(self.stages) - 1
@@ -97,6 +109,73 @@ class Ensemble:
             b.rows = rows
             b.table = torch.tensor(np.array(rows, dtype="f8"), device=self.counter.device)
         self.counter.zero_()
+        if self.batched and self.nsteps > 0 and all(b.step.stencil for b in self.boxes):
+            # groups of about equal mesh volume, largest boxes first
+            k = min(self.nplans, len(self.boxes), len(self.streams))
+            groups, load = [[] for _ in range(k)], [0] * k
+            for b in sorted(self.boxes, key=lambda b: -int(b.solver.fpm.Nmesh.prod())):
+                i = int(np.argmin(load))
+                groups[i].append(b)
+                load[i] += int(b.solver.fpm.Nmesh.prod())
+            self.plans = [self._build_plan(g) for g in groups if g]
+
+    @property
+    def plan(self):
+        """The first batch plan (None when the boxes run one by one)."""
+        return self.plans[0][0] if self.plans else None
+
+    def _build_plan(self, group):
+        """A batch plan of the boxes in `group`: every box's buffers, force
+        multiplier and coefficient table in one fpm_batch_box array; returns
+        (plan, its device table memory)."""
+        import ctypes
+
+        n = len(group)
+        boxes = (_lib.FpmBatchBox * n)()
+        for i, b in enumerate(group):
+            s, step, fpm = b.state, b.step, b.solver.fpm
+            ws = step.workspace()
+            parts = s.particles
+            bx = boxes[i]
+            bx.part = parts.desc(fpm.dtype)
+            bx.mesh = fpm._desc
+            bx.order = fpm.order
+            # the paint mass of FastPMStep.prepare_force, the same expression
+            nbar = 1.0 * s.csize / float(np.prod(fpm.Nmesh))
+            bx.mass = 1.0 / nbar
+            bx.common = step.force_common().fpm_kop()
+            bx.pitch2 = fpm.pitch2
+            bx.rho = ws["rho"].value.data_ptr()
+            bx.phi = ws["force"][0].value.data_ptr()
+            bx.work = ws["work"][0].data_ptr()
+            for d, t in enumerate(fpm.twiddles()):
+                bx.tw[d] = t.data_ptr()
+            tws = fpm.tile_workspace(len(parts), fpm.order)
+            bx.ws, bx.ws_bytes = tws.data_ptr(), tws.numel()
+            bx.rho_out, bx.rho_stride = s.RHO.data_ptr(), 1
+            bx.force_out, bx.force_stride = s.F.data_ptr(), 3
+            bx.rho_scale = bx.force_scale = 1.0
+            bx.P = s.P.data_ptr()
+            bx.table = b.table.data_ptr()
+            bx.rows, bx.stride, bx.close, bx.open, bx.drift = self.nsteps, 3, 2, 0, 1
+        L = _lib.lib()
+        nbytes = int(L.fpm_batch_device_bytes(n, boxes))
+        if nbytes < 0:
+            raise RuntimeError("fpm_batch_device_bytes failed: %s" % L.fpm_strerror(nbytes).decode())
+        mem = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.counter.device)
+        st = ctypes.c_int(0)
+        plan = L.fpm_batch_create(n, boxes, mem.data_ptr(), mem.numel(), ctypes.byref(st))
+        if not plan:
+            raise RuntimeError("fpm_batch_create failed: %s" % L.fpm_strerror(st.value).decode())
+        return plan, mem
+
+    def __del__(self):
+        plans, self.plans = getattr(self, "plans", []), []
+        for plan, _ in plans:
+            try:
+                _lib.lib().fpm_batch_destroy(plan)
+            except Exception:  # interpreter shutdown: the library may be gone
+                pass
 
     def _enqueue_box(self, b):
         st = _stream()
@@ -110,9 +189,19 @@ class Ensemble:
         b.step.force_kernels(s, keep_delta_k=False)
 
     def _enqueue_step(self):
-        """[K,] K, D, F of every box for the step held in the device counter:
-        fork to the box streams, join, then advance the counter."""
+        """[K,] K, D, F of every box for the step held in the device counter
+        (one batched step, or fork to the box streams and join), then advance
+        the counter."""
         main = torch.cuda.current_stream()
+        if self.plans:
+            for (plan, _), stream in zip(self.plans, self.streams):
+                stream.wait_stream(main)
+                with torch.cuda.stream(stream):
+                    _lib.call_plain("fpm_batch_step", plan, self.counter.data_ptr(), _stream())
+            for stream in self.streams[:len(self.plans)]:
+                main.wait_stream(stream)
+            _lib.call_plain("fpm_counter_add", self.counter.data_ptr(), 1, _stream())
+            return
         for lane, stream in zip(self.lanes, self.streams):
             stream.wait_stream(main)
             with torch.cuda.stream(stream):

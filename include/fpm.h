@@ -502,6 +502,70 @@ int fpm_lpt2_source_vjp_f32(const void* gsource, const void* const* phi, void* c
 int fpm_lpt2_source_vjp_f64(const void* gsource, const void* const* phi, void* const* gphi,
                             int64_t ncells, fpm_stream_t stream);
 
+/* ---- Batched boxes: one step boundary and force of many independent boxes
+ * in a few launches.  No counterpart in the reference, which runs one box
+ * per process; the per-box arithmetic is the single-box path's
+ * (fpm_kick_drift_table, fpm_bin_particles_f32, fpm_paint_tiled_f32,
+ * fpm_fft_r2c_c2r_f32 without the spectrum, fpm_readout_tiled_grad4_f32:
+ * the Kick, Kick, Drift and Force of fastpm/core.py:168-210), and the
+ * results equal those calls bit for bit.  Each launch covers every box that
+ * uses the same compiled kernel (the window order, the mesh's variant
+ * fields, the transform lengths); its workgroups stride the boxes' tiles,
+ * spans or rows in one index space through a table of per-box arguments in
+ * device memory.
+ *
+ * One box: float32 storage, a force mesh that is a multiple of 16 and at
+ * least 16 + order + 3 cells per side, lattice or explicit particles. */
+typedef struct fpm_batch_box {
+  fpm_particles part;      /* part.s is the displacement S, updated by the drift */
+  fpm_mesh mesh;           /* force mesh, with its variant word */
+  int32_t order;           /* 2 CIC, 3 TSC */
+  int32_t pad_;
+  double mass;             /* paint mass per particle (1 / nbar) */
+  fpm_kop common;          /* potential multiplier, e.g. 3/2 Om0 / k^2 [/ W^p] */
+  int64_t pitch2;
+  void* rho;               /* real mesh: the painted density */
+  void* phi;               /* real mesh: the potential */
+  void* work;              /* complex mesh of pitch2: the transform's work space */
+  const void* tw[3];       /* twiddle tables of the three axes */
+  void* ws;                /* tile workspace, fpm_tile_workspace_size bytes */
+  int64_t ws_bytes;
+  void* rho_out;           /* RHO [n] (stride rho_stride), or NULL */
+  int64_t rho_stride;
+  void* force_out;         /* F [n][3], stride force_stride (>= 3) per particle */
+  int64_t force_stride;
+  double rho_scale, force_scale;
+  /* the step boundary before the force, as fpm_kick_drift_table: S = part.s,
+   * P, F = force_out (force_stride 3), table [rows][stride] */
+  void* P;
+  const double* table;
+  int32_t rows, stride, close, open, drift, pad2_;
+} fpm_batch_box;
+
+typedef struct fpm_batch fpm_batch;  /* host-side plan (opaque) */
+
+/* Device bytes the plan of these boxes needs for its argument tables. */
+int64_t fpm_batch_device_bytes(int nbox, const fpm_batch_box* boxes);
+
+/* Builds the plan: checks every box, writes the argument tables to `dev`
+ * (fpm_batch_device_bytes bytes of device memory, owned by the caller and
+ * kept alive and unchanged while the plan is used) and returns the host-side
+ * plan, or NULL with *status set.  A set-up call: it copies to the device
+ * synchronously.  The boxes' buffers must stay where they are. */
+fpm_batch* fpm_batch_create(int nbox, const fpm_batch_box* boxes, void* dev, int64_t dev_bytes,
+                            int* status);
+
+/* Enqueues, for every box: with counter != NULL the step boundary
+ * P += table[row-1][close] F (skipped for row 0), P += table[row][open] F,
+ * S += table[row][drift] P (row = counter[0]; a row outside [0, rows)
+ * leaves S and P unchanged), then the force: binning, paint of rho (mass
+ * per particle), phi = c2r(common * r2c(rho)), RHO = rho_scale
+ * readout(rho) and F = force_scale readout(D phi) with the four-point
+ * gradient stencil.  Asynchronous, no allocation, graph-capture safe. */
+int fpm_batch_step(const fpm_batch* plan, const int32_t* counter, fpm_stream_t stream);
+
+void fpm_batch_destroy(fpm_batch* plan);
+
 /* Growth functions of a matter + Lambda (+ curvature) universe, host-side
  * double precision.  For each a[i] writes 9 values to out[9 i ...]:
  * D1, dD1/dlna, d2D1/dlna2, D2, dD2/dlna, d2D2/dlna2, E, dE/dlna, Omega_m(a),

@@ -510,9 +510,9 @@ int fpm_lpt2_source_vjp_f64(const void* gsource, const void* const* phi, void* c
  * the Kick, Kick, Drift and Force of fastpm/core.py:168-210), and the
  * results equal those calls bit for bit.  Each launch covers every box that
  * uses the same compiled kernel (the window order, the mesh's variant
- * fields, the transform lengths); its workgroups stride the boxes' tiles,
- * spans or rows in one index space through a table of per-box arguments in
- * device memory.
+ * fields, the transform lengths), with one workgroup per tile, particle span
+ * or row group of every box in one index space, each box's arguments read
+ * from a table in device memory.
  *
  * One box: float32 storage, a force mesh that is a multiple of 16 and at
  * least 16 + order + 3 cells per side, lattice or explicit particles. */

@@ -212,14 +212,18 @@ constexpr int pow2floor(int x) {
 }
 
 // Column tile width (z columns per tile): row segments of FPM_COL_BYTES, or
-// twice that when WIDE, capped so the tile stays within 96 KiB of LDS (the
-// rest holds twiddles and multiplier tables).  Both widths are compiled; the
-// one used per axis length is chosen at run time (fpm_fft_set_tiling, set by
-// the autotuner).  A column's arithmetic does not depend on the width (up to
-// the compiler's multiply-add contraction in each instantiation).
-template <typename T, int N, int WIDE = 0> struct ColW {
+// twice that when WIDE, capped so the tile stays within 96 KiB of LDS (144 KiB
+// for a single-output pass when WIDE: one workgroup per CU, so that
+// 1024-point columns still get 128-byte segments; the rest holds twiddles
+// and multiplier tables, six outputs' worth for MC).  Both
+// widths are compiled; the one used per mesh is chosen at run time (the
+// mesh's variant or fpm_fft_set_tiling, set by the autotuner).  A column's
+// arithmetic does not depend on the width (up to the compiler's multiply-add
+// contraction in each instantiation).
+template <typename T, int N, int WIDE = 0, bool MC = false> struct ColW {
   static constexpr int a = (FPM_COL_BYTES << WIDE) / (int)sizeof(C<T>);
-  static constexpr int b = pow2floor((96 * 1024 / (int)sizeof(C<T>)) / N);
+  static constexpr int b =
+      pow2floor(((WIDE && !MC ? 144 : 96) * 1024 / (int)sizeof(C<T>)) / N);
   static constexpr int BZ = a < b ? a : b;
 };
 
@@ -915,19 +919,21 @@ static bool supported_len(int n) {
   }
 }
 
-template <typename T, int N, bool INV, bool KOP, int WIDE>
-static void launch_col_nw(ColArgs<T> a, hipStream_t st) {
-  constexpr int BZ = ColW<T, N, WIDE>::BZ;
+template <typename T, int N, bool INV, bool KOP, bool MC, int WIDE>
+static void launch_col_nwm(ColArgs<T> a, hipStream_t st) {
+  constexpr int BZ = ColW<T, N, WIDE, MC>::BZ;
   constexpr int NT = threads_for<T>(N * BZ);
   a.ntz = (a.ncols + BZ - 1) / BZ;
   a.nblk = (int64_t)a.ntz * a.nouter;
   if (a.nblk <= 0) return;
-  if (KOP && a.ncomp > 1)
-    hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, KOP>), dim3((unsigned)a.nblk), dim3(NT),
-                       0, st, a);
-  else
-    hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, false>), dim3((unsigned)a.nblk),
-                       dim3(NT), 0, st, a);
+  hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, MC>), dim3((unsigned)a.nblk), dim3(NT),
+                     0, st, a);
+}
+
+template <typename T, int N, bool INV, bool KOP, int WIDE>
+static void launch_col_nw(ColArgs<T> a, hipStream_t st) {
+  if (KOP && a.ncomp > 1) launch_col_nwm<T, N, INV, KOP, KOP, WIDE>(a, st);
+  else launch_col_nwm<T, N, INV, KOP, false, WIDE>(a, st);
 }
 
 template <typename T, int N, bool INV, bool KOP>
@@ -936,19 +942,21 @@ static void launch_col_n(ColArgs<T> a, const fpm_mesh* m, hipStream_t st) {
   else launch_col_nw<T, N, INV, KOP, 0>(a, st);
 }
 
-template <typename T, int N, int WIDE>
-static void launch_col_fused_nw(ColArgs<T> a, hipStream_t st) {
-  constexpr int BZ = ColW<T, N, WIDE>::BZ;
+template <typename T, int N, bool MC, int WIDE>
+static void launch_col_fused_nwm(ColArgs<T> a, hipStream_t st) {
+  constexpr int BZ = ColW<T, N, WIDE, MC>::BZ;
   constexpr int NT = threads_for<T>(N * BZ);
   a.ntz = (a.ncols + BZ - 1) / BZ;
   a.nblk = (int64_t)a.ntz * a.nouter;
   if (a.nblk <= 0) return;
-  if (a.ncomp > 1)
-    hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, true, true, true, true>), dim3((unsigned)a.nblk),
-                       dim3(NT), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, true, true, false, true>), dim3((unsigned)a.nblk),
-                       dim3(NT), 0, st, a);
+  hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, true, true, MC, true>), dim3((unsigned)a.nblk),
+                     dim3(NT), 0, st, a);
+}
+
+template <typename T, int N, int WIDE>
+static void launch_col_fused_nw(ColArgs<T> a, hipStream_t st) {
+  if (a.ncomp > 1) launch_col_fused_nwm<T, N, true, WIDE>(a, st);
+  else launch_col_fused_nwm<T, N, false, WIDE>(a, st);
 }
 
 template <typename T, int N>

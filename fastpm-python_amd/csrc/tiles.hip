@@ -71,7 +71,7 @@ struct Workspace {
   PRec* rec;           // particle record per slot
   int32_t* count;
   int32_t* offs;       // nbins + 1
-  int32_t* misplaced;  // particles the scatter could not place in their sub-bin
+  int32_t* misplaced;  // particles whose slot the scatter found outside [0, n)
   int32_t* btot;       // per-workgroup totals of the sub-bin count scan
   int64_t bytes;
 };
@@ -494,12 +494,11 @@ __device__ __forceinline__ void bin_scatter_body(const PartDesc<T>& P, const Til
         int i0[3];
         T t[3];
         const int64_t p = bin_window<T, ORDER>(o, P, g.nm, i, i0, t);
-        const int32_t b = bin_of<ORDER>(i0, g);
-        const int32_t lo = w.offs[b];
-        // a slot outside the sub-bin's range from the count would leave some
-        // other slot unwritten: count it instead of storing it
-        if (rank >= 0 && rank < w.offs[b + 1] - lo && lo + rank < P.n) {
-          slot[u] = lo + rank;
+        const int32_t sl = w.offs[bin_of<ORDER>(i0, g)] + rank;
+        // a slot outside [0, n) (count and scatter disagreeing) is counted,
+        // not stored
+        if (sl >= 0 && sl < P.n) {
+          slot[u] = sl;
         } else {
           atomicAdd(w.misplaced, 1);
         }
@@ -512,8 +511,8 @@ __device__ __forceinline__ void bin_scatter_body(const PartDesc<T>& P, const Til
       }
     }
     // the count placed each particle with the same arithmetic (no FMA
-    // contraction in particle_cell), so its slot is in its sub-bin; the range
-    // check above counts any that is not, and stores stay in bounds
+    // contraction in particle_cell), so its slot is in its sub-bin; the guard
+    // above keeps the stores in bounds and counts any slot it drops
 #pragma unroll
     for (int u = 0; u < kBinILP; ++u) {
       if (slot[u] >= 0) {

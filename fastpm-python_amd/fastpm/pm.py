@@ -289,9 +289,9 @@ class ParticleMesh:
         return ws
 
     def tile_misplaced(self, n, order):
-        """Particles the last bin call's scatter could not store in their
-        sub-bin (0 unless the count and scatter passes disagreed; then the
-        tile lists are incomplete).  Synchronises."""
+        """Particles the last bin call's scatter could not store (a slot
+        outside [0, n): 0 unless the count and scatter passes disagreed; then
+        the tile lists are incomplete).  Synchronises."""
         off = int(_lib.lib().fpm_tile_misplaced_offset(self._desc, int(order), int(n)))
         ws = self._tile_ws
         return int(ws[off:off + 4].view(torch.int32).item())

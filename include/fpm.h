@@ -160,9 +160,10 @@ int fpm_readout_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, 
  * (fastpm/core.py:191,203; fastpm/force/gravity.py:24).
  * fpm_tile_misplaced_offset gives the byte offset in the workspace of an int32
  * counter, zeroed by each bin call, of particles the scatter could not store
- * in their sub-bin (a slot outside the range the count pass gave that
- * sub-bin); nonzero means the lists are incomplete and the tiled kernels
- * must not be used with them.  0 is the only valid outcome. */
+ * (a slot outside [0, n), which only a disagreement between the count and
+ * scatter passes could produce); nonzero means the lists are incomplete and
+ * the tiled kernels must not be used with them.  0 is the only valid
+ * outcome. */
 int fpm_tiles_supported(const fpm_mesh* mesh);
 int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n);
 int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n);

@@ -18,7 +18,12 @@ Every choice gives the same results (bitwise for the particle kernels, to
 rounding for the transform tilings).  `autotune` times them on the mesh by
 coordinate descent (the transform fields together on the Force's transform,
 then each particle-kernel field on the binning, paint and readout of a
-particle set), keeps the fastest, and writes it into the mesh descriptor.
+particle set) and writes the choice into the mesh descriptor.  A particle
+field keeps its default unless another choice is faster by MARGIN: the
+particles are timed as they are when tuning runs (usually the initial
+conditions), and the defaults are the choices measured best on the clustered
+late-time distribution (at a = 0.1 the binning walks differ by 1%, at a = 1
+blocks beat patches by 12%).
 Choices are cached per (precision, mesh shape, window order, lattice) for the
 process and, when FASTPM_AMD_TUNE_CACHE names a JSON file, across processes.
 """
@@ -31,6 +36,8 @@ import torch
 from . import _lib
 
 _CACHE = {}
+
+MARGIN = 0.05  # a particle field leaves its default only for a gain above this
 
 
 def _key(pm, order=None, parts=None):
@@ -177,7 +184,8 @@ def autotune_particles(pm, parts, order=None, reps=None, verbose=False):
 
     fields = [("bin", (0, 1, 2), bin_), ("paint", (0, 1, 2), paint)]
     if stencil:
-        fields.append(("readout", (1, 2, 3), readout))
+        # the default thread count of the readout as an explicit choice
+        fields.append(("readout", (3 if order == 2 else 2, 1, 2, 3), readout))
     for name, opts, fn in fields:
         t = {}
         for c in opts:
@@ -185,6 +193,8 @@ def autotune_particles(pm, parts, order=None, reps=None, verbose=False):
             bin_()
             t[c] = _time(fn, reps)
         c = min(t, key=t.get)
+        if t[c] > (1 - MARGIN) * t[opts[0]]:
+            c = opts[0]
         pm.set_variant(**{name: c})
         choices[name] = c
         times[name] = t

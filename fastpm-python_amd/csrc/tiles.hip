@@ -1063,10 +1063,33 @@ struct ScalarBox {
   }
 };
 
+// The density at tile-box coordinates (x, y, z) (box origin at the tile's
+// first cell): staged in LDS, or read from the mesh through the cache.
 template <typename T, int ORDER>
-__device__ __forceinline__ void grad_one(const T* rbox, const T* pbox, const GradRead<T>& G,
+struct RhoLds {
+  const T* rbox;
+  __device__ __forceinline__ T operator()(int x, int y, int z) const {
+    constexpr int TR = TILE + ORDER - 1;
+    return rbox[(x * TR + y) * TR + z];
+  }
+};
+
+template <typename T>
+struct RhoMesh {
+  const T* rho;
+  int lo[3], nm[3];
+  __device__ __forceinline__ T operator()(int x, int y, int z) const {
+    int g[3] = {lo[0] + x, lo[1] + y, lo[2] + z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (g[d] >= nm[d]) g[d] -= nm[d];
+    return rho[((int64_t)g[0] * nm[1] + g[1]) * nm[2] + g[2]];
+  }
+};
+
+template <typename T, int ORDER, typename RhoAt>
+__device__ __forceinline__ void grad_one(const RhoAt& rho_at, const T* pbox, const GradRead<T>& G,
                                          int64_t p, const int l[3], const T wt[3][ORDER]) {
-  constexpr int TR = TILE + ORDER - 1;   // density box side
   constexpr int TP = TILE + ORDER + 3;   // potential box side (2 more cells each way)
   constexpr int NLN = ORDER + 4;         // stencil line: window points -2 .. +2
   const int lx = l[0], ly = l[1], lz = l[2];
@@ -1077,9 +1100,8 @@ __device__ __forceinline__ void grad_one(const T* rbox, const T* pbox, const Gra
 #pragma unroll
       for (int b = 0; b < ORDER; ++b) {
         const T wab = wt[0][a] * wt[1][b];
-        const int row = ((lx + a) * TR + (ly + b)) * TR + lz;
 #pragma unroll
-        for (int c = 0; c < ORDER; ++c) acc += wab * wt[2][c] * rbox[row + c];
+        for (int c = 0; c < ORDER; ++c) acc += wab * wt[2][c] * rho_at(lx + a, ly + b, lz + c);
       }
     G.rho_out[p * G.rho_stride] = acc * G.rho_scale;
   }
@@ -1113,13 +1135,15 @@ __device__ __forceinline__ void grad_one(const T* rbox, const T* pbox, const Gra
   for (int d = 0; d < 3; ++d) G.f_out[p * G.f_stride + d] = f[d] * G.f_scale[d];
 }
 
-template <typename T, int ORDER, int NT>
+// RLDS: the density box staged in LDS (true) or the density read through
+// the cache (false: a third less LDS, the same values in the same order).
+template <typename T, int ORDER, int NT, bool RLDS = true>
 __device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const TileGeom& g,
                                                   const Workspace& w, const GradRead<T>& G,
                                                   int64_t blk, int64_t nblk) {
   constexpr int TR = TILE + ORDER - 1;
   constexpr int TP = TILE + ORDER + 3;
-  __shared__ T rbox[TR * TR * TR];
+  __shared__ T rbox[RLDS ? TR * TR * TR : 1];
   __shared__ T pbox[TP * TP * TP];
   const int64_t t = xcd_remap(blk, nblk);
   int lo[3];
@@ -1133,31 +1157,41 @@ __device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const Ti
   const PRec r1 = s1 < e ? w.rec[s1] : PRec{};
   ScalarBox<T, 2, ORDER + 1, TILE + ORDER - 1, NT> pb;
   pb.load(g, G.phi, lo);
-  ScalarBox<T, 0, ORDER - 1, 0, NT> rb;
-  if (G.rho_out) rb.load(g, G.rho, lo);
-  pb.store(pbox);
-  if (G.rho_out) rb.store(rbox);
+  if constexpr (RLDS) {
+    ScalarBox<T, 0, ORDER - 1, 0, NT> rb;
+    if (G.rho_out) rb.load(g, G.rho, lo);
+    pb.store(pbox);
+    if (G.rho_out) rb.store(rbox);
+  } else {
+    pb.store(pbox);
+  }
   __syncthreads();
+  const RhoLds<T, ORDER> rl{rbox};
+  const RhoMesh<T> rm{G.rho, {lo[0], lo[1], lo[2]}, {g.nm[0], g.nm[1], g.nm[2]}};
+  auto one = [&](int64_t p, const int l[3], const T wt[3][ORDER]) {
+    if constexpr (RLDS) grad_one<T, ORDER>(rl, pbox, G, p, l, wt);
+    else grad_one<T, ORDER>(rm, pbox, G, p, l, wt);
+  };
   int l[3];
   T wt[3][ORDER];
   if (s0 < e) {
     window_rec<T, ORDER>(r0, l, wt);
-    grad_one<T, ORDER>(rbox, pbox, G, p0, l, wt);
+    one(p0, l, wt);
   }
   if (s1 < e) {
     window_rec<T, ORDER>(r1, l, wt);
-    grad_one<T, ORDER>(rbox, pbox, G, p1, l, wt);
+    one(p1, l, wt);
   }
   for (int32_t s = s1 + NT; s < e; s += NT) {
     window_rec<T, ORDER>(w.rec[s], l, wt);
-    grad_one<T, ORDER>(rbox, pbox, G, (int64_t)w.list[s], l, wt);
+    one((int64_t)w.list[s], l, wt);
   }
 }
 
-template <typename T, int ORDER, int NT = grad_threads<ORDER>()>
+template <typename T, int ORDER, int NT = grad_threads<ORDER>(), bool RLDS = true>
 __global__ __launch_bounds__(NT) void k_readout_grad_tiles(PartDesc<T> P, TileGeom g,
                                                                       Workspace w, GradRead<T> G) {
-  readout_grad_body<T, ORDER, NT>(P, g, w, G, blockIdx.x, gridDim.x);
+  readout_grad_body<T, ORDER, NT, RLDS>(P, g, w, G, blockIdx.x, gridDim.x);
 }
 
 // Position gradient of the stencil force readout (the window-derivative
@@ -1526,6 +1560,12 @@ static int grad_setup(const fpm_mesh* m, int order, int64_t n, const void* rho, 
   return FPM_OK;
 }
 
+// Whether the stencil readout stages the density box in LDS (FPM_VAR_RHO 0)
+// or reads the density through the cache (1).
+static int grad_rho_lds(const fpm_mesh* m) {
+  return FPM_VAR_FIELD(m->variant, FPM_VAR_RHO) == 1 ? 0 : 1;
+}
+
 // Threads per tile of the stencil readout on mesh m (FPM_VAR_READOUT).
 static int grad_nt(const fpm_mesh* m, int order) {
   switch (FPM_VAR_FIELD(m->variant, FPM_VAR_READOUT)) {
@@ -1553,13 +1593,17 @@ static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int o
   if (P.n == 0) return FPM_OK;
   const dim3 grid((unsigned)g.ntiles);
   const int nt = grad_nt(m, order);
-  switch (nt * 4 + order) {
-#define FPM_GRAD(O_, NT_)                                                                       \
-  case NT_ * 4 + O_:                                                                            \
-    hipLaunchKernelGGL((k_readout_grad_tiles<T, O_, NT_>), grid, dim3(NT_), 0, st, P, g, w, G); \
+  const int rl = grad_rho_lds(m);
+  switch ((nt * 4 + order) * 2 + rl) {
+#define FPM_GRAD(O_, NT_, R_)                                                                   \
+  case (NT_ * 4 + O_) * 2 + R_:                                                                 \
+    hipLaunchKernelGGL((k_readout_grad_tiles<T, O_, NT_, R_>), grid, dim3(NT_), 0, st, P, g, w,  \
+                       G);                                                                      \
     break;
-    FPM_GRAD(2, 256) FPM_GRAD(2, 512) FPM_GRAD(2, 1024)
-    FPM_GRAD(3, 256) FPM_GRAD(3, 512) FPM_GRAD(3, 1024)
+    FPM_GRAD(2, 256, 1) FPM_GRAD(2, 512, 1) FPM_GRAD(2, 1024, 1)
+    FPM_GRAD(3, 256, 1) FPM_GRAD(3, 512, 1) FPM_GRAD(3, 1024, 1)
+    FPM_GRAD(2, 256, 0) FPM_GRAD(2, 512, 0) FPM_GRAD(2, 1024, 0)
+    FPM_GRAD(3, 256, 0) FPM_GRAD(3, 512, 0) FPM_GRAD(3, 1024, 0)
 #undef FPM_GRAD
     default: return FPM_ERR_ARG;
   }
@@ -1689,7 +1733,7 @@ __global__ __launch_bounds__(NT) void k_batch_paint(const TileBox<T>* __restrict
   paint_tiles_body<T, ORDER, NT>(x.P, x.g, x.w, x.rho, x.mass_fs, x.sh, x.inv_fx, j, cnt[b]);
 }
 
-template <typename T, int ORDER, int NT>
+template <typename T, int ORDER, int NT, bool RLDS>
 __global__ __launch_bounds__(NT) void k_batch_readout(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total) {
   const int64_t it = blockIdx.x;
   if (it >= total) return;
@@ -1697,14 +1741,14 @@ __global__ __launch_bounds__(NT) void k_batch_readout(const TileBox<T>* __restri
   const int64_t j = it - pre[b];
   if (j >= cnt[b]) return;
   const TileBox<T>& x = tab[b];
-  readout_grad_body<T, ORDER, NT>(x.P, x.g, x.w, x.G, j, cnt[b]);
+  readout_grad_body<T, ORDER, NT, RLDS>(x.P, x.g, x.w, x.G, j, cnt[b]);
 }
 
 int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
                      std::vector<BatchLaunch>& bin, std::vector<BatchLaunch>& paint,
                      std::vector<BatchLaunch>& readout) {
   std::vector<TileBox<float>> args(nbox);
-  std::vector<int> order(nbox), pnt(nbox), rnt(nbox);
+  std::vector<int> order(nbox), pnt(nbox), rnt(nbox), rlds(nbox);
   for (int i = 0; i < nbox; ++i) {
     const fpm_batch_box& bx = boxes[i];
     TileBox<float>& x = args[i];
@@ -1722,6 +1766,7 @@ int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     order[i] = bx.order;
     pnt[i] = paint_nt(&bx.mesh);
     rnt[i] = grad_nt(&bx.mesh, bx.order);
+    rlds[i] = grad_rho_lds(&bx.mesh);
   }
   // keys carry the box index through the grouping lambdas
   auto idx = [&](const TileBox<float>& x) { return (int)(&x - args.data()); };
@@ -1746,7 +1791,9 @@ int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     return rc;
   if ((rc = batch_groups(
            args, kTbReadout,
-           [&](const TileBox<float>& x) { return BatchKey{order[idx(x)], rnt[idx(x)], 0, 0}; },
+           [&](const TileBox<float>& x) {
+             return BatchKey{order[idx(x)], rnt[idx(x)], rlds[idx(x)], 0};
+           },
            [&](const TileBox<float>& x) { return x.P.n > 0 ? x.g.ntiles : (int64_t)0; }, arena,
            readout)))
     return rc;
@@ -1800,14 +1847,16 @@ int tile_batch_run(const std::vector<BatchLaunch>& ls, hipStream_t st) {
         }
         break;
       case kTbReadout:
-        switch (L.b * 4 + L.a) {
-#define FPM_BREAD(O_, NT_)                                                                     \
-  case NT_ * 4 + O_:                                                                           \
-    hipLaunchKernelGGL((k_batch_readout<float, O_, NT_>), grid, dim3(NT_), 0, st, tab, L.pre,    \
-                       L.cnt, L.nb, L.total);                                                  \
+        switch ((L.b * 4 + L.a) * 2 + L.c) {
+#define FPM_BREAD(O_, NT_, R_)                                                                 \
+  case (NT_ * 4 + O_) * 2 + R_:                                                                \
+    hipLaunchKernelGGL((k_batch_readout<float, O_, NT_, R_>), grid, dim3(NT_), 0, st, tab,       \
+                       L.pre, L.cnt, L.nb, L.total);                                           \
     break;
-          FPM_BREAD(2, 256) FPM_BREAD(3, 256) FPM_BREAD(2, 512) FPM_BREAD(3, 512)
-          FPM_BREAD(2, 1024) FPM_BREAD(3, 1024)
+          FPM_BREAD(2, 256, 1) FPM_BREAD(3, 256, 1) FPM_BREAD(2, 512, 1) FPM_BREAD(3, 512, 1)
+          FPM_BREAD(2, 1024, 1) FPM_BREAD(3, 1024, 1)
+          FPM_BREAD(2, 256, 0) FPM_BREAD(3, 256, 0) FPM_BREAD(2, 512, 0) FPM_BREAD(3, 512, 0)
+          FPM_BREAD(2, 1024, 0) FPM_BREAD(3, 1024, 0)
 #undef FPM_BREAD
           default: return FPM_ERR_ARG;
         }

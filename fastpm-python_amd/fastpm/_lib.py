@@ -40,6 +40,7 @@ VARIANT_FIELDS = {
     "bin": (4, (0, 1, 2)),
     "readout": (6, (0, 1, 2, 3)),
     "paint": (8, (0, 1, 2)),
+    "rho": (10, (0, 1)),
 }
 
 

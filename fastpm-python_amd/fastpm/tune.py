@@ -12,7 +12,9 @@ per mesh, among compiled variants of the hot kernels:
   bin       the binning walk over lattice particles: 4 x 8 x 16 blocks,
             8 x 8 patches or index order;
   paint     threads per tile of the tiled paint (256, 512, 1024);
-  readout   threads per tile of the stencil force readout (256, 512, 1024).
+  readout   threads per tile of the stencil force readout (256, 512, 1024);
+  rho       the stencil readout's density: staged in LDS, or read through
+            the cache (a third less LDS per tile).
 
 Every choice gives the same results (bitwise for the particle kernels, to
 rounding for the transform tilings).  `autotune` times them on the mesh by
@@ -182,22 +184,25 @@ def autotune_particles(pm, parts, order=None, reps=None, verbose=False):
     def readout():
         pm.readout_tiled_grad4(phi, parts, F, rho=rho, rho_out=R, resampler=rs)
 
-    fields = [("bin", (0, 1, 2), bin_), ("paint", (0, 1, 2), paint)]
+    # (fields, choices with the default first, timed call); the readout's
+    # thread count and density source are searched together
+    fields = [(("bin",), [(0,), (1,), (2,)], bin_), (("paint",), [(0,), (1,), (2,)], paint)]
     if stencil:
-        # the default thread count of the readout as an explicit choice
-        fields.append(("readout", (3 if order == 2 else 2, 1, 2, 3), readout))
-    for name, opts, fn in fields:
+        nt0 = 3 if order == 2 else 2
+        opts = [(nt0, 0)] + [(n, r) for r in (0, 1) for n in (1, 2, 3) if (n, r) != (nt0, 0)]
+        fields.append((("readout", "rho"), opts, readout))
+    for names, opts, fn in fields:
         t = {}
         for c in opts:
-            pm.set_variant(**{name: c})
+            pm.set_variant(**dict(zip(names, c)))
             bin_()
             t[c] = _time(fn, reps)
         c = min(t, key=t.get)
         if t[c] > (1 - MARGIN) * t[opts[0]]:
             c = opts[0]
-        pm.set_variant(**{name: c})
-        choices[name] = c
-        times[name] = t
+        pm.set_variant(**dict(zip(names, c)))
+        choices.update(dict(zip(names, c)))
+        times["/".join(names)] = t
     bin_()
     _CACHE[key] = choices
     _save_file({key: choices})

@@ -77,12 +77,16 @@ typedef struct fpm_mesh {
  *                      (fpm_readout_tiled_grad4_f32), 0 = default (1024 CIC,
  *                      512 TSC), 1 = 256, 2 = 512, 3 = 1024
  *   FPM_VAR_PAINT    : threads per tile of the tiled paint, 0 = default
- *                      (512), 1 = 256, 2 = 1024 */
+ *                      (512), 1 = 256, 2 = 1024
+ *   FPM_VAR_RHO      : the stencil readout's RHO, 0 = from a density box
+ *                      staged in LDS, 1 = read from the mesh through the
+ *                      cache (a third less LDS per tile) */
 #define FPM_VAR_FFT_COLS 0
 #define FPM_VAR_Z_LINES 2
 #define FPM_VAR_BIN 4
 #define FPM_VAR_READOUT 6
 #define FPM_VAR_PAINT 8
+#define FPM_VAR_RHO 10
 #define FPM_VAR_FIELD(variant, shift) (((variant) >> (shift)) & 3)
 
 /* A k-space multiplier, evaluated per mode and fused into the first pass of the

@@ -90,13 +90,20 @@ def test_particle_variants_bitwise(gpu, nmesh, resampler):
     X = O.lattice(parts.Q.nlat, box) + parts.S.cpu().numpy().astype("f8")
     dens = O.paint(X, nmesh, box, order, 0.7)
     assert np.abs(ref[0].cpu().numpy() - dens).max() < 2e-6 * dens.max()
-    for field, opts in (("bin", (1, 2)), ("paint", (1, 2)), ("readout", (1, 2, 3))):
+    for field, opts in (("bin", (1, 2)), ("paint", (1, 2)), ("readout", (1, 2, 3)), ("rho", (1,))):
         for c in opts:
             pm.set_variant(**{field: c})
             got = run()
             for a, b in zip(got, ref):
                 assert torch.equal(a, b), (field, c)
         pm.set_variant(**{field: 0})
+    # the density read through the cache with every thread count
+    for c in (1, 2, 3):
+        pm.set_variant(rho=1, readout=c)
+        got = run()
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b), ("rho+readout", c)
+    pm.set_variant(rho=0, readout=0)
 
 
 def test_autotune_chooses_valid_variants(gpu):
@@ -109,6 +116,7 @@ def test_autotune_chooses_valid_variants(gpu):
     v = tune.autotune(pm, parts)
     assert v["fft_cols"] in (1, 2) and v["z_lines"] in (0, 1, 2)
     assert v["bin"] in (0, 1, 2) and v["paint"] in (0, 1, 2) and v["readout"] in (1, 2, 3)
+    assert v["rho"] in (0, 1)
     # a second mesh of the same shape takes the cached choice without timing
     pm2 = ParticleMesh(Nmesh=(64, 64, 64), BoxSize=50.0, dtype="f4")
     assert tune.autotune(pm2, _lattice_parts(pm2, 2, np.random.default_rng(2), 1.0, gpu)) == v

@@ -341,7 +341,7 @@ def main():
     # density and potential meshes read once, S in, RHO and F out
     ro_bytes = 8 * Nc + 28 * Np
     ach = ro_bytes / (ro_ms * 1e-3) / 1e9
-    kname = "k_readout_grad_tiles<float, 2, 1024>"
+    kname = "k_readout_grad_tiles<float, 2, 1024, true>"
     out["roofline"] = {"kernel": kname + " (RHO + four-point-gradient force, CIC)",
                        "bound": "hbm",
                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -128,6 +128,18 @@ def gates(r32, r64, args):
                 max_disp_err_cells=float(dx.abs().max()) * r64["nmesh"] / args.box)
 
 
+def readout_kernel_name(fpm, order):
+    """The compiled stencil-readout kernel that fpm's variant fields select
+    (as rocprofv3 names it)."""
+    v = fpm.variant
+    nt = {0: 1024 if order == 2 else 512, 1: 256, 2: 512, 3: 1024}[v["readout"]]
+    span = 2 if v["span"] == 1 and (int(fpm.Nmesh[2]) // 16) % 2 == 0 else 1
+    if span == 2:
+        nt = max(nt, 512)
+    return "k_readout_grad_tiles<float, %d, %d, %s, %d>" % (
+        order, nt, "true" if v["rho"] == 0 else "false", span)
+
+
 def _profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` at this workload from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json), or None."""
@@ -341,7 +353,7 @@ def main():
     # density and potential meshes read once, S in, RHO and F out
     ro_bytes = 8 * Nc + 28 * Np
     ach = ro_bytes / (ro_ms * 1e-3) / 1e9
-    kname = "k_readout_grad_tiles<float, 2, 1024, true>"
+    kname = readout_kernel_name(r32["solver"].fpm, 2)
     out["roofline"] = {"kernel": kname + " (RHO + four-point-gradient force, CIC)",
                        "bound": "hbm",
                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

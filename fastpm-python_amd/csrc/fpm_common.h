@@ -38,6 +38,16 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nblk) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// Linear index of mesh cell (x, y, z), each coordinate within its axis: the
+// row x * n1 + y by a full-rate 24-bit multiply-add (both factors are below
+// 2^24 and the row count n0 * n1 below 2^32), then one 32 x 32 -> 64-bit
+// multiply-add for the row's first cell.  The general 64-bit form costs
+// three quarter-rate multiplies.
+__device__ __forceinline__ int64_t mesh_index(int x, int y, int z, const int nm[3]) {
+  const uint32_t row = __umul24((uint32_t)x, (uint32_t)nm[1]) + (uint32_t)y;
+  return (int64_t)((uint64_t)row * (uint32_t)nm[2] + (uint32_t)z);
+}
+
 // Periodic wrap of a cell index into [0, n).  Indices within one period of
 // the box (every particle the solver moves) take the compare-and-add paths;
 // only others pay for the 64-bit remainder.

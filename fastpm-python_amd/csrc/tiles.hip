@@ -184,10 +184,14 @@ __device__ __forceinline__ int32_t bin_of(const int i0[3], const TileGeom& g) {
   return (int32_t)(t * kSub + mask);
 }
 
+// Tile t's first cell; the tile count is below 2^28 (the host keeps the
+// sub-bin count below 2^31), so 32-bit division is exact.
 __device__ __forceinline__ void tile_coords(int64_t t, const TileGeom& g, int lo[3]) {
-  lo[2] = (int)(t % g.nt[2]) * TILE;
-  lo[1] = (int)((t / g.nt[2]) % g.nt[1]) * TILE;
-  lo[0] = (int)(t / ((int64_t)g.nt[2] * g.nt[1])) * TILE;
+  const uint32_t u = (uint32_t)t, n2 = (uint32_t)g.nt[2], n1 = (uint32_t)g.nt[1];
+  const uint32_t r = u / n2;
+  lo[2] = (int)(u - r * n2) * TILE;
+  lo[1] = (int)(r % n1) * TILE;
+  lo[0] = (int)(r / n1) * TILE;
 }
 
 // ---------------------------------------------------------------- binning
@@ -641,7 +645,7 @@ __device__ __forceinline__ void paint_tiles_body(const PartDesc<T>& P, const Til
   __syncthreads();
   for (int i = threadIdx.x; i < NB; i += NT) {
     const int z = i % TILE, y = (i / TILE) % TILE, x = i / (TILE * TILE);
-    mesh[((int64_t)(lo[0] + x) * g.nm[1] + (lo[1] + y)) * g.nm[2] + (lo[2] + z)] =
+    mesh[mesh_index(lo[0] + x, lo[1] + y, lo[2] + z, g.nm)] =
         (T)((double)(long long)acc[i] * inv_fx);
   }
 }
@@ -767,7 +771,7 @@ __global__ __launch_bounds__(kTileThreads) void k_paint_tiles_weighted(TileGeom 
     __syncthreads();
     for (int i = threadIdx.x; i < NB; i += kTileThreads) {
       const int z = i % TILE, y = (i / TILE) % TILE, x = i / (TILE * TILE);
-      const int64_t oo = ((int64_t)(lo[0] + x) * g.nm[1] + (lo[1] + y)) * g.nm[2] + (lo[2] + z);
+      const int64_t oo = mesh_index(lo[0] + x, lo[1] + y, lo[2] + z, g.nm);
       A.out[c][oo] = (T)ldexp((double)(long long)acc[i], -e);
     }
     __syncthreads();  // the next component clears the accumulator
@@ -829,7 +833,7 @@ struct BoxRegs {
         int gx = lo[0] + x, gy = lo[1] + y;
         if (gx >= g.nm[0]) gx -= g.nm[0];
         if (gy >= g.nm[1]) gy -= g.nm[1];
-        const int64_t off = ((int64_t)gx * g.nm[1] + gy) * g.nm[2] + lo[2] + 4 * q;
+        const int64_t off = mesh_index(gx, gy, lo[2] + 4 * q, g.nm);
 #pragma unroll
         for (int c = 0; c < NC; ++c) vr[k][c] = *reinterpret_cast<const V4*>(R.mesh[c] + off);
       }
@@ -844,7 +848,7 @@ struct BoxRegs {
         if (gx >= g.nm[0]) gx -= g.nm[0];
         if (gy >= g.nm[1]) gy -= g.nm[1];
         if (gz >= g.nm[2]) gz -= g.nm[2];
-        const int64_t off = ((int64_t)gx * g.nm[1] + gy) * g.nm[2] + gz;
+        const int64_t off = mesh_index(gx, gy, gz, g.nm);
 #pragma unroll
         for (int c = 0; c < NC; ++c) hr[k][c] = R.mesh[c][off];
       }
@@ -974,17 +978,20 @@ struct GradRead {
   T f_scale[3];      // force factor / (12 h_d)
 };
 
-// Scalar box of (TILE + L + U)^3 cells starting at lo - L, staged with the
-// aligned TILE-cell middle of every row as four-wide vectors and L + U wrapped
-// edge cells.  With CORE > 0 only the cells a stencil readout can reach are
-// loaded: the core [L, L + CORE)^3 extended along one axis at a time (the
-// four-point stencil runs along one axis while the window stays in the core
-// of the other two); the rest of the box is left unwritten and never read.
-template <typename T, int L, int U, int CORE = 0, int NT = kTileThreads>
+// Scalar box of (TILE + L + U)^2 x (ZT + L + U) cells starting at lo - L
+// (ZT = TILE for one tile, 2 TILE for two tiles adjacent along z), staged
+// with the aligned ZT-cell middle of every row as four-wide vectors and
+// L + U wrapped edge cells.  With CORE > 0 only the cells a stencil readout
+// can reach are loaded: the core [L, L + CORE)^2 x [L, L + CORE + ZT - TILE)
+// extended along one axis at a time (the four-point stencil runs along one
+// axis while the window stays in the core of the other two); the rest of the
+// box is left unwritten and never read.
+template <typename T, int L, int U, int CORE = 0, int NT = kTileThreads, int ZT = TILE>
 struct ScalarBox {
   typedef T V4 __attribute__((ext_vector_type(4)));
-  static constexpr int TB = TILE + L + U;
-  static constexpr int NQ = TILE / 4;
+  static constexpr int TB = TILE + L + U;   // x and y
+  static constexpr int TBZ = ZT + L + U;
+  static constexpr int NQ = ZT / 4;
   static constexpr int NV = TB * TB * NQ;
   static constexpr int PV = (NV + NT - 1) / NT;
   static constexpr int NE = TB * TB * (L + U);
@@ -996,23 +1003,83 @@ struct ScalarBox {
     return i < 0 ? i + n : (i >= n ? i - n : i);
   }
 
-  // in the core along one axis (box coordinate b)
+  // in the core along x or y (box coordinate b), or along z
   __device__ __forceinline__ static bool in_core(int b) { return b >= L && b < L + CORE; }
+  __device__ __forceinline__ static bool in_core_z(int b) {
+    return b >= L && b < L + CORE + ZT - TILE;
+  }
+
+  // Row-middle element i: box row (bx, by) and four-cell group q.
+  __device__ __forceinline__ static void mid_of(int i, int& bx, int& by, int& q) {
+    q = i % NQ;
+    const int r = i / NQ;
+    by = r % TB;
+    bx = r / TB;
+  }
+  // Edge element i: box row (bx, by) and box z coordinate bz.
+  __device__ __forceinline__ static void edge_of(int i, int& bx, int& by, int& bz) {
+    const int h = i % (L + U), r = i / (L + U);
+    by = r % TB;
+    bx = r / TB;
+    bz = h < L ? h : ZT + h;
+  }
+  __device__ __forceinline__ static bool mid_needed(int bx, int by) {
+    // the middle of a row is in the z core: the row is needed unless both
+    // x and y are outside the core
+    return CORE == 0 || in_core(bx) || in_core(by);
+  }
+  __device__ __forceinline__ static bool edge_needed(int bx, int by, int bz) {
+    // an edge cell outside the z core is needed only on a core row
+    return CORE == 0 || (in_core(bx) && in_core(by)) ||
+           (in_core_z(bz) && (in_core(bx) || in_core(by)));
+  }
 
   __device__ __forceinline__ void load(const TileGeom& g, const T* mesh, const int lo[3]) {
+    // A box inside the mesh (most tiles) is indexed without wrapping: the
+    // box origin's cell plus bx * n1 n2 + by * n2 + bz, one 24-bit and one
+    // 32 x 32 -> 64-bit multiply-add per element (n1 n2 < 2^32).
+    bool inner = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      inner = inner && lo[d] >= L && lo[d] + (d == 2 ? ZT : TILE) + U <= g.nm[d];
+    if (inner) {
+      const T* base = mesh + mesh_index(lo[0] - L, lo[1] - L, lo[2] - L, g.nm);
+      const uint32_t n2 = (uint32_t)g.nm[2], n12 = (uint32_t)g.nm[1] * n2;
+      auto at = [&](int bx, int by, int bz) {
+        return base + ((uint64_t)(uint32_t)bx * n12 +
+                       (uint32_t)(__umul24((uint32_t)by, n2) + (uint32_t)bz));
+      };
+#pragma unroll
+      for (int k = 0; k < PV; ++k) {
+        const int i = threadIdx.x + k * NT;
+        if (i < NV) {
+          int bx, by, q;
+          mid_of(i, bx, by, q);
+          vr[k] = mid_needed(bx, by) ? *reinterpret_cast<const V4*>(at(bx, by, L + 4 * q))
+                                     : V4{(T)0, (T)0, (T)0, (T)0};
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < PE; ++k) {
+        const int i = threadIdx.x + k * NT;
+        if (i < NE) {
+          int bx, by, bz;
+          edge_of(i, bx, by, bz);
+          er[k] = edge_needed(bx, by, bz) ? *at(bx, by, bz) : (T)0;
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < PV; ++k) {
       const int i = threadIdx.x + k * NT;
       if (i < NV) {
-        const int q = i % NQ, r = i / NQ;
-        const int by = r % TB, bx = r / TB;
-        // the middle of a row is in the z core: the row is needed unless both
-        // x and y are outside the core
-        if (CORE == 0 || in_core(bx) || in_core(by)) {
+        int bx, by, q;
+        mid_of(i, bx, by, q);
+        if (mid_needed(bx, by)) {
           const int gy = wrap(lo[1] - L + by, g.nm[1]);
           const int gx = wrap(lo[0] - L + bx, g.nm[0]);
-          vr[k] = *reinterpret_cast<const V4*>(mesh + ((int64_t)gx * g.nm[1] + gy) * g.nm[2] +
-                                               lo[2] + 4 * q);
+          vr[k] = *reinterpret_cast<const V4*>(mesh + mesh_index(gx, gy, lo[2] + 4 * q, g.nm));
         } else {
           vr[k] = V4{(T)0, (T)0, (T)0, (T)0};
         }
@@ -1022,16 +1089,13 @@ struct ScalarBox {
     for (int k = 0; k < PE; ++k) {
       const int i = threadIdx.x + k * NT;
       if (i < NE) {
-        const int h = i % (L + U), r = i / (L + U);
-        const int by = r % TB, bx = r / TB;
-        const int bz = h < L ? h : TILE + h;
-        // an edge cell outside the z core is needed only on a core row
-        if (CORE == 0 || (in_core(bx) && in_core(by)) ||
-            (in_core(bz) && (in_core(bx) || in_core(by)))) {
+        int bx, by, bz;
+        edge_of(i, bx, by, bz);
+        if (edge_needed(bx, by, bz)) {
           const int gy = wrap(lo[1] - L + by, g.nm[1]);
           const int gx = wrap(lo[0] - L + bx, g.nm[0]);
-          const int gz = wrap(h < L ? lo[2] - L + h : lo[2] + TILE + (h - L), g.nm[2]);
-          er[k] = mesh[((int64_t)gx * g.nm[1] + gy) * g.nm[2] + gz];
+          const int gz = wrap(lo[2] - L + bz, g.nm[2]);
+          er[k] = mesh[mesh_index(gx, gy, gz, g.nm)];
         } else {
           er[k] = (T)0;
         }
@@ -1045,7 +1109,7 @@ struct ScalarBox {
       const int i = threadIdx.x + k * NT;
       if (i < NV) {
         const int q = i % NQ, r = i / NQ;
-        T* dst = &box[r * TB + L + 4 * q];
+        T* dst = &box[r * TBZ + L + 4 * q];
         dst[0] = vr[k].x;
         dst[1] = vr[k].y;
         dst[2] = vr[k].z;
@@ -1057,20 +1121,21 @@ struct ScalarBox {
       const int i = threadIdx.x + k * NT;
       if (i < NE) {
         const int h = i % (L + U), r = i / (L + U);
-        box[r * TB + (h < L ? h : TILE + h)] = er[k];
+        box[r * TBZ + (h < L ? h : ZT + h)] = er[k];
       }
     }
   }
 };
 
 // The density at tile-box coordinates (x, y, z) (box origin at the tile's
-// first cell): staged in LDS, or read from the mesh through the cache.
-template <typename T, int ORDER>
+// first cell): staged in LDS (a box ZT + ORDER - 1 cells long along z), or
+// read from the mesh through the cache.
+template <typename T, int ORDER, int ZT = TILE>
 struct RhoLds {
   const T* rbox;
   __device__ __forceinline__ T operator()(int x, int y, int z) const {
-    constexpr int TR = TILE + ORDER - 1;
-    return rbox[(x * TR + y) * TR + z];
+    constexpr int TR = TILE + ORDER - 1, TRZ = ZT + ORDER - 1;
+    return rbox[(x * TR + y) * TRZ + z];
   }
 };
 
@@ -1083,30 +1148,39 @@ struct RhoMesh {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
       if (g[d] >= nm[d]) g[d] -= nm[d];
-    return rho[((int64_t)g[0] * nm[1] + g[1]) * nm[2] + g[2]];
+    return rho[mesh_index(g[0], g[1], g[2], nm)];
   }
 };
 
+// Window origin l in box coordinates of the density box (the potential
+// box starts two cells earlier on every axis); the boxes are ZT + ORDER - 1
+// and ZT + ORDER + 3 cells long along z.  rho_one writes RHO, force_one the
+// force; grad_one does both.
 template <typename T, int ORDER, typename RhoAt>
-__device__ __forceinline__ void grad_one(const RhoAt& rho_at, const T* pbox, const GradRead<T>& G,
-                                         int64_t p, const int l[3], const T wt[3][ORDER]) {
+__device__ __forceinline__ void rho_one(const RhoAt& rho_at, const GradRead<T>& G, int64_t p,
+                                        const int l[3], const T wt[3][ORDER]) {
+  const int lx = l[0], ly = l[1], lz = l[2];
+  T acc = (T)0;
+#pragma unroll
+  for (int a = 0; a < ORDER; ++a)
+#pragma unroll
+    for (int b = 0; b < ORDER; ++b) {
+      const T wab = wt[0][a] * wt[1][b];
+#pragma unroll
+      for (int c = 0; c < ORDER; ++c) acc += wab * wt[2][c] * rho_at(lx + a, ly + b, lz + c);
+    }
+  G.rho_out[p * G.rho_stride] = acc * G.rho_scale;
+}
+
+template <typename T, int ORDER, int ZT = TILE>
+__device__ __forceinline__ void force_one(const T* pbox, const GradRead<T>& G, int64_t p,
+                                          const int l[3], const T wt[3][ORDER]) {
   constexpr int TP = TILE + ORDER + 3;   // potential box side (2 more cells each way)
+  constexpr int TPZ = ZT + ORDER + 3;
   constexpr int NLN = ORDER + 4;         // stencil line: window points -2 .. +2
   const int lx = l[0], ly = l[1], lz = l[2];
-  if (G.rho_out) {
-    T acc = (T)0;
-#pragma unroll
-    for (int a = 0; a < ORDER; ++a)
-#pragma unroll
-      for (int b = 0; b < ORDER; ++b) {
-        const T wab = wt[0][a] * wt[1][b];
-#pragma unroll
-        for (int c = 0; c < ORDER; ++c) acc += wab * wt[2][c] * rho_at(lx + a, ly + b, lz + c);
-      }
-    G.rho_out[p * G.rho_stride] = acc * G.rho_scale;
-  }
   // potential box index of window point (a, b, c): (lx+2+a, ly+2+b, lz+2+c)
-  auto at = [&](int x, int y, int z) { return pbox[(x * TP + y) * TP + z]; };
+  auto at = [&](int x, int y, int z) { return pbox[(x * TP + y) * TPZ + z]; };
   T f[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -1135,30 +1209,91 @@ __device__ __forceinline__ void grad_one(const RhoAt& rho_at, const T* pbox, con
   for (int d = 0; d < 3; ++d) G.f_out[p * G.f_stride + d] = f[d] * G.f_scale[d];
 }
 
+template <typename T, int ORDER, int ZT = TILE, typename RhoAt>
+__device__ __forceinline__ void grad_one(const RhoAt& rho_at, const T* pbox, const GradRead<T>& G,
+                                         int64_t p, const int l[3], const T wt[3][ORDER]) {
+  if (G.rho_out) rho_one<T, ORDER>(rho_at, G, p, l, wt);
+  force_one<T, ORDER, ZT>(pbox, G, p, l, wt);
+}
+
 // RLDS: the density box staged in LDS (true) or the density read through
 // the cache (false: a third less LDS, the same values in the same order).
-template <typename T, int ORDER, int NT, bool RLDS = true>
+// SPAN: tiles per workgroup along z (1 or 2; the tile count along z is a
+// multiple of SPAN).  Two z-adjacent tiles share their boxes: each box row's
+// middle is then a whole 128-byte line, and the halo rows are loaded once
+// for both, so a particle costs about 40% fewer cache-line requests.  A
+// particle of the second tile has its window origin moved by TILE along z
+// into the shared boxes; the values read and the arithmetic are the same.
+template <typename T, int ORDER, int NT, bool RLDS = true, int SPAN = 1>
 __device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const TileGeom& g,
                                                   const Workspace& w, const GradRead<T>& G,
                                                   int64_t blk, int64_t nblk) {
-  constexpr int TR = TILE + ORDER - 1;
-  constexpr int TP = TILE + ORDER + 3;
-  __shared__ T rbox[RLDS ? TR * TR * TR : 1];
-  __shared__ T pbox[TP * TP * TP];
-  const int64_t t = xcd_remap(blk, nblk);
+  constexpr int ZT = SPAN * TILE;
+  constexpr int TR = TILE + ORDER - 1, TRZ = ZT + ORDER - 1;
+  constexpr int TP = TILE + ORDER + 3, TPZ = ZT + ORDER + 3;
+  // two tiles with the density from LDS: the density box and then the
+  // potential box in one buffer (RHO read out in between), so that two
+  // workgroups share a CU
+  constexpr bool PHASED = SPAN > 1 && RLDS;
+  static_assert(TR * TR * TRZ <= TP * TP * TPZ, "the density box fits the potential buffer");
+  __shared__ T rbox[RLDS && !PHASED ? TR * TR * TRZ : 1];
+  __shared__ T pbox[TP * TP * TPZ];
+  // z is the fastest tile axis, so unit u's tiles are u * SPAN + [0, SPAN)
+  const int64_t t = xcd_remap(blk, nblk) * SPAN;
   int lo[3];
   tile_coords(t, g, lo);
-  const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
+  const int32_t b = w.offs[t * kSub], e = w.offs[(t + SPAN) * kSub];
+  const int32_t mid = SPAN == 1 ? e : w.offs[(t + 1) * kSub];  // first slot of the second tile
   if (b == e) return;  // uniform across the workgroup
   const int32_t s0 = b + threadIdx.x, s1 = s0 + NT;
   const int64_t p0 = s0 < e ? w.list[s0] : 0;
   const int64_t p1 = s1 < e ? w.list[s1] : 0;
   const PRec r0 = s0 < e ? w.rec[s0] : PRec{};
   const PRec r1 = s1 < e ? w.rec[s1] : PRec{};
-  ScalarBox<T, 2, ORDER + 1, TILE + ORDER - 1, NT> pb;
+  auto window = [&](int32_t s, const PRec& r, int l[3], T wt[3][ORDER]) {
+    window_rec<T, ORDER>(r, l, wt);
+    if (SPAN > 1 && s >= mid) l[2] += TILE;
+  };
+  ScalarBox<T, 2, ORDER + 1, TILE + ORDER - 1, NT, ZT> pb;
+  if constexpr (PHASED) {
+    if (G.rho_out) {
+      {
+        ScalarBox<T, 0, ORDER - 1, 0, NT, ZT> rb;
+        rb.load(g, G.rho, lo);
+        rb.store(pbox);
+      }
+      __syncthreads();
+      pb.load(g, G.phi, lo);  // in flight while RHO is read out
+      const RhoLds<T, ORDER, ZT> rl{pbox};
+      auto rone = [&](int32_t s, int64_t p, const PRec& r) {
+        int l[3];
+        T wt[3][ORDER];
+        window(s, r, l, wt);
+        rho_one<T, ORDER>(rl, G, p, l, wt);
+      };
+      if (s0 < e) rone(s0, p0, r0);
+      if (s1 < e) rone(s1, p1, r1);
+      for (int32_t s = s1 + NT; s < e; s += NT) rone(s, (int64_t)w.list[s], w.rec[s]);
+      __syncthreads();  // every read of the density box is done
+    } else {
+      pb.load(g, G.phi, lo);
+    }
+    pb.store(pbox);
+    __syncthreads();
+    auto fone = [&](int32_t s, int64_t p, const PRec& r) {
+      int l[3];
+      T wt[3][ORDER];
+      window(s, r, l, wt);
+      force_one<T, ORDER, ZT>(pbox, G, p, l, wt);
+    };
+    if (s0 < e) fone(s0, p0, r0);
+    if (s1 < e) fone(s1, p1, r1);
+    for (int32_t s = s1 + NT; s < e; s += NT) fone(s, (int64_t)w.list[s], w.rec[s]);
+    return;
+  }
   pb.load(g, G.phi, lo);
   if constexpr (RLDS) {
-    ScalarBox<T, 0, ORDER - 1, 0, NT> rb;
+    ScalarBox<T, 0, ORDER - 1, 0, NT, ZT> rb;
     if (G.rho_out) rb.load(g, G.rho, lo);
     pb.store(pbox);
     if (G.rho_out) rb.store(rbox);
@@ -1166,32 +1301,24 @@ __device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const Ti
     pb.store(pbox);
   }
   __syncthreads();
-  const RhoLds<T, ORDER> rl{rbox};
+  const RhoLds<T, ORDER, ZT> rl{rbox};
   const RhoMesh<T> rm{G.rho, {lo[0], lo[1], lo[2]}, {g.nm[0], g.nm[1], g.nm[2]}};
-  auto one = [&](int64_t p, const int l[3], const T wt[3][ORDER]) {
-    if constexpr (RLDS) grad_one<T, ORDER>(rl, pbox, G, p, l, wt);
-    else grad_one<T, ORDER>(rm, pbox, G, p, l, wt);
+  auto one = [&](int32_t s, int64_t p, const PRec& r) {
+    int l[3];
+    T wt[3][ORDER];
+    window(s, r, l, wt);
+    if constexpr (RLDS) grad_one<T, ORDER, ZT>(rl, pbox, G, p, l, wt);
+    else grad_one<T, ORDER, ZT>(rm, pbox, G, p, l, wt);
   };
-  int l[3];
-  T wt[3][ORDER];
-  if (s0 < e) {
-    window_rec<T, ORDER>(r0, l, wt);
-    one(p0, l, wt);
-  }
-  if (s1 < e) {
-    window_rec<T, ORDER>(r1, l, wt);
-    one(p1, l, wt);
-  }
-  for (int32_t s = s1 + NT; s < e; s += NT) {
-    window_rec<T, ORDER>(w.rec[s], l, wt);
-    one((int64_t)w.list[s], l, wt);
-  }
+  if (s0 < e) one(s0, p0, r0);
+  if (s1 < e) one(s1, p1, r1);
+  for (int32_t s = s1 + NT; s < e; s += NT) one(s, (int64_t)w.list[s], w.rec[s]);
 }
 
-template <typename T, int ORDER, int NT = grad_threads<ORDER>(), bool RLDS = true>
+template <typename T, int ORDER, int NT = grad_threads<ORDER>(), bool RLDS = true, int SPAN = 1>
 __global__ __launch_bounds__(NT) void k_readout_grad_tiles(PartDesc<T> P, TileGeom g,
                                                                       Workspace w, GradRead<T> G) {
-  readout_grad_body<T, ORDER, NT, RLDS>(P, g, w, G, blockIdx.x, gridDim.x);
+  readout_grad_body<T, ORDER, NT, RLDS, SPAN>(P, g, w, G, blockIdx.x, gridDim.x);
 }
 
 // Position gradient of the stencil force readout (the window-derivative
@@ -1299,11 +1426,15 @@ __global__ __launch_bounds__(kTileThreads) void k_pos_grad_tiles(TileGeom g, Wor
 
 
 // ---------------------------------------------------------------- host side
+// Tiled meshes: sides multiples of TILE, below 2^24 cells, fewer than 2^32
+// rows (mesh_index's 32-bit row arithmetic) and fewer than 2^32 cells per
+// x plane (the box loads' 32-bit plane stride).
 static bool tiles_ok(const fpm_mesh* m) {
   if (!m) return false;
   for (int d = 0; d < 3; ++d)
-    if (m->nmesh[d] <= 0 || m->nmesh[d] % TILE) return false;
-  return true;
+    if (m->nmesh[d] <= 0 || m->nmesh[d] % TILE || m->nmesh[d] >= (1 << 24)) return false;
+  return (int64_t)m->nmesh[0] * m->nmesh[1] < ((int64_t)1 << 32) &&
+         (int64_t)m->nmesh[1] * m->nmesh[2] < ((int64_t)1 << 32);
 }
 
 static unsigned grid_parts(int64_t n) {
@@ -1566,6 +1697,12 @@ static int grad_rho_lds(const fpm_mesh* m) {
   return FPM_VAR_FIELD(m->variant, FPM_VAR_RHO) == 1 ? 0 : 1;
 }
 
+// Tiles per workgroup of the stencil readout on mesh m (FPM_VAR_SPAN 1:
+// two z-adjacent tiles where the tile count along z is even, else one).
+static int grad_span(const fpm_mesh* m, const TileGeom& g) {
+  return FPM_VAR_FIELD(m->variant, FPM_VAR_SPAN) == 1 && g.nt[2] % 2 == 0 ? 2 : 1;
+}
+
 // Threads per tile of the stencil readout on mesh m (FPM_VAR_READOUT).
 static int grad_nt(const fpm_mesh* m, int order) {
   switch (FPM_VAR_FIELD(m->variant, FPM_VAR_READOUT)) {
@@ -1591,19 +1728,23 @@ static int readout_grad_impl(const fpm_particles* part, const fpm_mesh* m, int o
                           rho_scale, f_scale, G)))
     return rc;
   if (P.n == 0) return FPM_OK;
-  const dim3 grid((unsigned)g.ntiles);
-  const int nt = grad_nt(m, order);
+  int nt = grad_nt(m, order);
   const int rl = grad_rho_lds(m);
-  switch ((nt * 4 + order) * 2 + rl) {
-#define FPM_GRAD(O_, NT_, R_)                                                                   \
-  case (NT_ * 4 + O_) * 2 + R_:                                                                 \
-    hipLaunchKernelGGL((k_readout_grad_tiles<T, O_, NT_, R_>), grid, dim3(NT_), 0, st, P, g, w,  \
-                       G);                                                                      \
+  const int span = grad_span(m, g);
+  if (span == 2 && nt < 512) nt = 512;  // pairs are compiled for 512 and 1024 threads
+  const dim3 grid((unsigned)(g.ntiles / span));
+  switch (((nt * 4 + order) * 2 + rl) * 4 + span) {
+#define FPM_GRAD(O_, NT_, R_, S_)                                                               \
+  case ((NT_ * 4 + O_) * 2 + R_) * 4 + S_:                                                      \
+    hipLaunchKernelGGL((k_readout_grad_tiles<T, O_, NT_, R_, S_>), grid, dim3(NT_), 0, st, P, g, \
+                       w, G);                                                                   \
     break;
-    FPM_GRAD(2, 256, 1) FPM_GRAD(2, 512, 1) FPM_GRAD(2, 1024, 1)
-    FPM_GRAD(3, 256, 1) FPM_GRAD(3, 512, 1) FPM_GRAD(3, 1024, 1)
-    FPM_GRAD(2, 256, 0) FPM_GRAD(2, 512, 0) FPM_GRAD(2, 1024, 0)
-    FPM_GRAD(3, 256, 0) FPM_GRAD(3, 512, 0) FPM_GRAD(3, 1024, 0)
+    FPM_GRAD(2, 256, 1, 1) FPM_GRAD(2, 512, 1, 1) FPM_GRAD(2, 1024, 1, 1)
+    FPM_GRAD(3, 256, 1, 1) FPM_GRAD(3, 512, 1, 1) FPM_GRAD(3, 1024, 1, 1)
+    FPM_GRAD(2, 256, 0, 1) FPM_GRAD(2, 512, 0, 1) FPM_GRAD(2, 1024, 0, 1)
+    FPM_GRAD(3, 256, 0, 1) FPM_GRAD(3, 512, 0, 1) FPM_GRAD(3, 1024, 0, 1)
+    FPM_GRAD(2, 512, 1, 2) FPM_GRAD(2, 1024, 1, 2) FPM_GRAD(3, 512, 1, 2) FPM_GRAD(3, 1024, 1, 2)
+    FPM_GRAD(2, 512, 0, 2) FPM_GRAD(2, 1024, 0, 2) FPM_GRAD(3, 512, 0, 2) FPM_GRAD(3, 1024, 0, 2)
 #undef FPM_GRAD
     default: return FPM_ERR_ARG;
   }

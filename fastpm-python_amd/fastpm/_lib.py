@@ -41,6 +41,7 @@ VARIANT_FIELDS = {
     "readout": (6, (0, 1, 2, 3)),
     "paint": (8, (0, 1, 2)),
     "rho": (10, (0, 1)),
+    "span": (12, (0, 1)),
 }
 
 

@@ -14,7 +14,10 @@ per mesh, among compiled variants of the hot kernels:
   paint     threads per tile of the tiled paint (256, 512, 1024);
   readout   threads per tile of the stencil force readout (256, 512, 1024);
   rho       the stencil readout's density: staged in LDS, or read through
-            the cache (a third less LDS per tile).
+            the cache (a third less LDS per tile);
+  span      tiles per workgroup of the stencil readout: one, or two adjacent
+            along z sharing their boxes (fewer cache-line requests per
+            particle, twice the LDS).
 
 Every choice gives the same results (bitwise for the particle kernels, to
 rounding for the transform tilings).  `autotune` times them on the mesh by
@@ -185,12 +188,14 @@ def autotune_particles(pm, parts, order=None, reps=None, verbose=False):
         pm.readout_tiled_grad4(phi, parts, F, rho=rho, rho_out=R, resampler=rs)
 
     # (fields, choices with the default first, timed call); the readout's
-    # thread count and density source are searched together
+    # thread count, density source and span are searched together (two-tile
+    # spans with 512 or 1024 threads)
     fields = [(("bin",), [(0,), (1,), (2,)], bin_), (("paint",), [(0,), (1,), (2,)], paint)]
     if stencil:
         nt0 = 3 if order == 2 else 2
-        opts = [(nt0, 0)] + [(n, r) for r in (0, 1) for n in (1, 2, 3) if (n, r) != (nt0, 0)]
-        fields.append((("readout", "rho"), opts, readout))
+        opts = [(nt0, 0, 0)] + [(n, r, sp) for sp in (0, 1) for r in (0, 1) for n in (1, 2, 3)
+                                if (n, r, sp) != (nt0, 0, 0) and not (sp and n == 1)]
+        fields.append((("readout", "rho", "span"), opts, readout))
     for names, opts, fn in fields:
         t = {}
         for c in opts:

@@ -80,13 +80,18 @@ typedef struct fpm_mesh {
  *                      (512), 1 = 256, 2 = 1024
  *   FPM_VAR_RHO      : the stencil readout's RHO, 0 = from a density box
  *                      staged in LDS, 1 = read from the mesh through the
- *                      cache (a third less LDS per tile) */
+ *                      cache (a third less LDS per tile)
+ *   FPM_VAR_SPAN     : tiles per workgroup of the stencil readout, 0 = one,
+ *                      1 = two adjacent along z sharing one pair of boxes
+ *                      (where the tile count along z is even; at least 512
+ *                      threads) */
 #define FPM_VAR_FFT_COLS 0
 #define FPM_VAR_Z_LINES 2
 #define FPM_VAR_BIN 4
 #define FPM_VAR_READOUT 6
 #define FPM_VAR_PAINT 8
 #define FPM_VAR_RHO 10
+#define FPM_VAR_SPAN 12
 #define FPM_VAR_FIELD(variant, shift) (((variant) >> (shift)) & 3)
 
 /* A k-space multiplier, evaluated per mode and fused into the first pass of the

@@ -2,9 +2,9 @@
 fastpm.tune) against the oracle and against each other (GPU only): the z
 transform pass with 1024 / 2048 / 4096 elements per workgroup and both column
 tilings to rounding against the oracle; the binning walks, paint and readout
-thread counts bit for bit against the default variant (which the other tests
-check against the oracle) and the paint against the oracle; the tuner keeps
-a valid choice."""
+thread counts and the two-tile readout bit for bit against the default
+variant (which the other tests check against the oracle) and the paint
+against the oracle; the tuner keeps a valid choice."""
 import numpy as np
 import pytest
 import torch
@@ -90,7 +90,8 @@ def test_particle_variants_bitwise(gpu, nmesh, resampler):
     X = O.lattice(parts.Q.nlat, box) + parts.S.cpu().numpy().astype("f8")
     dens = O.paint(X, nmesh, box, order, 0.7)
     assert np.abs(ref[0].cpu().numpy() - dens).max() < 2e-6 * dens.max()
-    for field, opts in (("bin", (1, 2)), ("paint", (1, 2)), ("readout", (1, 2, 3)), ("rho", (1,))):
+    for field, opts in (("bin", (1, 2)), ("paint", (1, 2)), ("readout", (1, 2, 3)), ("rho", (1,)),
+                        ("span", (1,))):
         for c in opts:
             pm.set_variant(**{field: c})
             got = run()
@@ -106,6 +107,61 @@ def test_particle_variants_bitwise(gpu, nmesh, resampler):
     pm.set_variant(rho=0, readout=0)
 
 
+@pytest.mark.parametrize("resampler", ["cic", "tsc"])
+@pytest.mark.parametrize("nmesh", [(32, 32, 32), (64, 48, 96), (128, 96, 64), (48, 64, 48)])
+def test_two_tile_readout_bitwise(gpu, nmesh, resampler):
+    """The stencil readout with two z-adjacent tiles per workgroup (FPM_VAR_SPAN
+    1) gives the one-tile readout's force and RHO bit for bit, for every thread
+    count and density source, with and without RHO, on a lattice set and on a
+    clustered set whose densest tile pairs hold more than 2048 particles (slots
+    past the first two per thread are loaded in the loop); an odd tile count
+    along z (the last mesh) falls back to one tile per workgroup."""
+    from fastpm.pm import ParticleMesh
+
+    print("two-tile readout", nmesh, resampler, flush=True)
+    box = (90.0, 75.0, 110.0)
+    order = 2 if resampler == "cic" else 3
+    pm = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4", resampler=resampler)
+    rng = np.random.default_rng(17)
+    phi = pm.create("real")
+    phi.value.copy_(torch.from_numpy(rng.standard_normal(nmesh).astype("f4")))
+    rho = pm.create("real")
+    rho.value.copy_(torch.from_numpy(rng.random(nmesh).astype("f4")))
+    lat = _lattice_parts(pm, 2, rng, 2.0, gpu)
+    # a clump of 6000 particles within a few cells of one point, plus a
+    # uniform background
+    clump = rng.normal(0.0, 1.5 * box[0] / nmesh[0], size=(6000, 3)) + np.array(box) * 0.37
+    back = rng.random((20000, 3)) * np.array(box)
+    clustered = (np.concatenate([clump, back]) % np.array(box)).astype("f4")
+    for parts in (lat, clustered):
+        n = len(parts)
+        pm.bin_particles(parts)
+        assert pm.tile_misplaced(n, order) == 0
+
+        def run(with_rho):
+            F = torch.full((n, 3), float("nan"), dtype=torch.float32, device=gpu)
+            R = torch.full((n,), float("nan"), dtype=torch.float32, device=gpu)
+            if with_rho:
+                pm.readout_tiled_grad4(phi, parts, F, rho=rho, rho_out=R, force_scale=0.8,
+                                       rho_scale=1.3)
+            else:
+                pm.readout_tiled_grad4(phi, parts, F, force_scale=0.8)
+            return F, R
+
+        for with_rho in (True, False):
+            pm.set_variant(span=0, readout=0, rho=0)
+            ref = run(with_rho)
+            assert torch.isfinite(ref[0]).all()
+            for nt in (0, 2, 3):
+                for r in (0, 1):
+                    pm.set_variant(span=1, readout=nt, rho=r)
+                    got = run(with_rho)
+                    assert torch.equal(got[0], ref[0]), (nmesh, with_rho, nt, r)
+                    if with_rho:
+                        assert torch.equal(got[1], ref[1]), (nmesh, nt, r)
+    pm.set_variant(span=0, readout=0, rho=0)
+
+
 def test_autotune_chooses_valid_variants(gpu):
     from fastpm.pm import ParticleMesh
     from fastpm import tune
@@ -116,7 +172,7 @@ def test_autotune_chooses_valid_variants(gpu):
     v = tune.autotune(pm, parts)
     assert v["fft_cols"] in (1, 2) and v["z_lines"] in (0, 1, 2)
     assert v["bin"] in (0, 1, 2) and v["paint"] in (0, 1, 2) and v["readout"] in (1, 2, 3)
-    assert v["rho"] in (0, 1)
+    assert v["rho"] in (0, 1) and v["span"] in (0, 1)
     # a second mesh of the same shape takes the cached choice without timing
     pm2 = ParticleMesh(Nmesh=(64, 64, 64), BoxSize=50.0, dtype="f4")
     assert tune.autotune(pm2, _lattice_parts(pm2, 2, np.random.default_rng(2), 1.0, gpu)) == v

@@ -1874,7 +1874,7 @@ __global__ __launch_bounds__(NT) void k_batch_paint(const TileBox<T>* __restrict
   paint_tiles_body<T, ORDER, NT>(x.P, x.g, x.w, x.rho, x.mass_fs, x.sh, x.inv_fx, j, cnt[b]);
 }
 
-template <typename T, int ORDER, int NT, bool RLDS>
+template <typename T, int ORDER, int NT, bool RLDS, int SPAN>
 __global__ __launch_bounds__(NT) void k_batch_readout(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total) {
   const int64_t it = blockIdx.x;
   if (it >= total) return;
@@ -1882,14 +1882,14 @@ __global__ __launch_bounds__(NT) void k_batch_readout(const TileBox<T>* __restri
   const int64_t j = it - pre[b];
   if (j >= cnt[b]) return;
   const TileBox<T>& x = tab[b];
-  readout_grad_body<T, ORDER, NT, RLDS>(x.P, x.g, x.w, x.G, j, cnt[b]);
+  readout_grad_body<T, ORDER, NT, RLDS, SPAN>(x.P, x.g, x.w, x.G, j, cnt[b]);
 }
 
 int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
                      std::vector<BatchLaunch>& bin, std::vector<BatchLaunch>& paint,
                      std::vector<BatchLaunch>& readout) {
   std::vector<TileBox<float>> args(nbox);
-  std::vector<int> order(nbox), pnt(nbox), rnt(nbox), rlds(nbox);
+  std::vector<int> order(nbox), pnt(nbox), rnt(nbox), rlds(nbox), rspan(nbox);
   for (int i = 0; i < nbox; ++i) {
     const fpm_batch_box& bx = boxes[i];
     TileBox<float>& x = args[i];
@@ -1908,6 +1908,8 @@ int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     pnt[i] = paint_nt(&bx.mesh);
     rnt[i] = grad_nt(&bx.mesh, bx.order);
     rlds[i] = grad_rho_lds(&bx.mesh);
+    rspan[i] = grad_span(&bx.mesh, x.g);
+    if (rspan[i] == 2 && rnt[i] < 512) rnt[i] = 512;  // as in the box's own launch
   }
   // keys carry the box index through the grouping lambdas
   auto idx = [&](const TileBox<float>& x) { return (int)(&x - args.data()); };
@@ -1933,9 +1935,12 @@ int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
   if ((rc = batch_groups(
            args, kTbReadout,
            [&](const TileBox<float>& x) {
-             return BatchKey{order[idx(x)], rnt[idx(x)], rlds[idx(x)], 0};
+             return BatchKey{order[idx(x)], rnt[idx(x)], rlds[idx(x)], rspan[idx(x)]};
            },
-           [&](const TileBox<float>& x) { return x.P.n > 0 ? x.g.ntiles : (int64_t)0; }, arena,
+           [&](const TileBox<float>& x) {
+             return x.P.n > 0 ? x.g.ntiles / rspan[idx(x)] : (int64_t)0;
+           },
+           arena,
            readout)))
     return rc;
   return FPM_OK;
@@ -1988,16 +1993,19 @@ int tile_batch_run(const std::vector<BatchLaunch>& ls, hipStream_t st) {
         }
         break;
       case kTbReadout:
-        switch ((L.b * 4 + L.a) * 2 + L.c) {
-#define FPM_BREAD(O_, NT_, R_)                                                                 \
-  case (NT_ * 4 + O_) * 2 + R_:                                                                \
-    hipLaunchKernelGGL((k_batch_readout<float, O_, NT_, R_>), grid, dim3(NT_), 0, st, tab,       \
+        switch (((L.b * 4 + L.a) * 2 + L.c) * 4 + L.d) {
+#define FPM_BREAD(O_, NT_, R_, S_)                                                             \
+  case ((NT_ * 4 + O_) * 2 + R_) * 4 + S_:                                                     \
+    hipLaunchKernelGGL((k_batch_readout<float, O_, NT_, R_, S_>), grid, dim3(NT_), 0, st, tab,   \
                        L.pre, L.cnt, L.nb, L.total);                                           \
     break;
-          FPM_BREAD(2, 256, 1) FPM_BREAD(3, 256, 1) FPM_BREAD(2, 512, 1) FPM_BREAD(3, 512, 1)
-          FPM_BREAD(2, 1024, 1) FPM_BREAD(3, 1024, 1)
-          FPM_BREAD(2, 256, 0) FPM_BREAD(3, 256, 0) FPM_BREAD(2, 512, 0) FPM_BREAD(3, 512, 0)
-          FPM_BREAD(2, 1024, 0) FPM_BREAD(3, 1024, 0)
+          FPM_BREAD(2, 256, 1, 1) FPM_BREAD(3, 256, 1, 1) FPM_BREAD(2, 512, 1, 1)
+          FPM_BREAD(3, 512, 1, 1) FPM_BREAD(2, 1024, 1, 1) FPM_BREAD(3, 1024, 1, 1)
+          FPM_BREAD(2, 256, 0, 1) FPM_BREAD(3, 256, 0, 1) FPM_BREAD(2, 512, 0, 1)
+          FPM_BREAD(3, 512, 0, 1) FPM_BREAD(2, 1024, 0, 1) FPM_BREAD(3, 1024, 0, 1)
+          FPM_BREAD(2, 512, 1, 2) FPM_BREAD(3, 512, 1, 2) FPM_BREAD(2, 1024, 1, 2)
+          FPM_BREAD(3, 1024, 1, 2) FPM_BREAD(2, 512, 0, 2) FPM_BREAD(3, 512, 0, 2)
+          FPM_BREAD(2, 1024, 0, 2) FPM_BREAD(3, 1024, 0, 2)
 #undef FPM_BREAD
           default: return FPM_ERR_ARG;
         }

@@ -5,9 +5,7 @@
 meanings, ownership rules and errors; the arrays of a state are device tensors
 and every particle or mesh operation runs in libfpm's HIP kernels.
 """
-import json
 import ctypes
-import os
 
 import numpy as np
 import torch
@@ -77,53 +75,57 @@ class StateVector:
         return self.pm.paint(self.particles, hold=False)
 
     def save(self, filename, attrs={}):
-        """Snapshot directory: ``Header.json`` with the attributes written by
-        core.py:65-89 and ``1/Position.npy``, ``1/Velocity.npy`` (km/s),
-        ``1/Density.npy``.  For restarts (``StateVector.load``) it also keeps
-        the solver state: ``1/Displacement.npy``, ``1/Momentum.npy``,
-        ``1/Force.npy`` and the time tags ``a_S``, ``a_P``, ``a_F``."""
+        """Snapshot in the bigfile directory layout (core.py:65-89): a
+        ``Header`` block with the attributes Om0, Tcmb0, Neff, [m_nu,] Ob0,
+        Ode0, Time, h, RSDFactor and `attrs` (those that encode), and the
+        blocks ``1/Position``, ``1/Velocity`` (km/s) and ``1/Density``.  For
+        restarts (``StateVector.load``) it also writes ``1/Displacement``,
+        ``1/Momentum``, ``1/Force`` and the time tags ``a_S``, ``a_P``,
+        ``a_F`` as Header attributes (blocks and attributes the reference
+        does not write)."""
+        from .snapshot import write_snapshot, _encode_attr
+
         a = self.a["S"]
-        os.makedirs(os.path.join(filename, "1"), exist_ok=True)
         header = {}
-        for key in ["Om0", "Tcmb0", "Neff", "Ob0", "Ode0", "m_nu"]:
-            v = getattr(self.cosmology, key, None)
-            if v is not None:
-                header[key] = v if np.isscalar(v) else list(np.ravel(v))
+        keylist = ["Om0", "Tcmb0", "Neff", "Ob0", "Ode0"]
+        if getattr(self.cosmology, "m_nu", None) is not None:
+            keylist.insert(3, "m_nu")
+        for key in keylist:
+            header[key] = getattr(self.cosmology, key)
         header["Time"] = a
-        for key in ("S", "P", "F"):
-            header["a_" + key] = self.a[key]
-        header["h"] = self.cosmology.H0 / self.H0
+        header["h"] = self.cosmology.H0 / self.H0  # relative h
         header["RSDFactor"] = 1.0 / (self.H0 * a * self.cosmology.efunc(1.0 / a - 1))
         for key in attrs:
-            try:
-                json.dumps(attrs[key])
+            try:  # best effort, as in the reference
+                _encode_attr(attrs[key])
                 header[key] = attrs[key]
             except (TypeError, ValueError):
                 pass
-        with open(os.path.join(filename, "Header.json"), "w") as f:
-            json.dump(header, f, indent=1, default=float)
-        np.save(os.path.join(filename, "1", "Position.npy"), self.X.cpu().numpy())
-        np.save(os.path.join(filename, "1", "Velocity.npy"), self.V.cpu().numpy())
-        np.save(os.path.join(filename, "1", "Density.npy"), self.RHO.cpu().numpy())
-        np.save(os.path.join(filename, "1", "Displacement.npy"), self.S.cpu().numpy())
-        np.save(os.path.join(filename, "1", "Momentum.npy"), self.P.cpu().numpy())
-        np.save(os.path.join(filename, "1", "Force.npy"), self.F.cpu().numpy())
+        for key in ("S", "P", "F"):
+            if self.a[key] is not None:
+                header["a_" + key] = self.a[key]
+        blocks = {"1/Position": self.X, "1/Velocity": self.V, "1/Density": self.RHO,
+                  "1/Displacement": self.S, "1/Momentum": self.P, "1/Force": self.F}
+        write_snapshot(filename, header, {k: v.cpu().numpy() for k, v in blocks.items()})
 
     @classmethod
     def load(cls, solver, filename, Q):
         """Restores a state written by ``save`` (the reference has no loader):
-        S, P, F, RHO and the time tags, on the Lagrangian positions Q."""
-        with open(os.path.join(filename, "Header.json")) as f:
-            header = json.load(f)
+        S, P, F, RHO and the time tags, on the Lagrangian positions Q.  Every
+        block's file sizes and byte sums are checked against its header."""
+        from .snapshot import read_snapshot
+
+        names = {"S": "1/Displacement", "P": "1/Momentum", "F": "1/Force", "RHO": "1/Density"}
+        header, arrs = read_snapshot(filename, list(names.values()))
         state = cls(solver, Q)
-        for attr, name in (("S", "Displacement"), ("P", "Momentum"), ("F", "Force"),
-                           ("RHO", "Density")):
-            arr = np.load(os.path.join(filename, "1", name + ".npy"))
+        for attr, name in names.items():
+            arr = arrs[name]
             t = getattr(state, attr)
             if tuple(arr.shape) != tuple(t.shape):
                 raise ValueError("%s has shape %s, expected %s" % (name, arr.shape, tuple(t.shape)))
-            t.copy_(torch.from_numpy(arr).to(dtype=t.dtype))
-        state.a = {key: header["a_" + key] for key in ("S", "P", "F")}
+            t.copy_(torch.from_numpy(np.ascontiguousarray(arr)).to(dtype=t.dtype))
+        state.a = {key: (float(header["a_" + key][0]) if "a_" + key in header else None)
+                   for key in ("S", "P", "F")}
         return state
 
 

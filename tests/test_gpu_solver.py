@@ -154,8 +154,13 @@ def test_state_api(gpu, tmp_path):
     m = st.to_mesh()
     assert abs(float(m.value.double().sum()) - 8 ** 3) < 1e-2
     st.save(str(tmp_path / "snap"))
-    X = np.load(tmp_path / "snap" / "1" / "Position.npy")
-    assert X.shape == (512, 3)
+    from fastpm.snapshot import Block
+
+    X = Block(str(tmp_path / "snap"), "1/Position").read()
+    assert X.shape == (512, 3) and X.dtype == np.float64
+    assert np.array_equal(X, st.X.cpu().numpy())
+    head = Block(str(tmp_path / "snap"), "Header").attrs()
+    assert float(head["Time"][0]) == st.a["S"] and float(head["Om0"][0]) == 0.3
 
 
 def test_pipeline_monitor_outputs(gpu, tmp_path):
@@ -193,7 +198,7 @@ def test_pipeline_monitor_outputs(gpu, tmp_path):
     lin = np.loadtxt(tmp_path / "power-linear.txt")
     assert lin.shape == (4, 4)
     snap = tmp_path / "fpm-1.0000"
-    assert (snap / "Header.json").exists() and (snap / "1" / "Position.npy").exists()
+    assert (snap / "Header" / "attr-v2").exists() and (snap / "1" / "Position" / "000000").exists()
 
 
 def test_state_restart_bitwise(gpu, tmp_path):

@@ -413,6 +413,18 @@ __device__ __forceinline__ V apply_mult(V v, decltype(V::x) m, int ngrad) {
 // Transform along a strided axis of a complex mesh.  The tile is the N points
 // of the axis times BZ adjacent z columns; `outer` enumerates the remaining
 // axis.  Element (e, outer, z) is at outer*ostride + e*estride + z.
+//
+// Nyquist super-tiles (nyq_split): a half-spectrum row has n2/2 + 1 columns,
+// so with n2/2 a multiple of BZ the last tile of every outer row would hold
+// the Nyquist column z = n2/2 alone (a workgroup doing a whole tile's work,
+// and a whole cache line fetched per element, for one column).  Instead the
+// main tiles cover z < n2/2, and nsup workgroups, dispatched before them,
+// each take the Nyquist columns of BZ consecutive outer rows as one tile
+// (column l of super-tile s is outer row s*BZ + l, at stride ostride).  A
+// column's arithmetic is the same in either kind of tile, and the
+// multiplier factors are multiplied in the same order (products and sums of
+// two factors are commutative in IEEE arithmetic), so the float32 results
+// are bitwise those of whole-row tiling.
 template <typename T>
 struct ColArgs {
   const C<T>* in;
@@ -420,6 +432,9 @@ struct ColArgs {
   int ncomp;
   int64_t estride, ostride;
   int nouter, ncols, ntz;
+  int nyq_split;   // request: Nyquist super-tiles where n2/2 divides into tiles
+  int nyq_col;     // set at launch: the Nyquist column (n2/2), 0 = none
+  int64_t nsup;    // set at launch: super-tile workgroups (before the main ones)
   T scale;
   const C<T>* tw;
   int64_t nblk;
@@ -463,9 +478,29 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
   __shared__ T kz[KOP ? BZ : 1];
   __shared__ T fo[KOP ? MAXC : 1];
 
-  const int64_t bid = xcd_remap(blk, a.nblk);
-  const int outer = (int)(bid / a.ntz);
-  const int z0 = (int)(bid % a.ntz) * BZ;
+  // main tile (outer row, BZ columns from z0) or Nyquist super-tile (outer
+  // rows from z0, the Nyquist column of each): base offset, column stride
+  // and column count of this workgroup's tile
+  // (the super-tiles come first: each gathers single elements from BZ outer
+  // rows, so they take longest and run while the main tiles stream)
+  const int64_t nmain = a.nblk - a.nsup;
+  const bool sup = blk < a.nsup;  // uniform across the workgroup
+  int outer, z0, ncols;
+  int64_t base, cs;
+  if (!sup) {
+    const int64_t bid = xcd_remap(blk - a.nsup, nmain);
+    outer = (int)(bid / a.ntz);
+    z0 = (int)(bid % a.ntz) * BZ;
+    ncols = a.ncols;
+    base = (int64_t)outer * a.ostride + z0;
+    cs = 1;
+  } else {
+    outer = a.nyq_col;
+    z0 = (int)blk * BZ;
+    ncols = a.nouter;
+    base = (int64_t)z0 * a.ostride + a.nyq_col;
+    cs = a.ostride;
+  }
 
   // the twiddle loads go out first and the tile loads second, so the tables
   // (built from each thread's own twiddles) wait only for the former
@@ -477,16 +512,17 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
     twr[q] = ((N % NT == 0) || i < N) ? a.tw[i] : cmk<V>((T)0, (T)0);
   }
   // butterfly b of the first stage: column l = b % BZ, j = b / BZ
-  const V* src = a.in + (int64_t)outer * a.ostride + z0;
+  const V* src = a.in + base;
   V v0[PER0][R0];
 #pragma unroll
   for (int q = 0; q < PER0; ++q) {
     const int b = threadIdx.x + q * NT;
     const int l = b % BZ, j = b / BZ;
-    const bool ok = ((NB0 % NT == 0) || b < NB0) && z0 + l < a.ncols;
+    const bool ok = ((NB0 % NT == 0) || b < NB0) && z0 + l < ncols;
+    const int64_t lc = (int64_t)l * cs;
 #pragma unroll
     for (int r = 0; r < R0; ++r)
-      v0[q][r] = ok ? src[(int64_t)(j + r * (N / R0)) * a.estride + l] : cmk<V>((T)0, (T)0);
+      v0[q][r] = ok ? src[(int64_t)(j + r * (N / R0)) * a.estride + lc] : cmk<V>((T)0, (T)0);
   }
 #pragma unroll
   for (int q = 0; q < PTW; ++q) {
@@ -510,21 +546,34 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
         for (int c = 0; c < nc; ++c) fe[c * N + e] = comp_factor_t<T>(K, c, a.eaxis, g0, g1, sn);
       }
     }
+    // a super-tile's columns run along the outer axis and its outer index is
+    // the Nyquist column: the tables swap roles (fo takes the z factor,
+    // fz and kz the outer axis' factor and k^2), the products stay the same
     for (int i = threadIdx.x; i < BZ * nc; i += NT) {
       const int c = i / BZ, l = i % BZ;
       double kk;
-      fz[c * BZ + l] = (T)comp_axis_factor(K, c, 2, z0 + l, true, kk);
+      if (sup)
+        fz[c * BZ + l] = (T)(K.factor[c] * comp_axis_factor(K, c, a.oaxis, z0 + l, false, kk));
+      else
+        fz[c * BZ + l] = (T)comp_axis_factor(K, c, 2, z0 + l, true, kk);
       if (c == 0) kz[l] = (T)kk;
     }
     if (threadIdx.x < nc) {
       double kk;
       const int c = threadIdx.x;
-      fo[c] = (T)(K.factor[c] * comp_axis_factor(K, c, a.oaxis, outer, false, kk));
+      fo[c] = sup ? (T)comp_axis_factor(K, c, 2, outer, true, kk)
+                  : (T)(K.factor[c] * comp_axis_factor(K, c, a.oaxis, outer, false, kk));
     }
-    const int no = K.nm[a.oaxis];
-    const int nno = outer < (no + 1) / 2 ? outer : outer - no;
-    const double kao = 6.283185307179586476925286766559 * (double)nno / K.box[a.oaxis];
-    ko = (T)(kao * kao);
+    if (sup) {
+      double kk;
+      comp_axis_factor(K, 0, 2, outer, true, kk);
+      ko = (T)kk;
+    } else {
+      const int no = K.nm[a.oaxis];
+      const int nno = outer < (no + 1) / 2 ? outer : outer - no;
+      const double kao = 6.283185307179586476925286766559 * (double)nno / K.box[a.oaxis];
+      ko = (T)(kao * kao);
+    }
     kstep = (T)(6.283185307179586476925286766559 / K.box[a.eaxis]);
   }
 
@@ -585,8 +634,8 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
         for (int r = 0; r < RS; ++r) {
           vf[q][r] = cscale(vf[q][r], a.kscale);
           const int e = NSTG == 1 ? j * R0 + r : j + r * (N / RL);
-          if (a.kout && z0 + l < a.ncols)
-            a.kout[(int64_t)outer * a.ostride + z0 + (int64_t)e * a.estride + l] = vf[q][r];
+          if (a.kout && z0 + l < ncols)
+            a.kout[base + (int64_t)e * a.estride + (int64_t)l * cs] = vf[q][r];
         }
       }
     }
@@ -625,7 +674,7 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
 
   // output c from the first-stage inputs v (overwritten)
   auto one_output = [&](int c, V (&v)[PER0][R0]) {
-    V* dst = a.out[c] + (int64_t)outer * a.ostride + z0;
+    V* dst = a.out[c] + base;
     bool lap = false;
     T r2 = (T)0;
     int ng = 0;
@@ -657,10 +706,10 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
         }
         Dft<R0, INV, V>::run(v[q]);
         if constexpr (NSTG == 1) {
-          if (z0 + l < a.ncols) {
+          if (z0 + l < ncols) {
 #pragma unroll
             for (int r = 0; r < R0; ++r)
-              dst[(int64_t)(j * R0 + r) * a.estride + l] = cscale(v[q][r], a.scale);
+              dst[(int64_t)(j * R0 + r) * a.estride + (int64_t)l * cs] = cscale(v[q][r], a.scale);
           }
         } else {
 #pragma unroll
@@ -687,10 +736,11 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
             w8[r] = cmul(w8[r], w);
           }
           Dft<RL, INV, V>::run(w8);
-          if (z0 + l < a.ncols) {
+          if (z0 + l < ncols) {
 #pragma unroll
             for (int r = 0; r < RL; ++r)
-              dst[(int64_t)(j + r * (N / RL)) * a.estride + l] = cscale(w8[r], a.scale);
+              dst[(int64_t)(j + r * (N / RL)) * a.estride + (int64_t)l * cs] =
+                  cscale(w8[r], a.scale);
           }
         }
       }
@@ -919,12 +969,27 @@ static bool supported_len(int n) {
   }
 }
 
+// Tile counts of a column pass with BZ-column tiles: whole-row tiles, or with
+// nyq_split (and n2/2 = ncols - 1 a multiple of BZ) tiles of the first
+// ncols - 1 columns plus the Nyquist super-tiles.
+template <typename T>
+static void col_tiles(ColArgs<T>& a, int BZ) {
+  a.nyq_col = 0;
+  a.nsup = 0;
+  if (a.nyq_split && a.ncols > 1 && (a.ncols - 1) % BZ == 0) {
+    a.nyq_col = a.ncols - 1;
+    a.ncols -= 1;
+    a.nsup = (a.nouter + BZ - 1) / BZ;
+  }
+  a.ntz = (a.ncols + BZ - 1) / BZ;
+  a.nblk = (int64_t)a.ntz * a.nouter + a.nsup;
+}
+
 template <typename T, int N, bool INV, bool KOP, bool MC, int WIDE>
 static void launch_col_nwm(ColArgs<T> a, hipStream_t st) {
   constexpr int BZ = ColW<T, N, WIDE, MC>::BZ;
   constexpr int NT = threads_for<T>(N * BZ);
-  a.ntz = (a.ncols + BZ - 1) / BZ;
-  a.nblk = (int64_t)a.ntz * a.nouter;
+  col_tiles(a, BZ);
   if (a.nblk <= 0) return;
   hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, INV, KOP, MC>), dim3((unsigned)a.nblk), dim3(NT),
                      0, st, a);
@@ -946,8 +1011,7 @@ template <typename T, int N, bool MC, int WIDE>
 static void launch_col_fused_nwm(ColArgs<T> a, hipStream_t st) {
   constexpr int BZ = ColW<T, N, WIDE, MC>::BZ;
   constexpr int NT = threads_for<T>(N * BZ);
-  a.ntz = (a.ncols + BZ - 1) / BZ;
-  a.nblk = (int64_t)a.ntz * a.nouter;
+  col_tiles(a, BZ);
   if (a.nblk <= 0) return;
   hipLaunchKernelGGL((k_fft_col<T, N, BZ, NT, true, true, MC, true>), dim3((unsigned)a.nblk),
                      dim3(NT), 0, st, a);
@@ -1210,6 +1274,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   y.ostride = (int64_t)n1 * pitch2;
   y.nouter = n0;
   y.ncols = nh;
+  y.nyq_split = 1;
   y.scale = (T)1;
   y.tw = static_cast<const C<T>*>(tw1);
   if ((rc = launch_col<T, false, false>(n1, y, m, st))) return rc;
@@ -1224,6 +1289,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   x.ostride = pitch2;
   x.nouter = n1;
   x.ncols = nh;
+  x.nyq_split = 1;
   x.scale = (T)1;
   x.tw = static_cast<const C<T>*>(tw0);
   x.eaxis = 0;
@@ -1523,9 +1589,8 @@ int fft_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     y.ncols = nh;
     y.scale = (T)1;
     y.tw = static_cast<const C<T>*>(bx.tw[1]);
-    const int bzy = col_bz<T>(n1, wy);
-    y.ntz = (nh + bzy - 1) / bzy;
-    y.nblk = (int64_t)y.ntz * y.nouter;
+    y.nyq_split = 1;
+    col_tiles(y, col_bz<T>(n1, wy));
     yf[i] = y;
     yi[i] = y;
     ky[i] = BatchKey{n1, wy, 0, 0};
@@ -1544,9 +1609,8 @@ int fft_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     x.eaxis = 0;
     x.oaxis = 1;
     if ((rc = fill_kop(x.kop, 1, &bx.common, m))) return rc;
-    const int bzx = col_bz<T>(n0, wx);
-    x.ntz = (nh + bzx - 1) / bzx;
-    x.nblk = (int64_t)x.ntz * x.nouter;
+    x.nyq_split = 1;
+    col_tiles(x, col_bz<T>(n0, wx));
     xf[i] = x;
     kx[i] = BatchKey{n0, wx, 0, 0};
   }

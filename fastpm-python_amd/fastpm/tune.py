@@ -113,6 +113,19 @@ def _time(fn, reps):
     return a.elapsed_time(b) / reps
 
 
+def _time_opts(opts, select, fn, reps, rounds=3):
+    """{option: seconds per call}: every option timed in each of `rounds`
+    interleaved rounds (select(option) first), keeping its fastest round, so
+    that a slow spell of the device does not land on one option only."""
+    best = {}
+    for _ in range(rounds):
+        for c in opts:
+            select(c)
+            t = _time(fn, reps)
+            best[c] = min(best.get(c, t), t)
+    return best
+
+
 def _reps(pm):
     return max(3, min(50, int(2 ** 27 // max(int(np.prod(pm.Nmesh)), 1))))
 
@@ -133,12 +146,9 @@ def autotune_fft(pm, reps=None, verbose=False):
                                             dtype=pm.tdtype))
     outs, work = [pm.create("real")], [pm._complex_tensor()]
     ops = [laplace * 1.0]
-    times = {}
-    for cols in (1, 2):
-        for zl in (0, 1, 2):
-            pm.set_variant(fft_cols=cols, z_lines=zl)
-            times[(cols, zl)] = _time(lambda: r2c_c2r_many(x, ops, kout=False, outs=outs,
-                                                           work=work), reps)
+    times = _time_opts([(cols, zl) for cols in (1, 2) for zl in (0, 1, 2)],
+                       lambda c: pm.set_variant(fft_cols=c[0], z_lines=c[1]),
+                       lambda: r2c_c2r_many(x, ops, kout=False, outs=outs, work=work), reps)
     cols, zl = min(times, key=times.get)
     best = dict(fft_cols=cols, z_lines=zl)
     pm.set_variant(**best)
@@ -168,7 +178,7 @@ def autotune_particles(pm, parts, order=None, reps=None, verbose=False):
         return hit
     rs = {2: "cic", 3: "tsc"}[order]
     assert resampler_order(rs) == order
-    reps = max(3, min(20, int(2 ** 25 // max(len(parts), 1)))) if reps is None else reps
+    reps = max(5, min(20, int(2 ** 25 // max(len(parts), 1)))) if reps is None else reps
     rho = pm.create("real")
     phi = pm.create("real")
     phi.value.normal_()
@@ -197,11 +207,11 @@ def autotune_particles(pm, parts, order=None, reps=None, verbose=False):
                                 if (n, r, sp) != (nt0, 0, 0) and not (sp and n == 1)]
         fields.append((("readout", "rho", "span"), opts, readout))
     for names, opts, fn in fields:
-        t = {}
-        for c in opts:
+        def select(c):
             pm.set_variant(**dict(zip(names, c)))
             bin_()
-            t[c] = _time(fn, reps)
+
+        t = _time_opts(opts, select, fn, reps)
         c = min(t, key=t.get)
         if t[c] > (1 - MARGIN) * t[opts[0]]:
             c = opts[0]

@@ -157,7 +157,7 @@ def test_state_api(gpu, tmp_path):
     from fastpm.snapshot import Block
 
     X = Block(str(tmp_path / "snap"), "1/Position").read()
-    assert X.shape == (512, 3) and X.dtype == np.float64
+    assert X.shape == (512, 3) and X.dtype == np.float32
     assert np.array_equal(X, st.X.cpu().numpy())
     head = Block(str(tmp_path / "snap"), "Header").attrs()
     assert float(head["Time"][0]) == st.a["S"] and float(head["Om0"][0]) == 0.3

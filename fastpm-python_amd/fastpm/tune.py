@@ -24,11 +24,12 @@ rounding for the transform tilings).  `autotune` times them on the mesh by
 coordinate descent (the transform fields together on the Force's transform,
 then each particle-kernel field on the binning, paint and readout of a
 particle set) and writes the choice into the mesh descriptor.  A particle
-field keeps its default unless another choice is faster by MARGIN: the
-particles are timed as they are when tuning runs (usually the initial
-conditions), and the defaults are the choices measured best on the clustered
-late-time distribution (at a = 0.1 the binning walks differ by 1%, at a = 1
-blocks beat patches by 12%).
+field takes the first choice, in an order of preference that starts with
+its default, whose time is within MARGIN of the fastest: the particles are
+timed as they are when tuning runs (usually the initial conditions), and the
+preferred choices are those measured best on the clustered late-time
+distribution (at a = 0.1 the binning walks differ by 1%, at a = 1 blocks
+beat patches by 12%).
 Choices are cached per (precision, mesh shape, window order, lattice) for the
 process and, when FASTPM_AMD_TUNE_CACHE names a JSON file, across processes.
 """
@@ -42,7 +43,7 @@ from . import _lib
 
 _CACHE = {}
 
-MARGIN = 0.05  # a particle field leaves its default only for a gain above this
+MARGIN = 0.05  # a particle field leaves a preferred choice only for a gain above this
 
 
 def _key(pm, order=None, parts=None):
@@ -197,13 +198,15 @@ def autotune_particles(pm, parts, order=None, reps=None, verbose=False):
     def readout():
         pm.readout_tiled_grad4(phi, parts, F, rho=rho, rho_out=R, resampler=rs)
 
-    # (fields, choices with the default first, timed call); the readout's
-    # thread count, density source and span are searched together (two-tile
-    # spans with 512 or 1024 threads)
+    # (fields, choices in order of preference with the default first, timed
+    # call); the readout's thread count, density source and span are searched
+    # together (two-tile spans with 512 or 1024 threads), the density box in
+    # LDS preferred over reads through the cache (measured at a = 1 on 512^3:
+    # 504 vs 596 us, while they tie on the initial conditions the tuner sees)
     fields = [(("bin",), [(0,), (1,), (2,)], bin_), (("paint",), [(0,), (1,), (2,)], paint)]
     if stencil:
         nt0 = 3 if order == 2 else 2
-        opts = [(nt0, 0, 0)] + [(n, r, sp) for sp in (0, 1) for r in (0, 1) for n in (1, 2, 3)
+        opts = [(nt0, 0, 0)] + [(n, r, sp) for r in (0, 1) for sp in (1, 0) for n in (3, 2, 1)
                                 if (n, r, sp) != (nt0, 0, 0) and not (sp and n == 1)]
         fields.append((("readout", "rho", "span"), opts, readout))
     for names, opts, fn in fields:
@@ -212,9 +215,9 @@ def autotune_particles(pm, parts, order=None, reps=None, verbose=False):
             bin_()
 
         t = _time_opts(opts, select, fn, reps)
-        c = min(t, key=t.get)
-        if t[c] > (1 - MARGIN) * t[opts[0]]:
-            c = opts[0]
+        # the first choice in order of preference within MARGIN of the fastest
+        best = min(t.values())
+        c = next(o for o in opts if t[o] <= (1 + MARGIN) * best)
         pm.set_variant(**dict(zip(names, c)))
         choices.update(dict(zip(names, c)))
         times["/".join(names)] = t

@@ -423,8 +423,16 @@ __device__ __forceinline__ V apply_mult(V v, decltype(V::x) m, int ngrad) {
 // (column l of super-tile s is outer row s*BZ + l, at stride ostride).  A
 // column's arithmetic is the same in either kind of tile, and the
 // multiplier factors are multiplied in the same order (products and sums of
-// two factors are commutative in IEEE arithmetic), so the float32 results
-// are bitwise those of whole-row tiling.
+// two factors are commutative in IEEE arithmetic).
+//
+// Nyquist plane (nyq_plane, with nyq_split): the Nyquist elements are not
+// kept at column n2/2 of their own rows (one cache line per element) but 16
+// to a line in the row padding: the element of row r (r = x n1 + y) at
+// column n2/2 + (r mod 16) of row r / 16, which needs n2/2 and n1 multiples
+// of 16 and a row pitch of at least n2/2 + 16.  The z passes write and read
+// them there; a y-pass super-tile then takes its elements (y) in blocks of
+// 16 per line (nyq_eblk), an x-pass super-tile its columns (y).  The
+// spectrum stored to kout keeps the usual layout.
 template <typename T>
 struct ColArgs {
   const C<T>* in;
@@ -433,8 +441,16 @@ struct ColArgs {
   int64_t estride, ostride;
   int nouter, ncols, ntz;
   int nyq_split;   // request: Nyquist super-tiles where n2/2 divides into tiles
+  int nyq_plane;   // request: the Nyquist elements in plane layout (see above)
+  int nyq_eblk;    // plane layout: this pass transforms along y (1) or x (0)
   int nyq_col;     // set at launch: the Nyquist column (n2/2), 0 = none
   int64_t nsup;    // set at launch: super-tile workgroups (before the main ones)
+  // set at launch: element e, column c of a super-tile at nyq_col plus
+  // (sup_eb ? blk(e) : e sup_es) + (sup_cb ? blk(c) : c sup_cs), where
+  // blk(i) = (i / 16) sup_p + i mod 16; its kout element at nyq_col + e
+  // estride + c ostride (the usual layout)
+  int64_t sup_es, sup_cs, sup_p;
+  int sup_eb, sup_cb;
   T scale;
   const C<T>* tw;
   int64_t nblk;
@@ -486,7 +502,8 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
   const int64_t nmain = a.nblk - a.nsup;
   const bool sup = blk < a.nsup;  // uniform across the workgroup
   int outer, z0, ncols;
-  int64_t base, cs;
+  int64_t base, cs, es, kbase, kcs;
+  bool eb = false;
   if (!sup) {
     const int64_t bid = xcd_remap(blk - a.nsup, nmain);
     outer = (int)(bid / a.ntz);
@@ -494,13 +511,28 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
     ncols = a.ncols;
     base = (int64_t)outer * a.ostride + z0;
     cs = 1;
+    es = a.estride;
+    kbase = base;
+    kcs = 1;
   } else {
     outer = a.nyq_col;
     z0 = (int)blk * BZ;
     ncols = a.nouter;
-    base = (int64_t)z0 * a.ostride + a.nyq_col;
-    cs = a.ostride;
+    // a tile of BZ <= 16 columns from z0 (a multiple of BZ) stays in one
+    // block of 16, so blocked columns are contiguous within it
+    base = a.nyq_col + (a.sup_cb ? (int64_t)(z0 >> 4) * a.sup_p + (z0 & 15)
+                                 : (int64_t)z0 * a.sup_cs);
+    cs = a.sup_cb ? 1 : a.sup_cs;
+    es = a.sup_es;
+    eb = a.sup_eb != 0;
+    kbase = a.nyq_col + (int64_t)z0 * a.ostride;
+    kcs = a.ostride;
   }
+  // offset of element e of a column (blocked along y in a plane-layout
+  // y-pass super-tile)
+  auto eoff = [&](int e) -> int64_t {
+    return eb ? (int64_t)(e >> 4) * a.sup_p + (e & 15) : (int64_t)e * es;
+  };
 
   // the twiddle loads go out first and the tile loads second, so the tables
   // (built from each thread's own twiddles) wait only for the former
@@ -522,7 +554,7 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
     const int64_t lc = (int64_t)l * cs;
 #pragma unroll
     for (int r = 0; r < R0; ++r)
-      v0[q][r] = ok ? src[(int64_t)(j + r * (N / R0)) * a.estride + lc] : cmk<V>((T)0, (T)0);
+      v0[q][r] = ok ? src[eoff(j + r * (N / R0)) + lc] : cmk<V>((T)0, (T)0);
   }
 #pragma unroll
   for (int q = 0; q < PTW; ++q) {
@@ -635,7 +667,7 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
           vf[q][r] = cscale(vf[q][r], a.kscale);
           const int e = NSTG == 1 ? j * R0 + r : j + r * (N / RL);
           if (a.kout && z0 + l < ncols)
-            a.kout[base + (int64_t)e * a.estride + (int64_t)l * cs] = vf[q][r];
+            a.kout[kbase + (int64_t)e * a.estride + (int64_t)l * kcs] = vf[q][r];
         }
       }
     }
@@ -709,7 +741,7 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
           if (z0 + l < ncols) {
 #pragma unroll
             for (int r = 0; r < R0; ++r)
-              dst[(int64_t)(j * R0 + r) * a.estride + (int64_t)l * cs] = cscale(v[q][r], a.scale);
+              dst[eoff(j * R0 + r) + (int64_t)l * cs] = cscale(v[q][r], a.scale);
           }
         } else {
 #pragma unroll
@@ -739,8 +771,7 @@ __device__ __forceinline__ void fft_col_body(const ColArgs<T>& a, int64_t blk) {
           if (z0 + l < ncols) {
 #pragma unroll
             for (int r = 0; r < RL; ++r)
-              dst[(int64_t)(j + r * (N / RL)) * a.estride + (int64_t)l * cs] =
-                  cscale(w8[r], a.scale);
+              dst[eoff(j + r * (N / RL)) + (int64_t)l * cs] = cscale(w8[r], a.scale);
           }
         }
       }
@@ -786,7 +817,16 @@ struct ZArgs {
   // zorder on an axis of n2 points and side L2
   int zgrad, zorder;
   double L2;
+  // the Nyquist mode of row r at column M + (r mod 16) of row r / 16 (the
+  // column passes' plane layout) instead of column M of row r
+  int nyq_plane;
 };
+
+// Offset of row `row`'s Nyquist mode (column M).
+template <typename T>
+__device__ __forceinline__ int64_t z_nyq_at(const ZArgs<T>& a, int64_t row, int M) {
+  return a.nyq_plane ? M + (row >> 4) * a.pitch2 + (row & 15) : row * a.pitch2 + M;
+}
 
 #ifndef FPM_Z_PAD
 #define FPM_Z_PAD 4  // LDS padding shift of the z-pass lines
@@ -855,7 +895,8 @@ __device__ __forceinline__ void fft_z_r2c_body(const ZArgs<T>& a, int64_t blk) {
       const int64_t row = row0 + l;
       if (row < a.nrows) {
         a.cout[row * a.pitch2 + k] = xa;
-        if (2 * k != M) a.cout[row * a.pitch2 + (M - k)] = xb;
+        if (k == 0) a.cout[z_nyq_at(a, row, M)] = xb;
+        else if (2 * k != M) a.cout[row * a.pitch2 + (M - k)] = xb;
       }
     }
   }
@@ -897,7 +938,7 @@ __device__ __forceinline__ void fft_z_c2r_body(const ZArgs<T>& a, int64_t blk) {
       V xa = cmk<V>((T)0, (T)0), xb = xa;
       if (row < a.nrows) {
         xa = a.cin[row * a.pitch2 + k];
-        xb = a.cin[row * a.pitch2 + (M - k)];
+        xb = a.cin[k == 0 ? z_nyq_at(a, row, M) : row * a.pitch2 + (M - k)];
       }
       if (a.zgrad) {
         T kk, g0, g1, sn;
@@ -980,6 +1021,23 @@ static void col_tiles(ColArgs<T>& a, int BZ) {
     a.nyq_col = a.ncols - 1;
     a.ncols -= 1;
     a.nsup = (a.nouter + BZ - 1) / BZ;
+    // Nyquist column of each row: columns are outer rows, elements along e
+    a.sup_es = a.estride;
+    a.sup_cs = a.ostride;
+    a.sup_eb = a.sup_cb = 0;
+    a.sup_p = 0;
+    if (a.nyq_plane && a.nyq_eblk) {
+      // y pass (estride = pitch, ostride = n1 pitch): rows x n1 + y, y in
+      // blocks of 16 per padding line, x at n1 / 16 lines each
+      a.sup_p = a.estride;
+      a.sup_cs = a.ostride / 16;
+      a.sup_eb = 1;
+    } else if (a.nyq_plane) {
+      // x pass (estride = n1 pitch, ostride = pitch): columns y blocked
+      a.sup_p = a.ostride;
+      a.sup_es = a.estride / 16;
+      a.sup_cb = 1;
+    }
   }
   a.ntz = (a.ncols + BZ - 1) / BZ;
   a.nblk = (int64_t)a.ntz * a.nouter + a.nsup;
@@ -1103,6 +1161,13 @@ static int check_mesh(const fpm_mesh* m, int64_t pitch2) {
   if (!supported_len(m->nmesh[0]) || !supported_len(m->nmesh[1])) return FPM_ERR_UNSUPPORTED;
   if (m->nmesh[2] % 2 || !supported_len(m->nmesh[2] / 2)) return FPM_ERR_UNSUPPORTED;
   return FPM_OK;
+}
+
+// Whether a mesh's transform chain can keep its Nyquist modes in plane
+// layout (ColArgs): n2/2 and n1 multiples of 16, and 16 columns of padding.
+static int nyq_plane_ok(int n1, int n2, int64_t pitch2) {
+  const int M = n2 / 2;
+  return M % 16 == 0 && n1 % 16 == 0 && pitch2 >= M + 16 ? 1 : 0;
 }
 
 template <typename T>
@@ -1257,13 +1322,17 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   }
   const int n0 = m->nmesh[0], n1 = m->nmesh[1], n2 = m->nmesh[2];
   const int nh = n2 / 2 + 1;
-  C<T>* ck = static_cast<C<T>*>(kout ? kout : work[0]);
+  // Nyquist modes in plane layout where the mesh allows; the forward passes
+  // then run in work[0], so that kout keeps the usual layout
+  const int plane = nyq_plane_ok(n1, n2, pitch2);
+  C<T>* ck = static_cast<C<T>*>(kout && !plane ? kout : work[0]);
   ZArgs<T> z{};
   z.rin = static_cast<const T*>(rin);
   z.cout = ck;
   z.nrows = (int64_t)n0 * n1;
   z.pitch2 = pitch2;
   z.tw = static_cast<const C<T>*>(tw2);
+  z.nyq_plane = plane;
   if ((rc = launch_z<T, false>(n2, z, m, st))) return rc;
   FPM_LAUNCH_CHECK();
   ColArgs<T> y{};
@@ -1275,6 +1344,8 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   y.nouter = n0;
   y.ncols = nh;
   y.nyq_split = 1;
+  y.nyq_plane = plane;
+  y.nyq_eblk = 1;
   y.scale = (T)1;
   y.tw = static_cast<const C<T>*>(tw1);
   if ((rc = launch_col<T, false, false>(n1, y, m, st))) return rc;
@@ -1290,6 +1361,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   x.nouter = n1;
   x.ncols = nh;
   x.nyq_split = 1;
+  x.nyq_plane = plane;
   x.scale = (T)1;
   x.tw = static_cast<const C<T>*>(tw0);
   x.eaxis = 0;
@@ -1310,6 +1382,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
     zi.nrows = (int64_t)n0 * n1;
     zi.pitch2 = pitch2;
     zi.tw = static_cast<const C<T>*>(tw2);
+    zi.nyq_plane = plane;
     if ((rc = launch_z<T, true>(n2, zi, m, st))) return rc;
     FPM_LAUNCH_CHECK();
   }
@@ -1563,12 +1636,14 @@ int fft_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     const int nh = n2 / 2 + 1;
     C<T>* wk = static_cast<C<T>*>(bx.work);
     const int zv = FPM_VAR_FIELD(m->variant, FPM_VAR_Z_LINES);
+    const int plane = nyq_plane_ok(n1, n2, bx.pitch2);  // as in the box's own transform
     ZArgs<T> z{};
     z.rin = static_cast<const T*>(bx.rho);
     z.cout = wk;
     z.nrows = (int64_t)n0 * n1;
     z.pitch2 = bx.pitch2;
     z.tw = static_cast<const C<T>*>(bx.tw[2]);
+    z.nyq_plane = plane;
     zf[i] = z;
     ZArgs<T> zc{};
     zc.cin = wk;
@@ -1576,6 +1651,7 @@ int fft_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     zc.nrows = (int64_t)n0 * n1;
     zc.pitch2 = bx.pitch2;
     zc.tw = static_cast<const C<T>*>(bx.tw[2]);
+    zc.nyq_plane = plane;
     zi[i] = zc;
     kz[i] = BatchKey{n2 / 2, zv, 0, 0};
     const int wy = col_choice(m, (int)sizeof(T), n1), wx = col_choice(m, (int)sizeof(T), n0);
@@ -1590,6 +1666,8 @@ int fft_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     y.scale = (T)1;
     y.tw = static_cast<const C<T>*>(bx.tw[1]);
     y.nyq_split = 1;
+    y.nyq_plane = plane;
+    y.nyq_eblk = 1;
     col_tiles(y, col_bz<T>(n1, wy));
     yf[i] = y;
     yi[i] = y;
@@ -1610,6 +1688,7 @@ int fft_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     x.oaxis = 1;
     if ((rc = fill_kop(x.kop, 1, &bx.common, m))) return rc;
     x.nyq_split = 1;
+    x.nyq_plane = plane;
     col_tiles(x, col_bz<T>(n0, wx));
     xf[i] = x;
     kx[i] = BatchKey{n0, wx, 0, 0};

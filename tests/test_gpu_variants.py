@@ -1,7 +1,9 @@
 """The compiled kernel variants a mesh can select (fpm.h FPM_VAR_*, chosen by
 fastpm.tune) against the oracle and against each other (GPU only): the z
 transform pass with 1024 / 2048 / 4096 elements per workgroup and both column
-tilings to rounding against the oracle; the binning walks, paint and readout
+tilings to rounding against the oracle (on meshes whose Nyquist modes take
+super-tiles in plane layout, in row layout with a partial last super-tile,
+or whole-row tiles); the binning walks, paint and readout
 thread counts and the two-tile readout bit for bit against the default
 variant (which the other tests check against the oracle) and the paint
 against the oracle; the tuner keeps a valid choice."""
@@ -19,7 +21,8 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("dtype", ["f4", "f8"])
-@pytest.mark.parametrize("nmesh", [(32, 32, 32), (64, 48, 96), (16, 128, 512), (96, 24, 8)])
+@pytest.mark.parametrize("nmesh", [(32, 32, 32), (64, 48, 96), (16, 128, 512), (96, 24, 8),
+                                   (48, 24, 64), (96, 12, 32)])
 def test_transform_variants_vs_oracle(gpu, nmesh, dtype):
     from fastpm.pm import ParticleMesh, r2c_c2r_many, c2r_many
     from fastpm.force.kernels import laplace, gradient, compensation

@@ -849,7 +849,10 @@ __device__ __forceinline__ void fft_z_r2c_body(const ZArgs<T>& a, int64_t blk) {
   constexpr int NSTG = Plan<M>::ns;
   constexpr int R0 = Plan<M>::r[0];
   constexpr int NB0 = NL * (M / R0), PER0 = (NB0 + NT - 1) / NT;
-  constexpr int NP = M / 2 + 1;                 // pairs (k, M-k), k = 0..M/2
+  // pairs (k, M-k) for k < (M + 1) / 2; for even M the task k = 0 also
+  // takes the middle mode, X_{M/2} = conj Z_{M/2}, so that NL * M/2 tasks
+  // fill the threads' last round instead of spilling 1 per line into another
+  constexpr int NP = (M + 1) / 2;
   constexpr int PERP = (NL * NP + NT - 1) / NT;
   __shared__ V s[NL * LP];
   __shared__ V tw[2 * M];
@@ -895,8 +898,12 @@ __device__ __forceinline__ void fft_z_r2c_body(const ZArgs<T>& a, int64_t blk) {
       const int64_t row = row0 + l;
       if (row < a.nrows) {
         a.cout[row * a.pitch2 + k] = xa;
-        if (k == 0) a.cout[z_nyq_at(a, row, M)] = xb;
-        else if (2 * k != M) a.cout[row * a.pitch2 + (M - k)] = xb;
+        if (k == 0) {
+          a.cout[z_nyq_at(a, row, M)] = xb;
+          if (M % 2 == 0) a.cout[row * a.pitch2 + M / 2] = cconj(s[l * LP + padx<PAD>(M / 2)]);
+        } else {
+          a.cout[row * a.pitch2 + (M - k)] = xb;
+        }
       }
     }
   }
@@ -917,11 +924,30 @@ __device__ __forceinline__ void fft_z_c2r_body(const ZArgs<T>& a, int64_t blk) {
   constexpr int NSTG = Plan<M>::ns;
   constexpr int RL = Plan<M>::r[NSTG - 1];
   constexpr int NBL = NL * (M / RL), PERL = (NBL + NT - 1) / NT;
-  constexpr int NP = M / 2 + 1;
+  // pairs (k, M-k) for k < (M + 1) / 2; for even M the task k = 0 also
+  // packs the middle mode, Z_{M/2} = 2 conj X_{M/2}
+  constexpr int NP = (M + 1) / 2;
   constexpr int PERP = (NL * NP + NT - 1) / NT;
   __shared__ V s[NL * LP];
   __shared__ V tw[2 * M];
   const int64_t row0 = blk * NL;
+  // every pair's loads go out first (they do not need the twiddles), then
+  // the twiddle table is staged while they are in flight
+  V xa[PERP], xb[PERP], xm[PERP];
+#pragma unroll
+  for (int q = 0; q < PERP; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    xa[q] = xb[q] = xm[q] = cmk<V>((T)0, (T)0);
+    if (idx < NL * NP) {
+      const int l = idx / NP, k = idx % NP;
+      const int64_t row = row0 + l;
+      if (row < a.nrows) {
+        xa[q] = a.cin[row * a.pitch2 + k];
+        xb[q] = a.cin[k == 0 ? z_nyq_at(a, row, M) : row * a.pitch2 + (M - k)];
+        if (k == 0 && M % 2 == 0) xm[q] = a.cin[row * a.pitch2 + M / 2];
+      }
+    }
+  }
   for (int i = threadIdx.x; i < 2 * M; i += NT) tw[i] = a.tw[i];
   T two_pi_l = (T)0, inv6h = (T)0;
   if (a.zgrad) {
@@ -934,34 +960,39 @@ __device__ __forceinline__ void fft_z_c2r_body(const ZArgs<T>& a, int64_t blk) {
     const int idx = threadIdx.x + q * NT;
     if (idx < NL * NP) {
       const int l = idx / NP, k = idx % NP;
-      const int64_t row = row0 + l;
-      V xa = cmk<V>((T)0, (T)0), xb = xa;
-      if (row < a.nrows) {
-        xa = a.cin[row * a.pitch2 + k];
-        xb = a.cin[k == 0 ? z_nyq_at(a, row, M) : row * a.pitch2 + (M - k)];
-      }
+      V xa_ = xa[q], xb_ = xb[q];
       if (a.zgrad) {
         T kk, g0, g1, sn;
         axis_factors_tw<T>(k, 2 * M, two_pi_l, inv6h, true, tw, kk, g0, g1, sn);
-        xa = apply_mult(xa, a.zorder == 0 ? g0 : g1, 1);
+        xa_ = apply_mult(xa_, a.zorder == 0 ? g0 : g1, 1);
         axis_factors_tw<T>(M - k, 2 * M, two_pi_l, inv6h, true, tw, kk, g0, g1, sn);
-        xb = apply_mult(xb, a.zorder == 0 ? g0 : g1, 1);
+        xb_ = apply_mult(xb_, a.zorder == 0 ? g0 : g1, 1);
       }
-      if (k == 0) { xa.y = (T)0; xb.y = (T)0; }
-      const V ca = cconj(xa), cb = cconj(xb);
+      if (k == 0) { xa_.y = (T)0; xb_.y = (T)0; }
+      const V ca = cconj(xa_), cb = cconj(xb_);
       V za, zb;
       {
-        const V sum = cadd(xa, cb), dif = csub(xa, cb);
+        const V sum = cadd(xa_, cb), dif = csub(xa_, cb);
         const V t = cmul(cconj(tw[k]), dif);  // w^-k (X_k - conj X_{M-k})
         za = cmk<V>(sum.x - t.y, sum.y + t.x);
       }
       {
-        const V sum = cadd(xb, ca), dif = csub(xb, ca);
+        const V sum = cadd(xb_, ca), dif = csub(xb_, ca);
         const V t = cmul(cconj(tw[M - k]), dif);
         zb = cmk<V>(sum.x - t.y, sum.y + t.x);
       }
       s[l * LP + padx<PAD>(k)] = za;
-      if (k != 0) s[l * LP + padx<PAD>(M - k)] = zb;
+      if (k != 0) {
+        s[l * LP + padx<PAD>(M - k)] = zb;
+      } else if (M % 2 == 0) {
+        V x = xm[q];
+        if (a.zgrad) {
+          T kk, g0, g1, sn;
+          axis_factors_tw<T>(M / 2, 2 * M, two_pi_l, inv6h, true, tw, kk, g0, g1, sn);
+          x = apply_mult(x, a.zorder == 0 ? g0 : g1, 1);
+        }
+        s[l * LP + padx<PAD>(M / 2)] = cmk<V>((T)2 * x.x, (T)-2 * x.y);
+      }
     }
   }
   __syncthreads();

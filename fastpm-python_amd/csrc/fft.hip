@@ -887,12 +887,12 @@ __device__ __forceinline__ void fft_z_r2c_body(const ZArgs<T>& a, int64_t blk) {
       V xa, xb;
       {  // X_k
         const V sum = cadd(za, cb), dif = csub(za, cb);
-        const V t = cmul(tw[k], dif);  // w^k (Z_k - conj Z_{M-k})
+        const V t = cmul_fma(tw[k], dif);  // w^k (Z_k - conj Z_{M-k})
         xa = cmk<V>((T)0.5 * (sum.x + t.y), (T)0.5 * (sum.y - t.x));
       }
       {  // X_{M-k}
         const V sum = cadd(zb, ca), dif = csub(zb, ca);
-        const V t = cmul(tw[M - k], dif);
+        const V t = cmul_fma(tw[M - k], dif);
         xb = cmk<V>((T)0.5 * (sum.x + t.y), (T)0.5 * (sum.y - t.x));
       }
       const int64_t row = row0 + l;
@@ -973,12 +973,12 @@ __device__ __forceinline__ void fft_z_c2r_body(const ZArgs<T>& a, int64_t blk) {
       V za, zb;
       {
         const V sum = cadd(xa_, cb), dif = csub(xa_, cb);
-        const V t = cmul(cconj(tw[k]), dif);  // w^-k (X_k - conj X_{M-k})
+        const V t = cmul_fma(cconj(tw[k]), dif);  // w^-k (X_k - conj X_{M-k})
         za = cmk<V>(sum.x - t.y, sum.y + t.x);
       }
       {
         const V sum = cadd(xb_, ca), dif = csub(xb_, ca);
-        const V t = cmul(cconj(tw[M - k]), dif);
+        const V t = cmul_fma(cconj(tw[M - k]), dif);
         zb = cmk<V>(sum.x - t.y, sum.y + t.x);
       }
       s[l * LP + padx<PAD>(k)] = za;

@@ -25,6 +25,15 @@ template <typename V> __host__ __device__ __forceinline__ V cmk(decltype(V::x) a
 template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cmk<V>(a.x + b.x, a.y + b.y); }
 template <typename V> __device__ __forceinline__ V csub(V a, V b) { return cmk<V>(a.x - b.x, a.y - b.y); }
 template <typename V> __device__ __forceinline__ V cmul(V a, V b) { return cmk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// Complex product with its rounding fixed: each component is one explicit
+// fused multiply-add onto a rounded product, which the compiler cannot
+// contract in another way, so two kernels inlining the same body (a box's own
+// launch and a batched one) round it alike.
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <typename V> __device__ __forceinline__ V cmul_fma(V a, V b) {
+  return cmk<V>(fmadd(a.x, b.x, -(a.y * b.y)), fmadd(a.x, b.y, a.y * b.x));
+}
 template <typename V> __device__ __forceinline__ V cconj(V a) { return cmk<V>(a.x, -a.y); }
 template <typename V> __device__ __forceinline__ V cscale(V a, decltype(V::x) s) { return cmk<V>(a.x * s, a.y * s); }
 

@@ -582,7 +582,7 @@ __device__ __forceinline__ void deposit_rec(const PRec& rec, int o, const TileGe
 // rpre[64] = total); the caller synchronises.
 __device__ __forceinline__ void gather_ranges(const TileGeom& g, const Workspace& w,
                                               const int lo[3], int32_t* rstart, int32_t* rpre,
-                                              int32_t* rdir) {
+                                              int32_t* rdir, bool skip_own0 = false) {
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
     const int o = lane >> 3, m = lane & 7;
@@ -590,7 +590,7 @@ __device__ __forceinline__ void gather_ranges(const TileGeom& g, const Workspace
     const int need = ox | (oy << 1) | (oz << 2);
     int32_t b = 0, len = 0;
     const bool ok = !((ox && g.nt[0] == 1) || (oy && g.nt[1] == 1) || (oz && g.nt[2] == 1)) &&
-                    (m & need) == need;
+                    (m & need) == need && !(skip_own0 && lane == 0);
     if (ok) {
       const int tx = lo[0] / TILE, ty = lo[1] / TILE, tz = lo[2] / TILE;
       const int sx = (tx - ox + g.nt[0]) % g.nt[0];
@@ -630,9 +630,26 @@ __device__ __forceinline__ void paint_tiles_body(const PartDesc<T>& P, const Til
   const int64_t t = xcd_remap(blk, nblk);
   int lo[3];
   tile_coords(t, g, lo);
+  // the tile's own sub-bin 0 (windows inside the tile: most of its
+  // deposits) is a range known from two offsets, so its first records are
+  // loaded at once, in flight while the accumulator is cleared and the other
+  // ranges are gathered and scanned
+  constexpr int PF = 2;
+  const int32_t b0 = w.offs[t * kSub], e0 = w.offs[t * kSub + 1];
+  PRec pre[PF];
+#pragma unroll
+  for (int u = 0; u < PF; ++u) {
+    const int32_t sl = b0 + threadIdx.x + u * NT;
+    pre[u] = sl < e0 ? w.rec[sl] : PRec{};
+  }
   for (int i = threadIdx.x; i < NB; i += NT) acc[i] = 0ull;
-  gather_ranges(g, w, lo, rstart, rpre, rdir);
+  gather_ranges(g, w, lo, rstart, rpre, rdir, true);
   __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PF; ++u)
+    if (b0 + (int32_t)threadIdx.x + u * NT < e0) deposit_rec<T, ORDER>(pre[u], 0, g, acc, mass_fs, sh);
+  for (int32_t sl = b0 + threadIdx.x + PF * NT; sl < e0; sl += NT)
+    deposit_rec<T, ORDER>(w.rec[sl], 0, g, acc, mass_fs, sh);
   const int32_t total = rpre[64];
   for (int32_t f = threadIdx.x; f < total; f += NT) {
     // range r with rpre[r] <= f < rpre[r + 1]

@@ -38,6 +38,46 @@ __global__ void k_kick_drift(T* S, T* P, const T* F, int64_t n, T k0, T k1, int 
   }
 }
 
+// The same on four-element vectors (float32 arrays with 16-byte aligned
+// pointers): a quarter of the load and store instructions, the same
+// arithmetic per element.  The n % 4 tail elements go to the first threads.
+__global__ void k_kick_drift_v4(float* S, float* P, const float* F, int64_t n, float k0,
+                                float k1, int nkick, float d) {
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  const int64_t n4 = n / 4;
+  V4* S4 = reinterpret_cast<V4*>(S);
+  V4* P4 = reinterpret_cast<V4*>(P);
+  const V4* F4 = reinterpret_cast<const V4*>(F);
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int64_t i = tid; i < n4; i += (int64_t)gridDim.x * blockDim.x) {
+    V4 p = P4[i];
+    if (nkick > 0) {
+      const V4 f = F4[i];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        p[c] = madd(k0, f[c], p[c]);
+        if (nkick > 1) p[c] = madd(k1, f[c], p[c]);
+      }
+      P4[i] = p;
+    }
+    V4 x = S4[i];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = madd(d, p[c], x[c]);
+    S4[i] = x;
+  }
+  const int64_t r = n4 * 4 + tid;
+  if (r < n) {
+    float p = P[r];
+    if (nkick > 0) {
+      const float f = F[r];
+      p = madd(k0, f, p);
+      if (nkick > 1) p = madd(k1, f, p);
+      P[r] = p;
+    }
+    S[r] = madd(d, p, S[r]);
+  }
+}
+
 // Adjoint of a step boundary, traversed backwards: the drift's transpose
 // gP += d gS, then the kicks' transposes gF += k_i gP, each rounded as
 // k_axpy.  fresh = 1: gF holds no gradient yet (it was consumed by a force's
@@ -177,8 +217,17 @@ static int kick_drift_impl(void* S, void* P, const void* F, int64_t n, const dou
     return FPM_ERR_ARG;
   if (n == 0) return FPM_OK;
   const T k0 = nkick > 0 ? (T)kicks[0] : (T)0, k1 = nkick > 1 ? (T)kicks[1] : (T)0;
-  hipLaunchKernelGGL(k_kick_drift<T>, dim3(grid_elems(n)), dim3(256), 0, st, static_cast<T*>(S),
-                     static_cast<T*>(P), static_cast<const T*>(F), n, k0, k1, nkick, (T)drift);
+  auto aligned = [](const void* q) { return q == nullptr || ((uintptr_t)q & 15) == 0; };
+  if (sizeof(T) == 4 && aligned(S) && aligned(P) && aligned(F)) {
+    hipLaunchKernelGGL(k_kick_drift_v4, dim3(grid_elems((n + 3) / 4)), dim3(256), 0, st,
+                       static_cast<float*>(S), static_cast<float*>(P),
+                       static_cast<const float*>(F), n, (float)k0, (float)k1, nkick,
+                       (float)drift);
+  } else {
+    hipLaunchKernelGGL(k_kick_drift<T>, dim3(grid_elems(n)), dim3(256), 0, st,
+                       static_cast<T*>(S), static_cast<T*>(P), static_cast<const T*>(F), n, k0,
+                       k1, nkick, (T)drift);
+  }
   FPM_LAUNCH_CHECK();
   return FPM_OK;
 }

@@ -344,6 +344,7 @@ struct KdBox {
   int64_t n;
   const double* table;
   int rows, stride, close, open, drift;
+  int v4;  // S, P and F 16-byte aligned: four-element vector accesses
 };
 
 constexpr int kKdPer = 8;  // elements per thread of a batched kick-drift item
@@ -366,17 +367,46 @@ __global__ __launch_bounds__(256) void k_batch_kick_drift(const KdBox<T>* __rest
   const T k0 = prev ? (T)x.table[(int64_t)(row - 1) * x.stride + x.close] : (T)0;
   const T k1 = (T)x.table[(int64_t)row * x.stride + x.open];
   const T d = (T)x.table[(int64_t)row * x.stride + x.drift];
+  auto one = [&](int64_t i) {
+    const T f = x.F[i];
+    T p = x.P[i];
+    if (prev) p = madd(k0, f, p);
+    p = madd(k1, f, p);
+    x.P[i] = p;
+    x.S[i] = madd(d, p, x.S[i]);
+  };
+  if (x.v4) {
+    // the item's 256 kKdPer elements as kKdPer / 4 vectors per thread; the
+    // thread whose first vector is the n / 4-th takes the n % 4 tail
+    typedef T V4 __attribute__((ext_vector_type(4)));
+    V4* S4 = reinterpret_cast<V4*>(x.S);
+    V4* P4 = reinterpret_cast<V4*>(x.P);
+    const V4* F4 = reinterpret_cast<const V4*>(x.F);
+    const int64_t n4 = x.n / 4;
+#pragma unroll
+    for (int u = 0; u < kKdPer / 4; ++u) {
+      const int64_t i4 = (j * (kKdPer / 4) + u) * 256 + threadIdx.x;
+      if (i4 < n4) {
+        const V4 f = F4[i4];
+        V4 p = P4[i4], s = S4[i4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (prev) p[c] = madd(k0, f[c], p[c]);
+          p[c] = madd(k1, f[c], p[c]);
+          s[c] = madd(d, p[c], s[c]);
+        }
+        P4[i4] = p;
+        S4[i4] = s;
+      } else if (i4 == n4) {
+        for (int64_t i = n4 * 4; i < x.n; ++i) one(i);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < kKdPer; ++u) {
     const int64_t i = (j * kKdPer + u) * 256 + threadIdx.x;
-    if (i < x.n) {
-      const T f = x.F[i];
-      T p = x.P[i];
-      if (prev) p = madd(k0, f, p);
-      p = madd(k1, f, p);
-      x.P[i] = p;
-      x.S[i] = madd(d, p, x.S[i]);
-    }
+    if (i < x.n) one(i);
   }
 }
 
@@ -402,6 +432,8 @@ int kd_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     x.close = bx.close;
     x.open = bx.open;
     x.drift = bx.drift;
+    auto al = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
+    x.v4 = al(x.S) && al(x.P) && al(x.F) ? 1 : 0;
   }
   return batch_groups(args, 1, [](const KdBox<float>&) { return BatchKey{0, 0, 0, 0}; },
                       [](const KdBox<float>& x) { return (x.n + 256 * kKdPer - 1) / (256 * kKdPer); },

@@ -1347,7 +1347,7 @@ static int r2c_c2r_impl(const void* rin, void* kout, int ncomp, const fpm_kop* o
   if (ncomp < 1 || ncomp > 6 || !rin || !outs || !work || !tw0 || !tw1 || !tw2)
     return FPM_ERR_ARG;
   for (int c = 0; c < ncomp; ++c) {
-    if (!outs[c] || !work[c] || work[c] == kout) return FPM_ERR_ARG;
+    if (!outs[c] || !work[c] || work[c] == kout || work[c] == rin) return FPM_ERR_ARG;
     for (int d = 0; d < c; ++d)
       if (work[c] == work[d]) return FPM_ERR_ARG;
   }

@@ -254,9 +254,11 @@ int fpm_fft_c2r_f64(const void* in, int ncomp, const fpm_kop* ops, void* const* 
  * complex scratch meshes as in fpm_fft_c2r.  The last forward pass and the
  * first inverse pass run along the same axis, so one kernel does both and
  * the spectrum is read once less than with separate calls.  `in` is left
- * intact.  kout may be NULL when the spectrum itself is not wanted: it is
- * then not stored (one half-spectrum write less) and work[0] carries the
- * forward passes.  Replaces the Force's rho.r2c (fastpm/core.py:205) followed by
+ * intact (no work[c] may alias it).  kout may be NULL when the spectrum
+ * itself is not wanted: it is then not stored (one half-spectrum write
+ * less).  The forward passes run in work[0] (in kout instead where n2/2 or
+ * n1 is not a multiple of 16); kout, when given, receives the spectrum in
+ * the usual layout.  Replaces the Force's rho.r2c (fastpm/core.py:205) followed by
  * pot_k = delta_k.apply(laplace).apply(longrange) and its c2r
  * (fastpm/force/gravity.py:14-22). */
 int fpm_fft_r2c_c2r_f32(const void* in, void* kout, int ncomp, const fpm_kop* ops,

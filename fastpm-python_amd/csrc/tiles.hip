@@ -67,14 +67,38 @@ struct PRec {
 
 struct Workspace {
   int32_t* rank;       // each particle's slot within its sub-bin (thread order)
-  int32_t* list;       // particle index per slot
+  int32_t* list;       // particle index per slot (ncap slots)
   PRec* rec;           // particle record per slot
-  int32_t* count;
-  int32_t* offs;       // nbins + 1
-  int32_t* misplaced;  // particles whose slot the scatter found outside [0, n)
-  int32_t* btot;       // per-workgroup totals of the sub-bin count scan
+  int32_t* count;      // particles per sub-bin
+  int32_t* offs;       // first slot of each sub-bin (nbins + 1)
+  int32_t* misplaced;  // particles whose slot the exact scatter found outside [0, n)
+  int32_t* ovf;        // nonzero: the records are placed exactly (the scatter runs)
+  int64_t* hdr;        // magic, particles, sub-bins of the capacity layout in nxt
+  int32_t* btot;       // per-workgroup totals of the count scan
+  int32_t* btotc;      // per-workgroup totals of the capacity scan
+  int32_t* nxt;        // first slot of each sub-bin in the next call's capacity layout
+  int64_t ncap;        // slots of list and rec
   int64_t bytes;
+  int32_t allow_cap;   // the caller lets this bin call use the capacity layout
 };
+
+// Capacity layout.  A binning call places each record straight at its
+// sub-bin's start plus its rank when the workspace holds the capacity
+// layout its previous call left (sub-bin b gets cap(count) slots of that
+// call, the header says for how many particles and sub-bins): one pass over
+// the particles instead of a count pass and a scatter pass.  A record that
+// does not fit its sub-bin's capacity (or a workspace without that layout)
+// sets ovf, and the scan and the scatter then place every record exactly as
+// before.  Either way sub-bin b's records are slots offs[b] ..
+// offs[b] + count[b] - 1, which is all the paint and the readouts use; the
+// results do not depend on the layout.  cap(c) sums to at most ncap.
+constexpr int64_t kCapMagic = 0x46504d4341503031LL;
+__host__ __device__ __forceinline__ int32_t bin_cap(int32_t c) { return c + (c >> 2) + 32; }
+
+static int64_t capacity_slots(int64_t n, int64_t nbins) {
+  const int64_t c = n + n / 4 + 32 * nbins;
+  return c < ((int64_t)1 << 31) - 1 ? c : n;
+}
 
 static int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
@@ -85,20 +109,23 @@ constexpr int kScanBlock = kScanThreads * kScanPer;  // sub-bins per scan workgr
 static int64_t scan_blocks(const TileGeom& g) { return (g.nbins + kScanBlock - 1) / kScanBlock; }
 
 // Byte offsets of the workspace sections: rank, list, rec, count, offs,
-// misplaced, btot; off[7] is the total size.
-static void ws_offsets(const TileGeom& g, int64_t n, int64_t off[8]) {
-  const int64_t sizes[7] = {n * 4, n * 4, n * (int64_t)sizeof(PRec), g.nbins * 4,
-                            (g.nbins + 1) * 4, 4, scan_blocks(g) * 4};
+// control words (misplaced, ovf, header), btot, btotc, nxt; off[9] is the
+// total size.
+static void ws_offsets(const TileGeom& g, int64_t n, int64_t off[10]) {
+  const int64_t ncap = capacity_slots(n, g.nbins);
+  const int64_t sizes[9] = {n * 4, ncap * 4, ncap * (int64_t)sizeof(PRec), g.nbins * 4,
+                            (g.nbins + 1) * 4, 64, scan_blocks(g) * 4, scan_blocks(g) * 4,
+                            (g.nbins + 1) * 4};
   int64_t o = 0;
-  for (int i = 0; i < 7; ++i) {
+  for (int i = 0; i < 9; ++i) {
     off[i] = o;
     o += align256(sizes[i]);
   }
-  off[7] = o;
+  off[9] = o;
 }
 
 static int64_t layout_ws(const TileGeom& g, int64_t n, char* base, Workspace* w) {
-  int64_t off[8];
+  int64_t off[10];
   ws_offsets(g, n, off);
   if (w) {
     w->rank = reinterpret_cast<int32_t*>(base + off[0]);
@@ -107,10 +134,16 @@ static int64_t layout_ws(const TileGeom& g, int64_t n, char* base, Workspace* w)
     w->count = reinterpret_cast<int32_t*>(base + off[3]);
     w->offs = reinterpret_cast<int32_t*>(base + off[4]);
     w->misplaced = reinterpret_cast<int32_t*>(base + off[5]);
+    w->ovf = reinterpret_cast<int32_t*>(base + off[5] + 4);
+    w->hdr = reinterpret_cast<int64_t*>(base + off[5] + 16);
     w->btot = reinterpret_cast<int32_t*>(base + off[6]);
-    w->bytes = off[7];
+    w->btotc = reinterpret_cast<int32_t*>(base + off[7]);
+    w->nxt = reinterpret_cast<int32_t*>(base + off[8]);
+    w->ncap = capacity_slots(n, g.nbins);
+    w->bytes = off[9];
+    w->allow_cap = 0;
   }
-  return off[7];
+  return off[9];
 }
 
 // Window origin (wrapped) and the offset t it is weighted by (CIC: the
@@ -271,10 +304,10 @@ __device__ __forceinline__ int64_t bin_window(const BinOrder& o, const PartDesc<
 // bound by their latency, not by bandwidth.
 
 // Merges a wave's sub-bin keys (t[u] < 0: no particle) and adds each group
-// to its counter; rank[i] = the counter before the add + the lane's rank in
-// its group.
+// to its counter; rank[i] = rk[u] = the counter before the add + the lane's
+// rank in its group.
 __device__ __forceinline__ void count_ranks(const int32_t t[kBinILP], int64_t base, int64_t n,
-                                            const Workspace& w) {
+                                            const Workspace& w, int32_t rk[kBinILP]) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int32_t old[kBinILP];
@@ -302,7 +335,8 @@ __device__ __forceinline__ void count_ranks(const int32_t t[kBinILP], int64_t ba
   for (int u = 0; u < kBinILP; ++u) {
     const int64_t i = base + u * kBinThreads + threadIdx.x;
     const int32_t r0 = __shfl(old[u], myleader[u]);
-    if (i < n) w.rank[i] = r0 + myrank[u];
+    rk[u] = r0 + myrank[u];
+    if (i < n) w.rank[i] = rk[u];
   }
 }
 
@@ -317,7 +351,8 @@ constexpr int kBinTable = kBinThreads * kBinILP;
 static_assert((kBinTable & (kBinTable - 1)) == 0, "the sub-bin table size is a power of two");
 
 __device__ __forceinline__ void count_ranks_wg(const int32_t t[kBinILP], int64_t base, int64_t n,
-                                               const Workspace& w, int32_t* tkey, int32_t* tval) {
+                                               const Workspace& w, int32_t* tkey, int32_t* tval,
+                                               int32_t rk[kBinILP]) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __syncthreads();  // the previous span's readers of the table are done
@@ -369,36 +404,88 @@ __device__ __forceinline__ void count_ranks_wg(const int32_t t[kBinILP], int64_t
     const int64_t i = base + u * kBinThreads + threadIdx.x;
     const int32_t h = __shfl(slot[u], myleader[u]);
     const int32_t o = __shfl(off[u], myleader[u]);
-    if (i < n && t[u] >= 0) w.rank[i] = tval[h] + o + myrank[u];
+    rk[u] = tval[h] + o + myrank[u];
+    if (i < n && t[u] >= 0) w.rank[i] = rk[u];
   }
+}
+
+// A particle's binned record from its window: offsets t, origin i0.
+template <typename T>
+__device__ __forceinline__ PRec make_rec(const T t[3], const int i0[3]) {
+  PRec r;
+  r.t[0] = (float)t[0];
+  r.t[1] = (float)t[1];
+  r.t[2] = (float)t[2];
+  r.cells = (uint32_t)(i0[0] % TILE) | (uint32_t)(i0[1] % TILE) << 4 |
+            (uint32_t)(i0[2] % TILE) << 8;
+  return r;
+}
+
+// Whether this binning may use the workspace's capacity layout: the caller
+// allows it (the workspace is unchanged since the bin call that left it) and
+// the layout is for as many particles and sub-bins.
+__device__ __forceinline__ bool capacity_ready(const Workspace& w, int64_t n, int64_t nbins) {
+  return w.allow_cap && w.ncap > n && w.hdr[0] == kCapMagic && w.hdr[1] == n &&
+         w.hdr[2] == nbins;
 }
 
 // Kernel bodies take their block index and count explicitly, so one body
 // serves a box's own launch (blockIdx.x of gridDim.x) and a batched launch
 // over many boxes (the box's local block of its share of the items).
+//
+// The count pass: each particle's sub-bin is counted and its rank within
+// the sub-bin kept; with a capacity layout ready, the record and index go
+// straight to the sub-bin's capacity start plus the rank (the layout's
+// starts are copied to offs for the paint and readouts), and a record that
+// does not fit sets ovf.  Without a layout it sets ovf at once and the
+// scatter places the records.
 template <typename T, int ORDER>
 __device__ __forceinline__ void bin_count_body(const PartDesc<T>& P, const TileGeom& g,
                                                const Workspace& w, const BinOrder& o, int64_t blk,
                                                int64_t nblk) {
   __shared__ int32_t tkey[kBinTable], tval[kBinTable];
+  const bool cap = capacity_ready(w, P.n, g.nbins);  // uniform
+  if (!cap) {
+    if (blk == 0 && threadIdx.x == 0) *w.ovf = 1;
+  } else {
+    for (int64_t i = blk * kBinThreads + threadIdx.x; i <= g.nbins; i += nblk * kBinThreads)
+      w.offs[i] = w.nxt[i];
+  }
   const int64_t span = (int64_t)kBinThreads * kBinILP;
   for (int64_t base = blk * span; base < P.n; base += nblk * span) {
-    int32_t t[kBinILP];
+    int32_t t[kBinILP], rk[kBinILP], pp[kBinILP];
+    PRec r[kBinILP];
 #pragma unroll
     for (int u = 0; u < kBinILP; ++u) {
       const int64_t i = base + u * kBinThreads + threadIdx.x;
       t[u] = -1;
+      pp[u] = 0;
       if (i < P.n) {
         int i0[3];
         T tt[3];
-        bin_window<T, ORDER>(o, P, g.nm, i, i0, tt);
+        pp[u] = (int32_t)bin_window<T, ORDER>(o, P, g.nm, i, i0, tt);
         t[u] = bin_of<ORDER>(i0, g);
+        r[u] = make_rec<T>(tt, i0);
       }
     }
     if (o.blocked == kBinBlock)
-      count_ranks_wg(t, base, P.n, w, tkey, tval);
+      count_ranks_wg(t, base, P.n, w, tkey, tval, rk);
     else
-      count_ranks(t, base, P.n, w);
+      count_ranks(t, base, P.n, w, rk);
+    if (cap) {
+#pragma unroll
+      for (int u = 0; u < kBinILP; ++u) {
+        if (t[u] >= 0) {
+          const int32_t sl = w.nxt[t[u]] + rk[u];
+          if (sl >= 0 && sl < w.nxt[t[u] + 1] && sl < w.ncap) {
+            w.list[sl] = pp[u];
+            w.rec[sl] = r[u];
+          } else {
+            *w.ovf = 1;
+          }
+        }
+      }
+    }
   }
 }
 
@@ -408,10 +495,12 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_count(PartDesc<T> P, TileGe
   bin_count_body<T, ORDER>(P, g, w, o, blockIdx.x, gridDim.x);
 }
 
-// Exclusive scan of the sub-bin counts in two passes over kScanBlock-bin
-// chunks.  Pass one scans each chunk locally (into offs) and records the
-// chunk total; pass two adds the sum of the earlier chunk totals, which every
-// workgroup reduces for itself (a few hundred values).
+// Exclusive scans of the sub-bin counts (into offs, when the records are to
+// be placed exactly) and of their capacities (into nxt, the next call's
+// capacity layout) in two passes over kScanBlock-bin chunks.  Pass one scans
+// each chunk locally and records the chunk totals; pass two adds the sums of
+// the earlier chunk totals, which every workgroup reduces for itself (a few
+// hundred values), and the last workgroup writes the layout's header.
 __device__ __forceinline__ int32_t wave_incl_scan(int32_t v, int lane) {
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -422,27 +511,42 @@ __device__ __forceinline__ int32_t wave_incl_scan(int32_t v, int lane) {
 }
 
 __device__ __forceinline__ void scan_local_body(const Workspace& w, int64_t nbins, int64_t blk) {
-  __shared__ int32_t wsum[kScanThreads / 64];
+  __shared__ int32_t wsum[kScanThreads / 64], wsumc[kScanThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const bool exact = *w.ovf != 0;
   const int64_t i0 = blk * kScanBlock + (int64_t)tid * kScanPer;
   int32_t v[kScanPer];
-  int32_t s = 0;
+  int32_t s = 0, sc = 0;
 #pragma unroll
   for (int i = 0; i < kScanPer; ++i) {
     v[i] = i0 + i < nbins ? w.count[i0 + i] : 0;
     s += v[i];
+    sc += i0 + i < nbins ? bin_cap(v[i]) : 0;
   }
-  const int32_t incl = wave_incl_scan(s, lane);
-  if (lane == 63) wsum[wid] = incl;
+  const int32_t incl = wave_incl_scan(s, lane), inclc = wave_incl_scan(sc, lane);
+  if (lane == 63) {
+    wsum[wid] = incl;
+    wsumc[wid] = inclc;
+  }
   __syncthreads();
-  int32_t run = incl - s;
-  for (int k = 0; k < wid; ++k) run += wsum[k];
+  int32_t run = incl - s, runc = inclc - sc;
+  for (int k = 0; k < wid; ++k) {
+    run += wsum[k];
+    runc += wsumc[k];
+  }
 #pragma unroll
   for (int i = 0; i < kScanPer; ++i) {
-    if (i0 + i < nbins) w.offs[i0 + i] = run;
+    if (i0 + i < nbins) {
+      if (exact) w.offs[i0 + i] = run;
+      w.nxt[i0 + i] = runc;
+    }
     run += v[i];
+    runc += bin_cap(v[i]);
   }
-  if (tid == kScanThreads - 1) w.btot[blk] = run;
+  if (tid == kScanThreads - 1) {
+    w.btot[blk] = run;
+    w.btotc[blk] = runc;
+  }
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_bin_scan_local(Workspace w, int64_t nbins) {
@@ -451,39 +555,62 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan_local(Workspace w, in
 
 __device__ __forceinline__ void scan_add_body(const Workspace& w, int64_t nbins, int64_t blk,
                                               int64_t nblk) {
-  __shared__ int32_t wsum[kScanThreads / 64];
+  __shared__ int32_t wsum[kScanThreads / 64], wsumc[kScanThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  int32_t s = 0;
-  for (int b = tid; b < (int)blk; b += kScanThreads) s += w.btot[b];
+  const bool exact = *w.ovf != 0;
+  int32_t s = 0, sc = 0;
+  for (int b = tid; b < (int)blk; b += kScanThreads) {
+    s += w.btot[b];
+    sc += w.btotc[b];
+  }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) wsum[wid] = s;
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o);
+    sc += __shfl_xor(sc, o);
+  }
+  if (lane == 0) {
+    wsum[wid] = s;
+    wsumc[wid] = sc;
+  }
   __syncthreads();
-  int32_t base = 0;
+  int32_t base = 0, basec = 0;
 #pragma unroll
-  for (int k = 0; k < kScanThreads / 64; ++k) base += wsum[k];
+  for (int k = 0; k < kScanThreads / 64; ++k) {
+    base += wsum[k];
+    basec += wsumc[k];
+  }
   const int64_t c0 = blk * kScanBlock;
   for (int i = tid; i < kScanBlock; i += kScanThreads) {
     const int64_t b = c0 + i;
     if (b < nbins) {
-      w.offs[b] += base;
+      if (exact) w.offs[b] += base;
+      w.nxt[b] += basec;
     }
   }
-  if (blk == nblk - 1 && tid == 0) w.offs[nbins] = base + w.btot[blk];
+  if (blk == nblk - 1 && tid == 0) {
+    const int32_t n = base + w.btot[blk];
+    if (exact) w.offs[nbins] = n;
+    w.nxt[nbins] = basec + w.btotc[blk];
+    w.hdr[0] = w.ncap > n ? kCapMagic : 0;
+    w.hdr[1] = n;
+    w.hdr[2] = nbins;
+  }
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int64_t nbins) {
   scan_add_body(w, nbins, blockIdx.x, gridDim.x);
 }
 
-// Each particle goes to the start of its sub-bin (recomputed from its
-// position with the count's arithmetic) plus its slot from the count, with its
-// record (window offsets and tile-local origin), so paint and readout read
-// their tile's particles contiguously and need no positions.
+// Exact placement (when the count pass set ovf): each particle goes to the
+// start of its sub-bin (recomputed from its position with the count's
+// arithmetic) plus its slot from the count, with its record (window offsets
+// and tile-local origin), so paint and readout read their tile's particles
+// contiguously and need no positions.
 template <typename T, int ORDER>
 __device__ __forceinline__ void bin_scatter_body(const PartDesc<T>& P, const TileGeom& g,
                                                  const Workspace& w, const BinOrder& o,
                                                  int64_t blk, int64_t nblk) {
+  if (*w.ovf == 0) return;  // the count pass placed every record (uniform)
   const int64_t span = (int64_t)kBinThreads * kBinILP;
   for (int64_t base = blk * span; base < P.n; base += nblk * span) {
     int32_t slot[kBinILP];
@@ -507,11 +634,7 @@ __device__ __forceinline__ void bin_scatter_body(const PartDesc<T>& P, const Til
           atomicAdd(w.misplaced, 1);
         }
         pp[u] = (int32_t)p;
-        r[u].t[0] = (float)t[0];
-        r[u].t[1] = (float)t[1];
-        r[u].t[2] = (float)t[2];
-        r[u].cells = (uint32_t)(i0[0] % TILE) | (uint32_t)(i0[1] % TILE) << 4 |
-                     (uint32_t)(i0[2] % TILE) << 8;
+        r[u] = make_rec<T>(t, i0);
       }
     }
     // the count placed each particle with the same arithmetic (no FMA
@@ -532,6 +655,63 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_scatter(PartDesc<T> P, Tile
                                                              Workspace w, BinOrder o) {
   bin_scatter_body<T, ORDER>(P, g, w, o, blockIdx.x, gridDim.x);
 }
+
+// The records of NR / kSub consecutive tiles from tile t: NR sub-bin
+// ranges, range k at slots offs[t kSub + k] .. + count[t kSub + k] - 1 (one
+// run when the binning placed the records exactly, separated by spare
+// capacity otherwise).  The starts and the prefix of the lengths are read
+// through the scalar cache into registers every wave holds (the workspace
+// is not written while they are read), and record f of the tiles
+// (0 <= f < total) is found by comparing f with the prefix.
+template <int NR>
+struct TileSlots {
+  static_assert(NR <= 64, "one range per lane");
+  int32_t base0;       // start of range 0
+  int32_t pre[NR + 1]; // records before range k
+  int32_t gap;         // lane k < NR: slots of spare capacity before range k
+  bool contiguous;     // no spare capacity between the ranges (exact placement)
+  __device__ __forceinline__ void load(const Workspace& w, int64_t t) {
+    typedef const __attribute__((address_space(4))) int32_t* CP;
+    const CP offs = (CP)(w.offs + t * kSub);
+    const CP cnt = (CP)(w.count + t * kSub);
+    pre[0] = 0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) pre[k + 1] = pre[k] + cnt[k];
+    base0 = offs[0];
+    const int lane = threadIdx.x & 63;
+    gap = 0;
+    contiguous = true;
+#pragma unroll
+    for (int k = 1; k < NR; ++k) {
+      if (lane == k) gap = offs[k] - base0 - pre[k];
+      contiguous = contiguous && offs[k] == base0 + pre[k];
+    }
+  }
+  __device__ __forceinline__ int32_t total() const { return pre[NR]; }
+  // record f's range r (the count of ranges starting at or before it, less
+  // one), then its slot from range 0's start, f and the gap before r (read
+  // from lane r, or from a copy of the lanes' gaps in LDS, so that the gap
+  // register need not stay live through a kernel's busiest part)
+  __device__ __forceinline__ int range(int32_t f) const {
+    int r = 0;
+#pragma unroll
+    for (int k = 1; k < NR; ++k) r += f >= pre[k] ? 1 : 0;
+    return r;
+  }
+  // (a lane shuffle: every lane of the wave must call it, inactive source
+  // lanes would read as zero)
+  __device__ __forceinline__ int32_t slot(int32_t f) const {
+    if (contiguous) return base0 + f;  // uniform
+    return base0 + f + __shfl(gap, range(f));
+  }
+  __device__ __forceinline__ void store_gaps(int32_t* lds) const {
+    if (threadIdx.x < NR) lds[threadIdx.x] = gap;
+  }
+  __device__ __forceinline__ int32_t slot(int32_t f, const int32_t* lds) const {
+    if (contiguous) return base0 + f;  // uniform
+    return base0 + f + lds[range(f)];
+  }
+};
 
 // ---------------------------------------------------------------- paint
 // Deposits of one record into the target tile.  The record's origin is local
@@ -598,7 +778,7 @@ __device__ __forceinline__ void gather_ranges(const TileGeom& g, const Workspace
       const int sz = (tz - oz + g.nt[2]) % g.nt[2];
       const int64_t src = ((int64_t)sx * g.nt[1] + sy) * g.nt[2] + sz;
       b = w.offs[src * kSub + m];
-      len = w.offs[src * kSub + m + 1] - b;
+      len = w.count[src * kSub + m];
     }
     // inclusive scan of the lengths across the 64 lanes
     int32_t inc = len;
@@ -635,7 +815,7 @@ __device__ __forceinline__ void paint_tiles_body(const PartDesc<T>& P, const Til
   // loaded at once, in flight while the accumulator is cleared and the other
   // ranges are gathered and scanned
   constexpr int PF = 2;
-  const int32_t b0 = w.offs[t * kSub], e0 = w.offs[t * kSub + 1];
+  const int32_t b0 = w.offs[t * kSub], e0 = b0 + w.count[t * kSub];
   PRec pre[PF];
 #pragma unroll
   for (int u = 0; u < PF; ++u) {
@@ -945,16 +1125,21 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
   constexpr int NB = TB * TB * TB;
   typedef typename CellVec<T, NC>::type VC;
   __shared__ VC box[NB];
+  __shared__ int32_t sgap[kSub];
   const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
   int lo[3];
   tile_coords(t, g, lo);
-  const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
-  if (b == e) return;  // uniform across the workgroup
-  const int32_t s0 = b + threadIdx.x, s1 = s0 + kTileThreads;
-  const int64_t p0 = s0 < e ? w.list[s0] : 0;
-  const int64_t p1 = s1 < e ? w.list[s1] : 0;
-  const PRec r0 = s0 < e ? w.rec[s0] : PRec{};
-  const PRec r1 = s1 < e ? w.rec[s1] : PRec{};
+  TileSlots<kSub> ts;
+  ts.load(w, t);
+  const int32_t e = ts.total();
+  if (e == 0) return;  // uniform across the workgroup
+  const int32_t s0 = threadIdx.x, s1 = s0 + kTileThreads;
+  const int32_t q0 = ts.slot(s0), q1 = ts.slot(s1);  // used below only when < e
+  const int64_t p0 = s0 < e ? w.list[q0] : 0;
+  const int64_t p1 = s1 < e ? w.list[q1] : 0;
+  const PRec r0 = s0 < e ? w.rec[q0] : PRec{};
+  const PRec r1 = s1 < e ? w.rec[q1] : PRec{};
+  ts.store_gaps(sgap);
   BoxRegs<T, ORDER, NC> br;
   br.load(g, R, lo);
   br.store(box);
@@ -970,8 +1155,9 @@ __global__ __launch_bounds__(kTileThreads) void k_readout_tiles(PartDesc<T> P, T
     gather_one<T, ORDER, NC>(box, R, p1, l, wt);
   }
   for (int32_t s = s1 + kTileThreads; s < e; s += kTileThreads) {
-    window_rec<T, ORDER>(w.rec[s], l, wt);
-    gather_one<T, ORDER, NC>(box, R, (int64_t)w.list[s], l, wt);
+    const int32_t q = ts.slot(s, sgap);
+    window_rec<T, ORDER>(w.rec[q], l, wt);
+    gather_one<T, ORDER, NC>(box, R, (int64_t)w.list[q], l, wt);
   }
 }
 
@@ -1255,18 +1441,35 @@ __device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const Ti
   static_assert(TR * TR * TRZ <= TP * TP * TPZ, "the density box fits the potential buffer");
   __shared__ T rbox[RLDS && !PHASED ? TR * TR * TRZ : 1];
   __shared__ T pbox[TP * TP * TPZ];
+  __shared__ int32_t sgap[SPAN * kSub];
   // z is the fastest tile axis, so unit u's tiles are u * SPAN + [0, SPAN)
   const int64_t t = xcd_remap(blk, nblk) * SPAN;
   int lo[3];
   tile_coords(t, g, lo);
-  const int32_t b = w.offs[t * kSub], e = w.offs[(t + SPAN) * kSub];
-  const int32_t mid = SPAN == 1 ? e : w.offs[(t + 1) * kSub];  // first slot of the second tile
-  if (b == e) return;  // uniform across the workgroup
-  const int32_t s0 = b + threadIdx.x, s1 = s0 + NT;
-  const int64_t p0 = s0 < e ? w.list[s0] : 0;
-  const int64_t p1 = s1 < e ? w.list[s1] : 0;
-  const PRec r0 = s0 < e ? w.rec[s0] : PRec{};
-  const PRec r1 = s1 < e ? w.rec[s1] : PRec{};
+  // records 0 .. e - 1 of the tiles (index s; slot ts.slot(s)), the
+  // second tile's from mid
+  TileSlots<SPAN * kSub> ts;
+  ts.load(w, t);
+  const int32_t e = ts.total();
+  const int32_t mid = SPAN == 1 ? e : ts.pre[kSub];
+  if (e == 0) return;  // uniform across the workgroup
+  // each thread's first two particles, loaded while the first box's loads
+  // are in flight (their slots take a few instructions to find)
+  const int32_t s0 = threadIdx.x, s1 = s0 + NT;
+  int64_t p0 = 0, p1 = 0;
+  PRec r0{}, r1{};
+  auto preload = [&]() {
+    const int32_t q0 = ts.slot(s0), q1 = ts.slot(s1);  // used only when < e
+    if (s0 < e) {
+      p0 = w.list[q0];
+      r0 = w.rec[q0];
+    }
+    if (s1 < e) {
+      p1 = w.list[q1];
+      r1 = w.rec[q1];
+    }
+    ts.store_gaps(sgap);  // read after the box's barrier
+  };
   auto window = [&](int32_t s, const PRec& r, int l[3], T wt[3][ORDER]) {
     window_rec<T, ORDER>(r, l, wt);
     if (SPAN > 1 && s >= mid) l[2] += TILE;
@@ -1277,6 +1480,7 @@ __device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const Ti
       {
         ScalarBox<T, 0, ORDER - 1, 0, NT, ZT> rb;
         rb.load(g, G.rho, lo);
+        preload();
         rb.store(pbox);
       }
       __syncthreads();
@@ -1290,10 +1494,14 @@ __device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const Ti
       };
       if (s0 < e) rone(s0, p0, r0);
       if (s1 < e) rone(s1, p1, r1);
-      for (int32_t s = s1 + NT; s < e; s += NT) rone(s, (int64_t)w.list[s], w.rec[s]);
+      for (int32_t s = s1 + NT; s < e; s += NT) {
+        const int32_t q = ts.slot(s, sgap);
+        rone(s, (int64_t)w.list[q], w.rec[q]);
+      }
       __syncthreads();  // every read of the density box is done
     } else {
       pb.load(g, G.phi, lo);
+      preload();
     }
     pb.store(pbox);
     __syncthreads();
@@ -1305,10 +1513,14 @@ __device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const Ti
     };
     if (s0 < e) fone(s0, p0, r0);
     if (s1 < e) fone(s1, p1, r1);
-    for (int32_t s = s1 + NT; s < e; s += NT) fone(s, (int64_t)w.list[s], w.rec[s]);
+    for (int32_t s = s1 + NT; s < e; s += NT) {
+      const int32_t q = ts.slot(s, sgap);
+      fone(s, (int64_t)w.list[q], w.rec[q]);
+    }
     return;
   }
   pb.load(g, G.phi, lo);
+  preload();
   if constexpr (RLDS) {
     ScalarBox<T, 0, ORDER - 1, 0, NT, ZT> rb;
     if (G.rho_out) rb.load(g, G.rho, lo);
@@ -1329,7 +1541,10 @@ __device__ __forceinline__ void readout_grad_body(const PartDesc<T>& P, const Ti
   };
   if (s0 < e) one(s0, p0, r0);
   if (s1 < e) one(s1, p1, r1);
-  for (int32_t s = s1 + NT; s < e; s += NT) one(s, (int64_t)w.list[s], w.rec[s]);
+  for (int32_t s = s1 + NT; s < e; s += NT) {
+    const int32_t q = ts.slot(s, sgap);
+    one(s, (int64_t)w.list[q], w.rec[q]);
+  }
 }
 
 template <typename T, int ORDER, int NT = grad_threads<ORDER>(), bool RLDS = true, int SPAN = 1>
@@ -1424,20 +1639,25 @@ __global__ __launch_bounds__(kTileThreads) void k_pos_grad_tiles(TileGeom g, Wor
                                                                  PosGrad<T> G) {
   constexpr int TP = TILE + ORDER + 3;
   __shared__ T pbox[TP * TP * TP];
+  __shared__ int32_t sgap[kSub];
   const int64_t t = xcd_remap(blockIdx.x, gridDim.x);
   int lo[3];
   tile_coords(t, g, lo);
-  const int32_t b = w.offs[t * kSub], e = w.offs[t * kSub + kSub];
-  if (b == e) return;  // uniform across the workgroup
+  TileSlots<kSub> ts;
+  ts.load(w, t);
+  const int32_t e = ts.total();
+  if (e == 0) return;  // uniform across the workgroup
+  ts.store_gaps(sgap);
   ScalarBox<T, 2, ORDER + 1, TILE + ORDER - 1> pb;
   pb.load(g, G.phi, lo);
   pb.store(pbox);
   __syncthreads();
-  for (int32_t s = b + threadIdx.x; s < e; s += kTileThreads) {
+  for (int32_t s = threadIdx.x; s < e; s += kTileThreads) {
+    const int32_t q = ts.slot(s, sgap);
     int l[3];
     T wt[3][ORDER], dwt[3][ORDER];
-    window_dw<T, ORDER>(w.rec[s], l, wt, dwt);
-    pos_grad_one<T, ORDER>(pbox, G, (int64_t)w.list[s], l, wt, dwt);
+    window_dw<T, ORDER>(w.rec[q], l, wt, dwt);
+    pos_grad_one<T, ORDER>(pbox, G, (int64_t)w.list[q], l, wt, dwt);
   }
 }
 
@@ -1506,7 +1726,7 @@ static BinOrder bin_order(const fpm_particles* part, const fpm_mesh* m) {
 
 static void bin_clear(const TileGeom& g, const Workspace& w, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_i32, dim3(grid_parts(g.nbins)), dim3(256), 0, st, w.count, g.nbins);
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, w.misplaced, (int64_t)1);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, w.misplaced, (int64_t)2);  // and ovf
 }
 
 // the scan of the counts and the scatter, after a count pass
@@ -1517,12 +1737,14 @@ static int bin_finish(const PartDesc<T>& P, const TileGeom& g, const Workspace& 
   hipLaunchKernelGGL(k_bin_scan_local, dim3(nsb), dim3(kScanThreads), 0, st, w, g.nbins);
   hipLaunchKernelGGL(k_bin_scan_add, dim3(nsb), dim3(kScanThreads), 0, st, w, g.nbins);
   if (P.n > 0) {
+    // the scatter runs only when the count pass set ovf; its workgroups
+    // stride the particles, so a grid of a few per CU returns quickly when
+    // it has nothing to do
+    const unsigned gs = grid_bin(P.n) < 2048u ? grid_bin(P.n) : 2048u;
     if (order == 2)
-      hipLaunchKernelGGL((k_bin_scatter<T, 2>), dim3(grid_bin(P.n)), dim3(kBinThreads), 0, st, P,
-                         g, w, o);
+      hipLaunchKernelGGL((k_bin_scatter<T, 2>), dim3(gs), dim3(kBinThreads), 0, st, P, g, w, o);
     else
-      hipLaunchKernelGGL((k_bin_scatter<T, 3>), dim3(grid_bin(P.n)), dim3(kBinThreads), 0, st, P,
-                         g, w, o);
+      hipLaunchKernelGGL((k_bin_scatter<T, 3>), dim3(gs), dim3(kBinThreads), 0, st, P, g, w, o);
   }
   FPM_LAUNCH_CHECK();
   return FPM_OK;
@@ -1530,12 +1752,13 @@ static int bin_finish(const PartDesc<T>& P, const TileGeom& g, const Workspace& 
 
 template <typename T>
 static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, void* ws,
-                    int64_t ws_bytes, hipStream_t st) {
+                    int64_t ws_bytes, hipStream_t st, bool rebin) {
   PartDesc<T> P;
   TileGeom g;
   Workspace w;
   int rc = prep<T>(part, m, order, ws, ws_bytes, P, g, w);
   if (rc) return rc;
+  w.allow_cap = rebin ? 1 : 0;
   const BinOrder o = bin_order(part, m);
   bin_clear(g, w, st);
   if (P.n > 0) {
@@ -1835,7 +2058,10 @@ __global__ __launch_bounds__(256) void k_batch_bin_clear(const TileBox<T>* __res
   const TileBox<T>& x = tab[b];
   const int64_t i = j * 256 + threadIdx.x;
   if (i < x.g.nbins) x.w.count[i] = 0;
-  if (j == 0 && threadIdx.x == 0) x.w.misplaced[0] = 0;
+  if (j == 0 && threadIdx.x == 0) {
+    x.w.misplaced[0] = 0;
+    x.w.ovf[0] = 0;
+  }
 }
 
 template <typename T, int ORDER>
@@ -1914,6 +2140,11 @@ int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     int rc = prep<float>(&bx.part, &bx.mesh, bx.order, bx.ws, bx.ws_bytes, x.P, x.g, x.w);
     if (rc) return rc;
     if (!bx.rho) return FPM_ERR_ARG;
+    // the plan owns the box's workspace from here: its first step places
+    // the records exactly (no layout header yet), the later ones rebin
+    x.w.allow_cap = 1;
+    if (arena->base && hipMemset(x.w.hdr, 0, 3 * sizeof(int64_t)) != hipSuccess)
+      return FPM_ERR_HIP;
     x.o = bin_order(&bx.part, &bx.mesh);
     x.rho = static_cast<float*>(bx.rho);
     if ((rc = paint_scale(x.P.n, bx.mass, x.mass_fs, x.sh, x.inv_fx))) return rc;
@@ -2049,14 +2280,18 @@ int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n) {
 
 int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n) {
   if (!tiles_ok(mesh) || (order != 2 && order != 3) || n < 0) return -1;
-  int64_t off[8];
+  int64_t off[10];
   ws_offsets(tile_geom(mesh), n, off);
   return off[5];
 }
 
 int fpm_bin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,
                           int64_t ws_bytes, fpm_stream_t s) {
-  return bin_impl<float>(part, mesh, order, ws, ws_bytes, (hipStream_t)s);
+  return bin_impl<float>(part, mesh, order, ws, ws_bytes, (hipStream_t)s, false);
+}
+int fpm_rebin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,
+                            int64_t ws_bytes, fpm_stream_t s) {
+  return bin_impl<float>(part, mesh, order, ws, ws_bytes, (hipStream_t)s, true);
 }
 int fpm_paint_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, double mass,
                         void* out, void* ws, int64_t ws_bytes, fpm_stream_t s) {

@@ -387,7 +387,9 @@ class FastPMStep:
             self.solver._particles_tuned = True
         if self.tiled:
             with self._mark("bin"):
-                self.pm.bin_particles(X1)
+                # one pass into the previous Force's capacity layout (the
+                # particles moved little since)
+                self.pm.bin_particles(X1, rebin=True)
             with self._mark("paint"):
                 self.pm.paint_tiled(X1, mass=1.0 / nbar, out=rho)
         else:

@@ -163,6 +163,7 @@ class ParticleMesh:
         align = 16 if self.dtype == np.float32 else 8
         self.pitch2 = (self.nh + align - 1) // align * align
         self._tw = None
+        self._tile_binned = None  # (count, order, workspace) of the last bin call
         self._power_work = None
         self._desc = _lib.FpmMesh()
         for d in range(3):
@@ -296,12 +297,30 @@ class ParticleMesh:
         ws = self._tile_ws
         return int(ws[off:off + 4].view(torch.int32).item())
 
-    def bin_particles(self, pos, resampler=None):
+    def tile_layout_exact(self, n, order):
+        """Whether the last bin call placed its records exactly (count,
+        scan and scatter passes) rather than in one pass into the capacity
+        layout its previous call left.  Synchronises."""
+        off = int(_lib.lib().fpm_tile_misplaced_offset(self._desc, int(order), int(n)))
+        ws = self._tile_ws
+        return bool(ws[off + 4:off + 8].view(torch.int32).item())
+
+    def bin_particles(self, pos, resampler=None, rebin=False):
+        """Bins `pos` into the mesh's tile workspace for the tiled paint and
+        readouts.  rebin=True lets the call place the records in one pass
+        into the capacity layout the previous bin call on this mesh left
+        (it falls back to exact placement by itself when that layout does
+        not fit); it is used only when that call binned as many particles
+        with the same order into the same workspace."""
         parts = self._particles(pos)
         order = resampler_order(resampler or self.resampler)
         ws = self.tile_workspace(len(parts), order)
-        _lib.call_plain("fpm_bin_particles_f32", parts.desc(self.dtype), self._desc, order,
-                        ws.data_ptr(), ws.numel(), _stream())
+        key = (len(parts), order, ws.data_ptr())
+        fn = "fpm_rebin_particles_f32" if rebin and self._tile_binned == key else \
+            "fpm_bin_particles_f32"
+        _lib.call_plain(fn, parts.desc(self.dtype), self._desc, order, ws.data_ptr(), ws.numel(),
+                        _stream())
+        self._tile_binned = key
         return parts
 
     def paint_tiled(self, pos, mass=1.0, resampler=None, out=None):

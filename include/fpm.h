@@ -167,17 +167,31 @@ int fpm_readout_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, 
  * bitwise reproducible, overwrites `out` (hold=False) and needs no
  * clearing pass.  Same contracts as fpm_paint / fpm_readout
  * (fastpm/core.py:191,203; fastpm/force/gravity.py:24).
+ * Every bin call also leaves in the workspace a capacity layout for the
+ * next binning of as many particles: each sub-bin (tile and crossing mask)
+ * gets count + count/4 + 32 slots of this call's count.  fpm_rebin_particles
+ * places the records in one pass into that layout, straight at their
+ * sub-bin's start plus their rank (no count/scatter pair); it needs the
+ * workspace untouched since the bin or rebin call that left the layout.
+ * When a sub-bin outgrows its capacity, or the layout is for another
+ * particle count or mesh, it places every record exactly as
+ * fpm_bin_particles does.  The paints and readouts give the same results
+ * after either call.  No counterpart in the reference (pmesh paints with
+ * atomics and keeps no lists).
  * fpm_tile_misplaced_offset gives the byte offset in the workspace of an int32
  * counter, zeroed by each bin call, of particles the scatter could not store
  * (a slot outside [0, n), which only a disagreement between the count and
  * scatter passes could produce); nonzero means the lists are incomplete and
  * the tiled kernels must not be used with them.  0 is the only valid
- * outcome. */
+ * outcome.  The int32 after it is nonzero when the last bin or rebin call
+ * placed the records exactly, zero when it placed them in one pass. */
 int fpm_tiles_supported(const fpm_mesh* mesh);
 int64_t fpm_tile_workspace_size(const fpm_mesh* mesh, int order, int64_t n);
 int64_t fpm_tile_misplaced_offset(const fpm_mesh* mesh, int order, int64_t n);
 int fpm_bin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,
                           int64_t ws_bytes, fpm_stream_t stream);
+int fpm_rebin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,
+                            int64_t ws_bytes, fpm_stream_t stream);
 int fpm_paint_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, double mass,
                         void* out, void* ws, int64_t ws_bytes, fpm_stream_t stream);
 int fpm_readout_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, int ncomp,
@@ -573,7 +587,11 @@ fpm_batch* fpm_batch_create(int nbox, const fpm_batch_box* boxes, void* dev, int
  * leaves S and P unchanged), then the force: binning, paint of rho (mass
  * per particle), phi = c2r(common * r2c(rho)), RHO = rho_scale
  * readout(rho) and F = force_scale readout(D phi) with the four-point
- * gradient stencil.  Asynchronous, no allocation, graph-capture safe. */
+ * gradient stencil.  The binning rebins each box into the capacity layout
+ * its previous step left (fpm_rebin_particles; the plan's first step places
+ * the records exactly, as fpm_batch_create clears the layout's header), so
+ * the boxes' workspaces must not be used for anything else while the plan
+ * is.  Asynchronous, no allocation, graph-capture safe. */
 int fpm_batch_step(const fpm_batch* plan, const int32_t* counter, fpm_stream_t stream);
 
 void fpm_batch_destroy(fpm_batch* plan);

@@ -1,0 +1,164 @@
+"""One-pass binning into the capacity layout (GPU only): a bin call into a
+workspace whose previous call had the same particle count and mesh places
+every record straight at its sub-bin's capacity start plus its rank; a
+sub-bin that outgrows its capacity, or a workspace without that layout,
+falls back to exact placement.  Paint, weighted paint, the tiled readout and
+the stencil force readout must give bit for bit what an exact binning gives
+in every case."""
+import numpy as np
+import pytest
+import torch
+
+import fastpm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _outputs(pm, parts, phi, fields, w, gpu):
+    n = len(parts)
+    rho = pm.paint_tiled(parts, mass=0.7)
+    F = torch.empty((n, 3), dtype=torch.float32, device=gpu)
+    R = torch.empty((n,), dtype=torch.float32, device=gpu)
+    if pm.grad4_supported(pm.resampler):
+        pm.readout_tiled_grad4(phi, parts, F, rho=rho, rho_out=R)
+    G = torch.empty((n, 2), dtype=torch.float32, device=gpu)
+    pm.readout_tiled(fields, parts, [G[:, 0], G[:, 1]], scales=[1.0, -0.5])
+    outs = [pm.create("real") for _ in range(2)]
+    pm.paint_tiled_weighted(parts, [w[:, 0], w[:, 1]], outs, scale=0.3)
+    return [rho.value.clone(), F, R, G] + [o.value.clone() for o in outs]
+
+
+def _parts(pm, rng, spread, gpu, cluster=False):
+    from fastpm.pm import ParticleMesh, Particles
+
+    nlat = [int(n) // 2 for n in pm.Nmesh]
+    pm_p = ParticleMesh(Nmesh=nlat, BoxSize=pm.BoxSize, dtype="f4")
+    Q = pm_p.generate_uniform_particle_grid()
+    S = rng.normal(0, spread, size=(len(Q), 3))
+    if cluster:
+        # most particles pulled into a few cells near the box centre: their
+        # sub-bins overflow the capacity the spread-out set left
+        X = O.lattice(nlat, pm.BoxSize)
+        c = np.asarray(pm.BoxSize) * 0.5
+        S[: len(Q) * 3 // 4] = (c - X[: len(Q) * 3 // 4]) + rng.normal(0, 0.3, size=(len(Q) * 3 // 4, 3))
+    return Particles(Q, torch.from_numpy(S.astype("f4")).to(gpu))
+
+
+@pytest.mark.parametrize("resampler", ["cic", "tsc"])
+@pytest.mark.parametrize("nmesh", [(64, 64, 64), (32, 48, 64)])
+def test_capacity_layout_bitwise(gpu, nmesh, resampler):
+    from fastpm.pm import ParticleMesh
+
+    print("capacity layout", nmesh, resampler, flush=True)
+    box = (96.0, 72.0, 128.0)
+    order = 2 if resampler == "cic" else 3
+    rng = np.random.default_rng(5)
+    pm = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4", resampler=resampler)
+    fresh = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4", resampler=resampler)
+    phi = pm.create("real")
+    phi.value.copy_(torch.from_numpy(rng.standard_normal(nmesh).astype("f4")))
+    fields = [pm.create("real", value=torch.from_numpy(rng.standard_normal(nmesh).astype("f4")).to(gpu))
+              for _ in range(2)]
+    A = _parts(pm, rng, 2.0, gpu)
+    B = _parts(pm, rng, 2.5, gpu)  # the same count, moved a little
+    n = len(A)
+    w = torch.from_numpy(rng.standard_normal((n, 2)).astype("f4")).to(gpu)
+
+    pm.bin_particles(A)
+    assert pm.tile_layout_exact(n, order)  # bin: exact placement
+    ref_a = _outputs(pm, A, phi, fields, w, gpu)
+    # the same particles again: one pass into the capacity layout
+    pm.bin_particles(A, rebin=True)
+    assert not pm.tile_layout_exact(n, order)
+    assert pm.tile_misplaced(n, order) == 0
+    for x, y in zip(_outputs(pm, A, phi, fields, w, gpu), ref_a):
+        assert torch.equal(x, y)
+    # moved particles: capacity layout, the results of an exact binning
+    pm.bin_particles(B, rebin=True)
+    assert not pm.tile_layout_exact(n, order)
+    got = _outputs(pm, B, phi, fields, w, gpu)
+    fresh.bin_particles(B)
+    assert fresh.tile_layout_exact(n, order)
+    for x, y in zip(got, _outputs(fresh, B, phi, fields, w, gpu)):
+        assert torch.equal(x, y)
+    # the density against the oracle's paint
+    X = O.lattice(B.Q.nlat, box) + B.S.cpu().numpy().astype("f8")
+    dens = O.paint(X, nmesh, box, order, 0.7)
+    assert np.abs(got[0].cpu().numpy() - dens).max() < 2e-6 * dens.max()
+
+
+@pytest.mark.parametrize("resampler", ["cic", "tsc"])
+def test_capacity_overflow_falls_back(gpu, resampler):
+    """A set that crowds into a few sub-bins overflows the capacity the
+    previous, spread-out set left: the call places its records exactly and
+    gives an exact binning's results; binning it again uses its own
+    capacity layout."""
+    from fastpm.pm import ParticleMesh
+
+    print("capacity overflow", resampler, flush=True)
+    nmesh, box = (64, 64, 64), (100.0, 100.0, 100.0)
+    order = 2 if resampler == "cic" else 3
+    rng = np.random.default_rng(8)
+    pm = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4", resampler=resampler)
+    fresh = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4", resampler=resampler)
+    phi = pm.create("real")
+    phi.value.copy_(torch.from_numpy(rng.standard_normal(nmesh).astype("f4")))
+    fields = [pm.create("real", value=torch.from_numpy(rng.standard_normal(nmesh).astype("f4")).to(gpu))
+              for _ in range(2)]
+    A = _parts(pm, rng, 1.0, gpu)
+    C = _parts(pm, rng, 1.0, gpu, cluster=True)
+    n = len(A)
+    w = torch.from_numpy(rng.standard_normal((n, 2)).astype("f4")).to(gpu)
+    pm.bin_particles(A)
+    pm.bin_particles(A, rebin=True)
+    assert not pm.tile_layout_exact(n, order)
+    pm.bin_particles(C, rebin=True)
+    assert pm.tile_layout_exact(n, order)  # overflowed: exact placement
+    assert pm.tile_misplaced(n, order) == 0
+    got = _outputs(pm, C, phi, fields, w, gpu)
+    fresh.bin_particles(C)
+    ref = _outputs(fresh, C, phi, fields, w, gpu)
+    for x, y in zip(got, ref):
+        assert torch.equal(x, y)
+    pm.bin_particles(C, rebin=True)
+    assert not pm.tile_layout_exact(n, order)
+    for x, y in zip(_outputs(pm, C, phi, fields, w, gpu), ref):
+        assert torch.equal(x, y)
+
+
+def test_capacity_other_count(gpu):
+    """A workspace last binned for another particle count holds no layout
+    for this one: the call places its records exactly."""
+    from fastpm.pm import ParticleMesh, Particles
+
+    print("capacity other count", flush=True)
+    nmesh, box = (32, 32, 64), (50.0, 50.0, 100.0)
+    rng = np.random.default_rng(2)
+    pm = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4")
+    big = _parts(pm, rng, 1.5, gpu)
+    pm.bin_particles(big)
+    pm.bin_particles(big, rebin=True)
+    assert not pm.tile_layout_exact(len(big), 2)
+    # fewer particles (explicit positions) reuse the larger workspace; the
+    # C entry point is called directly, as a caller that ignores the
+    # count would: the layout's header sends it to exact placement
+    from fastpm import _lib
+    from fastpm.pm import _stream
+    m = len(big) // 3
+    q = torch.from_numpy((rng.uniform(0, 1, size=(m, 3)) * np.array(box)).astype("f4")).to(gpu)
+    s = torch.zeros_like(q)
+    small = Particles(q, s)
+    ws = pm.tile_workspace(m, 2)
+    _lib.call_plain("fpm_rebin_particles_f32", small.desc(pm.dtype), pm._desc, 2, ws.data_ptr(),
+                    ws.numel(), _stream())
+    pm._tile_binned = (m, 2, ws.data_ptr())
+    assert pm.tile_layout_exact(m, 2)
+    rho = pm.paint_tiled(small, mass=1.0).value.cpu().numpy()
+    ref = O.paint(q.cpu().numpy().astype("f8"), nmesh, box, 2, 1.0)
+    # explicit float32 positions: each window fraction carries ~1e-6 of a
+    # cell of rounding (the tolerance of the other explicit-position tests)
+    assert np.abs(rho - ref).max() < 5e-6 * ref.max()
+    # and bit for bit what a fresh exact binning gives
+    fresh = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4")
+    fresh.bin_particles(small)
+    assert np.array_equal(fresh.paint_tiled(small, mass=1.0).value.cpu().numpy(), rho)

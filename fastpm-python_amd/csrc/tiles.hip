@@ -91,12 +91,17 @@ struct Workspace {
 // sets ovf, and the scan and the scatter then place every record exactly as
 // before.  Either way sub-bin b's records are slots offs[b] ..
 // offs[b] + count[b] - 1, which is all the paint and the readouts use; the
-// results do not depend on the layout.  cap(c) sums to at most ncap.
+// results do not depend on the layout.  bin_cap(c) sums to at most ncap.
 constexpr int64_t kCapMagic = 0x46504d4341503031LL;
-__host__ __device__ __forceinline__ int32_t bin_cap(int32_t c) { return c + (c >> 2) + 32; }
+// After a call that had to place its records exactly (a sub-bin outgrew its
+// capacity: the counts are changing fast, as in the first steps), the next
+// layout gets twice the margin.
+__host__ __device__ __forceinline__ int32_t bin_cap(int32_t c, bool wide) {
+  return wide ? c + (c >> 1) + 64 : c + (c >> 2) + 32;
+}
 
 static int64_t capacity_slots(int64_t n, int64_t nbins) {
-  const int64_t c = n + n / 4 + 32 * nbins;
+  const int64_t c = n + n / 2 + 64 * nbins;
   return c < ((int64_t)1 << 31) - 1 ? c : n;
 }
 
@@ -521,7 +526,7 @@ __device__ __forceinline__ void scan_local_body(const Workspace& w, int64_t nbin
   for (int i = 0; i < kScanPer; ++i) {
     v[i] = i0 + i < nbins ? w.count[i0 + i] : 0;
     s += v[i];
-    sc += i0 + i < nbins ? bin_cap(v[i]) : 0;
+    sc += i0 + i < nbins ? bin_cap(v[i], exact) : 0;
   }
   const int32_t incl = wave_incl_scan(s, lane), inclc = wave_incl_scan(sc, lane);
   if (lane == 63) {
@@ -541,7 +546,7 @@ __device__ __forceinline__ void scan_local_body(const Workspace& w, int64_t nbin
       w.nxt[i0 + i] = runc;
     }
     run += v[i];
-    runc += bin_cap(v[i]);
+    runc += bin_cap(v[i], exact);
   }
   if (tid == kScanThreads - 1) {
     w.btot[blk] = run;

@@ -169,7 +169,8 @@ int fpm_readout_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, 
  * (fastpm/core.py:191,203; fastpm/force/gravity.py:24).
  * Every bin call also leaves in the workspace a capacity layout for the
  * next binning of as many particles: each sub-bin (tile and crossing mask)
- * gets count + count/4 + 32 slots of this call's count.  fpm_rebin_particles
+ * gets count + count/4 + 32 slots of this call's count (count + count/2 + 64
+ * after a call that had to place its records exactly).  fpm_rebin_particles
  * places the records in one pass into that layout, straight at their
  * sub-bin's start plus their rank (no count/scatter pair); it needs the
  * workspace untouched since the bin or rebin call that left the layout.

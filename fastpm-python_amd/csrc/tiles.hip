@@ -72,8 +72,10 @@ struct Workspace {
   int32_t* count;      // particles per sub-bin
   int32_t* offs;       // first slot of each sub-bin (nbins + 1)
   int32_t* misplaced;  // particles whose slot the exact scatter found outside [0, n)
-  int32_t* ovf;        // nonzero: the records are placed exactly (the scatter runs)
-  int64_t* hdr;        // magic, particles, sub-bins of the capacity layout in nxt
+  int32_t* ovf;        // nonzero: the records are placed exactly (the scatter runs):
+                       // 1 with the count's ranks, 2 re-ranked by the scatter
+  int64_t* hdr;        // magic, particles, sub-bins of the capacity layout in nxt,
+                       // and its wide-margin level (wide when above 0)
   int32_t* btot;       // per-workgroup totals of the count scan
   int32_t* btotc;      // per-workgroup totals of the capacity scan
   int32_t* nxt;        // first slot of each sub-bin in the next call's capacity layout
@@ -89,15 +91,27 @@ struct Workspace {
 // the particles instead of a count pass and a scatter pass.  A record that
 // does not fit its sub-bin's capacity (or a workspace without that layout)
 // sets ovf, and the scan and the scatter then place every record exactly as
-// before.  Either way sub-bin b's records are slots offs[b] ..
+// before.  The scatter needs each particle's rank: a count pass stores the
+// ranks when it has no layout, or one with the wide margin (ovf = 1 on an
+// overflow; those layouts follow an exact placement closely, while the
+// counts change fast); after an overflow of a pass that did not store them
+// (ovf = 2) the scan clears the counts and the scatter ranks the particles
+// again by counting them back up.  Either way sub-bin b's records are slots offs[b] ..
 // offs[b] + count[b] - 1, which is all the paint and the readouts use; the
 // results do not depend on the layout.  bin_cap(c) sums to at most ncap.
 constexpr int64_t kCapMagic = 0x46504d4341503031LL;
 // After a call that had to place its records exactly (a sub-bin outgrew its
 // capacity: the counts are changing fast, as in the first steps), the next
-// layout gets twice the margin.
+// kWideCalls layouts get twice the margin.
 __host__ __device__ __forceinline__ int32_t bin_cap(int32_t c, bool wide) {
   return wide ? c + (c >> 1) + 64 : c + (c >> 2) + 32;
+}
+constexpr int64_t kWideCalls = 4;
+// The wide-margin level of the layout a bin call leaves (a scan reads the
+// level of the layout the call used; exact: the call placed exactly).
+__device__ __forceinline__ int64_t next_wide_level(const Workspace& w, bool exact) {
+  const int64_t cur = w.hdr[3];
+  return exact ? kWideCalls : (cur > 1 ? cur - 1 : 0);
 }
 
 static int64_t capacity_slots(int64_t n, int64_t nbins) {
@@ -312,7 +326,8 @@ __device__ __forceinline__ int64_t bin_window(const BinOrder& o, const PartDesc<
 // to its counter; rank[i] = rk[u] = the counter before the add + the lane's
 // rank in its group.
 __device__ __forceinline__ void count_ranks(const int32_t t[kBinILP], int64_t base, int64_t n,
-                                            const Workspace& w, int32_t rk[kBinILP]) {
+                                            const Workspace& w, int32_t rk[kBinILP],
+                                            bool store_rank) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int32_t old[kBinILP];
@@ -341,7 +356,7 @@ __device__ __forceinline__ void count_ranks(const int32_t t[kBinILP], int64_t ba
     const int64_t i = base + u * kBinThreads + threadIdx.x;
     const int32_t r0 = __shfl(old[u], myleader[u]);
     rk[u] = r0 + myrank[u];
-    if (i < n) w.rank[i] = rk[u];
+    if (store_rank && i < n) w.rank[i] = rk[u];
   }
 }
 
@@ -357,7 +372,7 @@ static_assert((kBinTable & (kBinTable - 1)) == 0, "the sub-bin table size is a p
 
 __device__ __forceinline__ void count_ranks_wg(const int32_t t[kBinILP], int64_t base, int64_t n,
                                                const Workspace& w, int32_t* tkey, int32_t* tval,
-                                               int32_t rk[kBinILP]) {
+                                               int32_t rk[kBinILP], bool store_rank) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __syncthreads();  // the previous span's readers of the table are done
@@ -410,7 +425,7 @@ __device__ __forceinline__ void count_ranks_wg(const int32_t t[kBinILP], int64_t
     const int32_t h = __shfl(slot[u], myleader[u]);
     const int32_t o = __shfl(off[u], myleader[u]);
     rk[u] = tval[h] + o + myrank[u];
-    if (i < n && t[u] >= 0) w.rank[i] = rk[u];
+    if (store_rank && i < n && t[u] >= 0) w.rank[i] = rk[u];
   }
 }
 
@@ -439,17 +454,18 @@ __device__ __forceinline__ bool capacity_ready(const Workspace& w, int64_t n, in
 // over many boxes (the box's local block of its share of the items).
 //
 // The count pass: each particle's sub-bin is counted and its rank within
-// the sub-bin kept; with a capacity layout ready, the record and index go
+// the sub-bin found; with a capacity layout ready, the record and index go
 // straight to the sub-bin's capacity start plus the rank (the layout's
 // starts are copied to offs for the paint and readouts), and a record that
-// does not fit sets ovf.  Without a layout it sets ovf at once and the
-// scatter places the records.
+// does not fit sets ovf = 2.  Without a layout it sets ovf = 1 at once,
+// stores the ranks, and the scatter places the records.
 template <typename T, int ORDER>
 __device__ __forceinline__ void bin_count_body(const PartDesc<T>& P, const TileGeom& g,
                                                const Workspace& w, const BinOrder& o, int64_t blk,
                                                int64_t nblk) {
   __shared__ int32_t tkey[kBinTable], tval[kBinTable];
   const bool cap = capacity_ready(w, P.n, g.nbins);  // uniform
+  const bool keep = !cap || w.hdr[3] > 0;             // store the ranks (uniform)
   if (!cap) {
     if (blk == 0 && threadIdx.x == 0) *w.ovf = 1;
   } else {
@@ -473,10 +489,11 @@ __device__ __forceinline__ void bin_count_body(const PartDesc<T>& P, const TileG
         r[u] = make_rec<T>(tt, i0);
       }
     }
+    // the ranks are kept only for the exact placement's scatter
     if (o.blocked == kBinBlock)
-      count_ranks_wg(t, base, P.n, w, tkey, tval, rk);
+      count_ranks_wg(t, base, P.n, w, tkey, tval, rk, keep);
     else
-      count_ranks(t, base, P.n, w, rk);
+      count_ranks(t, base, P.n, w, rk, keep);
     if (cap) {
 #pragma unroll
       for (int u = 0; u < kBinILP; ++u) {
@@ -486,7 +503,7 @@ __device__ __forceinline__ void bin_count_body(const PartDesc<T>& P, const TileG
             w.list[sl] = pp[u];
             w.rec[sl] = r[u];
           } else {
-            *w.ovf = 1;
+            *w.ovf = keep ? 1 : 2;  // without stored ranks the scatter ranks again
           }
         }
       }
@@ -519,6 +536,7 @@ __device__ __forceinline__ void scan_local_body(const Workspace& w, int64_t nbin
   __shared__ int32_t wsum[kScanThreads / 64], wsumc[kScanThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const bool exact = *w.ovf != 0;
+  const bool wide = next_wide_level(w, exact) > 0;
   const int64_t i0 = blk * kScanBlock + (int64_t)tid * kScanPer;
   int32_t v[kScanPer];
   int32_t s = 0, sc = 0;
@@ -526,7 +544,7 @@ __device__ __forceinline__ void scan_local_body(const Workspace& w, int64_t nbin
   for (int i = 0; i < kScanPer; ++i) {
     v[i] = i0 + i < nbins ? w.count[i0 + i] : 0;
     s += v[i];
-    sc += i0 + i < nbins ? bin_cap(v[i], exact) : 0;
+    sc += i0 + i < nbins ? bin_cap(v[i], wide) : 0;
   }
   const int32_t incl = wave_incl_scan(s, lane), inclc = wave_incl_scan(sc, lane);
   if (lane == 63) {
@@ -546,7 +564,7 @@ __device__ __forceinline__ void scan_local_body(const Workspace& w, int64_t nbin
       w.nxt[i0 + i] = runc;
     }
     run += v[i];
-    runc += bin_cap(v[i], exact);
+    runc += bin_cap(v[i], wide);
   }
   if (tid == kScanThreads - 1) {
     w.btot[blk] = run;
@@ -584,11 +602,14 @@ __device__ __forceinline__ void scan_add_body(const Workspace& w, int64_t nbins,
     base += wsum[k];
     basec += wsumc[k];
   }
+  // after an overflow the scatter counts the particles back up to rank them
+  const bool rerank = *w.ovf == 2;
   const int64_t c0 = blk * kScanBlock;
   for (int i = tid; i < kScanBlock; i += kScanThreads) {
     const int64_t b = c0 + i;
     if (b < nbins) {
       if (exact) w.offs[b] += base;
+      if (rerank) w.count[b] = 0;
       w.nxt[b] += basec;
     }
   }
@@ -599,6 +620,7 @@ __device__ __forceinline__ void scan_add_body(const Workspace& w, int64_t nbins,
     w.hdr[0] = w.ncap > n ? kCapMagic : 0;
     w.hdr[1] = n;
     w.hdr[2] = nbins;
+    w.hdr[3] = next_wide_level(w, exact);
   }
 }
 
@@ -608,29 +630,51 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan_add(Workspace w, int6
 
 // Exact placement (when the count pass set ovf): each particle goes to the
 // start of its sub-bin (recomputed from its position with the count's
-// arithmetic) plus its slot from the count, with its record (window offsets
-// and tile-local origin), so paint and readout read their tile's particles
+// arithmetic) plus its rank (stored by the count pass, or found again by
+// counting up after an overflow), with its record (window offsets and
+// tile-local origin), so paint and readout read their tile's particles
 // contiguously and need no positions.
 template <typename T, int ORDER>
 __device__ __forceinline__ void bin_scatter_body(const PartDesc<T>& P, const TileGeom& g,
                                                  const Workspace& w, const BinOrder& o,
                                                  int64_t blk, int64_t nblk) {
-  if (*w.ovf == 0) return;  // the count pass placed every record (uniform)
+  __shared__ int32_t tkey[kBinTable], tval[kBinTable];
+  const int32_t mode = *w.ovf;  // uniform
+  if (mode == 0) return;        // the count pass placed every record
+  // mode 1: the count stored each particle's rank; mode 2: the scan cleared
+  // the counts and the particles are ranked again as they are counted back
+  // up (the order within a sub-bin does not change any result)
   const int64_t span = (int64_t)kBinThreads * kBinILP;
   for (int64_t base = blk * span; base < P.n; base += nblk * span) {
-    int32_t slot[kBinILP];
+    int32_t slot[kBinILP], key[kBinILP], rk[kBinILP];
     int32_t pp[kBinILP];
     PRec r[kBinILP];
 #pragma unroll
     for (int u = 0; u < kBinILP; ++u) {
       const int64_t i = base + u * kBinThreads + threadIdx.x;
-      slot[u] = -1;
+      key[u] = -1;
+      rk[u] = 0;
       if (i < P.n) {
-        const int32_t rank = w.rank[i];
+        if (mode == 1) rk[u] = w.rank[i];
         int i0[3];
         T t[3];
         const int64_t p = bin_window<T, ORDER>(o, P, g.nm, i, i0, t);
-        const int32_t sl = w.offs[bin_of<ORDER>(i0, g)] + rank;
+        key[u] = bin_of<ORDER>(i0, g);
+        pp[u] = (int32_t)p;
+        r[u] = make_rec<T>(t, i0);
+      }
+    }
+    if (mode != 1) {
+      if (o.blocked == kBinBlock)
+        count_ranks_wg(key, base, P.n, w, tkey, tval, rk, false);
+      else
+        count_ranks(key, base, P.n, w, rk, false);
+    }
+#pragma unroll
+    for (int u = 0; u < kBinILP; ++u) {
+      slot[u] = -1;
+      if (key[u] >= 0) {
+        const int32_t sl = w.offs[key[u]] + rk[u];
         // a slot outside [0, n) (count and scatter disagreeing) is counted,
         // not stored
         if (sl >= 0 && sl < P.n) {
@@ -638,8 +682,6 @@ __device__ __forceinline__ void bin_scatter_body(const PartDesc<T>& P, const Til
         } else {
           atomicAdd(w.misplaced, 1);
         }
-        pp[u] = (int32_t)p;
-        r[u] = make_rec<T>(t, i0);
       }
     }
     // the count placed each particle with the same arithmetic (no FMA
@@ -2148,7 +2190,7 @@ int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     // the plan owns the box's workspace from here: its first step places
     // the records exactly (no layout header yet), the later ones rebin
     x.w.allow_cap = 1;
-    if (arena->base && hipMemset(x.w.hdr, 0, 3 * sizeof(int64_t)) != hipSuccess)
+    if (arena->base && hipMemset(x.w.hdr, 0, 4 * sizeof(int64_t)) != hipSuccess)
       return FPM_ERR_HIP;
     x.o = bin_order(&bx.part, &bx.mesh);
     x.rho = static_cast<float*>(bx.rho);

@@ -87,15 +87,28 @@ def test_capacity_layout_bitwise(gpu, nmesh, resampler):
     assert np.abs(got[0].cpu().numpy() - dens).max() < 2e-6 * dens.max()
 
 
+def _placement(pm, n, order):
+    """The exact-placement word of the last bin call: 0 one pass into the
+    capacity layout, 1 exact with the count pass's ranks, 2 exact with the
+    particles ranked again by the scatter."""
+    from fastpm import _lib
+
+    off = int(_lib.lib().fpm_tile_misplaced_offset(pm._desc, int(order), int(n)))
+    return int(pm._tile_ws[off + 4:off + 8].view(torch.int32).item())
+
+
 @pytest.mark.parametrize("resampler", ["cic", "tsc"])
-def test_capacity_overflow_falls_back(gpu, resampler):
+@pytest.mark.parametrize("settle", [1, 5])
+def test_capacity_overflow_falls_back(gpu, resampler, settle):
     """A set that crowds into a few sub-bins overflows the capacity the
     previous, spread-out set left: the call places its records exactly and
     gives an exact binning's results; binning it again uses its own
-    capacity layout."""
+    capacity layout.  One rebin after an exact placement still has the wide
+    margin, whose count pass keeps the ranks; after five the margin is
+    narrow and the overflow is ranked again by the scatter."""
     from fastpm.pm import ParticleMesh
 
-    print("capacity overflow", resampler, flush=True)
+    print("capacity overflow", resampler, settle, flush=True)
     nmesh, box = (64, 64, 64), (100.0, 100.0, 100.0)
     order = 2 if resampler == "cic" else 3
     rng = np.random.default_rng(8)
@@ -110,10 +123,13 @@ def test_capacity_overflow_falls_back(gpu, resampler):
     n = len(A)
     w = torch.from_numpy(rng.standard_normal((n, 2)).astype("f4")).to(gpu)
     pm.bin_particles(A)
-    pm.bin_particles(A, rebin=True)
-    assert not pm.tile_layout_exact(n, order)
+    assert _placement(pm, n, order) == 1
+    for _ in range(settle):
+        pm.bin_particles(A, rebin=True)
+        assert not pm.tile_layout_exact(n, order)
     pm.bin_particles(C, rebin=True)
     assert pm.tile_layout_exact(n, order)  # overflowed: exact placement
+    assert _placement(pm, n, order) == (1 if settle == 1 else 2)
     assert pm.tile_misplaced(n, order) == 0
     got = _outputs(pm, C, phi, fields, w, gpu)
     fresh.bin_particles(C)

@@ -201,11 +201,13 @@ def region_bandwidth(kern, Np, Nc, Nk, bpe=4):
     """Algorithmic bytes of each timed region of one KDK step of the float32
     stencil path (per call) and the rate they imply: kick/drift 36 N_p, the
     fused kicks + drift of a step boundary 60 N_p (P, F, S in; P, S out), bin
-    12 N_p (S read), paint 4 N_c, transforms 8 N_c (density in, potential
-    out; without a monitor the spectrum is not stored), readout
-    8 N_c + 28 N_p."""
+    12 N_p (S read), the step boundary inside the next Force's binning pass
+    60 N_p (its S output is binned from registers), paint 4 N_c, transforms
+    8 N_c (density in, potential out; without a monitor the spectrum is not
+    stored), readout 8 N_c + 28 N_p."""
     s = bpe / 4.0
     model = {"kick": 36 * Np, "drift": 36 * Np, "kick_drift": 60 * Np, "bin": 12 * Np,
+             "kick_drift_bin": 60 * Np,
              "paint": 4 * Nc,
              "fft": 8 * Nc, "readout": 8 * Nc + 28 * Np}
     out = {}

@@ -121,6 +121,24 @@ __device__ __forceinline__ void lattice_cell(const PartDesc<T>& P, int64_t p, co
   }
 }
 
+// lattice_cell with the displacement given (sv[d], already in registers)
+// instead of read from P.s: the same arithmetic.
+template <typename T>
+__device__ __forceinline__ void lattice_cell_s(const PartDesc<T>& P, const int32_t li[3],
+                                               const T sv[3], int64_t cell[3], T frac[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double base = ((double)li[d] + P.lat_shift) * P.ratio[d];
+    const double fb = floor(base);
+    T t2 = (T)(base - fb);
+    t2 += sv[d] * P.inv_cell[d];
+    const T ft = floor(t2);
+    cell[d] = (int64_t)(int32_t)fb + (int64_t)(int32_t)ft;
+    frac[d] = t2 - ft;
+  }
+}
+
 // Cells are exact for positions within 2^31 cells of the box.
 template <typename T>
 __device__ __forceinline__ void particle_cell(const PartDesc<T>& P, int64_t p, int64_t cell[3], T frac[3]) {
@@ -147,6 +165,70 @@ __device__ __forceinline__ void particle_cell(const PartDesc<T>& P, int64_t p, i
 // for bit.
 __device__ __forceinline__ float madd(float a, float x, float y) { return __builtin_fmaf(a, x, y); }
 __device__ __forceinline__ double madd(double a, double x, double y) { return __builtin_fma(a, x, y); }
+
+// particle_cell with the displacement given (sv[d]) instead of read from P.s.
+template <typename T>
+__device__ __forceinline__ void particle_cell_s(const PartDesc<T>& P, int64_t p, const T sv[3],
+                                                int64_t cell[3], T frac[3]) {
+#pragma clang fp contract(off)
+  if (P.q == nullptr) {
+    int32_t li[3];
+    lattice_of(P, p, li);
+    lattice_cell_s(P, li, sv, cell, frac);
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T x = P.q[3 * p + d];
+      x += sv[d];
+      x *= P.inv_cell[d];
+      const T fx = floor(x);
+      cell[d] = (int64_t)(int32_t)fx;
+      frac[d] = x - fx;
+    }
+  }
+}
+
+// A step boundary's kicks and drift for one particle, fused into a pass that
+// then needs its position: P += k0 F [+ k1 F] (nkick of them), S += d P, each
+// element rounded as fpm_kick_drift rounds it.  S == nullptr: none.
+template <typename T>
+struct KickDrift {
+  T* S;
+  T* P;
+  const T* F;
+  T k0, k1, d;
+  int nkick;
+};
+
+// A particle's three components, moved as one 12-byte access.
+template <typename T>
+struct Vec3 {
+  T v[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void kick_drift_one(const KickDrift<T>& kd, int64_t p, T sv[3]) {
+  Vec3<T>* P3 = reinterpret_cast<Vec3<T>*>(kd.P);
+  Vec3<T>* S3 = reinterpret_cast<Vec3<T>*>(kd.S);
+  Vec3<T> pv = P3[p];
+  const Vec3<T> s0 = S3[p];
+  if (kd.nkick > 0) {
+    const Vec3<T> f = reinterpret_cast<const Vec3<T>*>(kd.F)[p];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pv.v[c] = madd(kd.k0, f.v[c], pv.v[c]);
+      if (kd.nkick > 1) pv.v[c] = madd(kd.k1, f.v[c], pv.v[c]);
+    }
+    P3[p] = pv;
+  }
+  Vec3<T> s1;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    s1.v[c] = madd(kd.d, pv.v[c], s0.v[c]);
+    sv[c] = s1.v[c];
+  }
+  S3[p] = s1;
+}
 
 }  // namespace fpm
 

@@ -280,9 +280,12 @@ struct BinOrder {
   uint32_t lat1, lat2;
 };
 
+// With kd (kd.S != nullptr, uniform) the particle's kicks and drift are
+// applied first and its new displacement is used from registers.
 template <typename T, int ORDER>
 __device__ __forceinline__ int64_t bin_window(const BinOrder& o, const PartDesc<T>& P,
-                                              const int nm[3], int64_t i, int i0[3], T t[3]) {
+                                              const int nm[3], int64_t i, int i0[3], T t[3],
+                                              const KickDrift<T>* kd = nullptr) {
   int64_t cell[3];
   T f[3];
   int64_t p = i;
@@ -305,7 +308,17 @@ __device__ __forceinline__ int64_t bin_window(const BinOrder& o, const PartDesc<
       li[2] = (int32_t)(pz * 8 + (lane & 7));
     }
     p = ((int64_t)li[0] * o.lat1 + li[1]) * o.lat2 + li[2];
-    lattice_cell(P, p, li, cell, f);
+    if (kd && kd->S) {
+      T sv[3];
+      kick_drift_one(*kd, p, sv);
+      lattice_cell_s(P, li, sv, cell, f);
+    } else {
+      lattice_cell(P, p, li, cell, f);
+    }
+  } else if (kd && kd->S) {
+    T sv[3];
+    kick_drift_one(*kd, p, sv);
+    particle_cell_s(P, p, sv, cell, f);
   } else {
     particle_cell(P, p, cell, f);
   }
@@ -462,7 +475,7 @@ __device__ __forceinline__ bool capacity_ready(const Workspace& w, int64_t n, in
 template <typename T, int ORDER>
 __device__ __forceinline__ void bin_count_body(const PartDesc<T>& P, const TileGeom& g,
                                                const Workspace& w, const BinOrder& o, int64_t blk,
-                                               int64_t nblk) {
+                                               int64_t nblk, const KickDrift<T>& kd) {
   __shared__ int32_t tkey[kBinTable], tval[kBinTable];
   const bool cap = capacity_ready(w, P.n, g.nbins);  // uniform
   const bool keep = !cap || w.hdr[3] > 0;             // store the ranks (uniform)
@@ -484,7 +497,7 @@ __device__ __forceinline__ void bin_count_body(const PartDesc<T>& P, const TileG
       if (i < P.n) {
         int i0[3];
         T tt[3];
-        pp[u] = (int32_t)bin_window<T, ORDER>(o, P, g.nm, i, i0, tt);
+        pp[u] = (int32_t)bin_window<T, ORDER>(o, P, g.nm, i, i0, tt, &kd);
         t[u] = bin_of<ORDER>(i0, g);
         r[u] = make_rec<T>(tt, i0);
       }
@@ -513,8 +526,8 @@ __device__ __forceinline__ void bin_count_body(const PartDesc<T>& P, const TileG
 
 template <typename T, int ORDER>
 __global__ __launch_bounds__(kBinThreads) void k_bin_count(PartDesc<T> P, TileGeom g, Workspace w,
-                                                           BinOrder o) {
-  bin_count_body<T, ORDER>(P, g, w, o, blockIdx.x, gridDim.x);
+                                                           BinOrder o, KickDrift<T> kd) {
+  bin_count_body<T, ORDER>(P, g, w, o, blockIdx.x, gridDim.x, kd);
 }
 
 // Exclusive scans of the sub-bin counts (into offs, when the records are to
@@ -1799,7 +1812,8 @@ static int bin_finish(const PartDesc<T>& P, const TileGeom& g, const Workspace& 
 
 template <typename T>
 static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, void* ws,
-                    int64_t ws_bytes, hipStream_t st, bool rebin) {
+                    int64_t ws_bytes, hipStream_t st, bool rebin,
+                    const KickDrift<T>& kd = KickDrift<T>{}) {
   PartDesc<T> P;
   TileGeom g;
   Workspace w;
@@ -1811,10 +1825,10 @@ static int bin_impl(const fpm_particles* part, const fpm_mesh* m, int order, voi
   if (P.n > 0) {
     if (order == 2)
       hipLaunchKernelGGL((k_bin_count<T, 2>), dim3(grid_bin(P.n)), dim3(kBinThreads), 0, st, P, g,
-                         w, o);
+                         w, o, kd);
     else
       hipLaunchKernelGGL((k_bin_count<T, 3>), dim3(grid_bin(P.n)), dim3(kBinThreads), 0, st, P, g,
-                         w, o);
+                         w, o, kd);
   }
   return bin_finish<T>(P, g, w, o, order, st);
 }
@@ -2119,7 +2133,7 @@ __global__ __launch_bounds__(kBinThreads) void k_batch_bin_count(const TileBox<T
   const int64_t j = it - pre[b];
   if (j >= cnt[b]) return;
   const TileBox<T>& x = tab[b];
-  bin_count_body<T, ORDER>(x.P, x.g, x.w, x.o, j, cnt[b]);
+  bin_count_body<T, ORDER>(x.P, x.g, x.w, x.o, j, cnt[b], KickDrift<T>{});
 }
 
 template <typename T>
@@ -2339,6 +2353,21 @@ int fpm_bin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int o
 int fpm_rebin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,
                             int64_t ws_bytes, fpm_stream_t s) {
   return bin_impl<float>(part, mesh, order, ws, ws_bytes, (hipStream_t)s, true);
+}
+int fpm_kick_drift_rebin_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* P,
+                             const void* F, const double* kicks, int nkick, double drift,
+                             void* ws, int64_t ws_bytes, fpm_stream_t s) {
+  if (!part || !part->s || !P || nkick < 0 || nkick > 2 || (nkick > 0 && (!kicks || !F)))
+    return FPM_ERR_ARG;
+  KickDrift<float> kd{};
+  kd.S = static_cast<float*>(const_cast<void*>(part->s));
+  kd.P = static_cast<float*>(P);
+  kd.F = static_cast<const float*>(F);
+  kd.k0 = nkick > 0 ? (float)kicks[0] : 0.0f;
+  kd.k1 = nkick > 1 ? (float)kicks[1] : 0.0f;
+  kd.d = (float)drift;
+  kd.nkick = nkick;
+  return bin_impl<float>(part, mesh, order, ws, ws_bytes, (hipStream_t)s, true, kd);
 }
 int fpm_paint_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, double mass,
                         void* out, void* ws, int64_t ws_bytes, fpm_stream_t s) {

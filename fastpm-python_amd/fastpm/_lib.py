@@ -160,6 +160,8 @@ _PLAIN = {
     "fpm_tile_misplaced_offset": (_i64, [_PM, _int, _i64]),
     "fpm_bin_particles_f32": (_int, [_PP, _PM, _int, _vp, _i64, _vp]),
     "fpm_rebin_particles_f32": (_int, [_PP, _PM, _int, _vp, _i64, _vp]),
+    "fpm_kick_drift_rebin_f32": (_int, [_PP, _PM, _int, _vp, _vp, _vp, _int, _dbl, _vp, _i64,
+                                        _vp]),
     "fpm_paint_tiled_f32": (_int, [_PP, _PM, _int, _dbl, _vp, _vp, _i64, _vp]),
     "fpm_readout_tiled_f32": (_int, [_PP, _PM, _int, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "fpm_paint_tiled_weighted_f32": (_int, [_PP, _PM, _int, _int, _vp, _vp, _dbl, _vp, _vp, _i64,

@@ -274,6 +274,9 @@ class FastPMStep:
         # whether Force stores delta_k; run() clears it for a Force whose
         # event no monitor receives
         self.keep_delta_k = True
+        # a step boundary's kicks and drift (P, F, kicks, drift) that the
+        # next Force's binning applies in its own pass (run_actions sets it)
+        self._pending_kd = None
 
     def _mark(self, name):
         return self.timer.region(name) if self.timer is not None else _NullRegion()
@@ -296,44 +299,63 @@ class FastPMStep:
         when Kick and Drift are this class's own, each run of Kick actions followed by a
         Drift (K, D at the first step; K, K, D at every later step boundary)
         is one fused pass (fpm_kick_drift) that gives the separate updates'
-        result bit for bit; every other action, and any sequence whose time
+        result bit for bit; when a Force follows on the tiled path, that
+        pass is the Force's binning pass (fpm_kick_drift_rebin, the same
+        result again).  Every other action, and any sequence whose time
         tags would fail an assertion, goes through run()."""
         actions = list(actions)
         fuse = (monitor is None and self.solver.fused and type(self).Kick is FastPMStep.Kick
                 and type(self).Drift is FastPMStep.Drift)
+        fuse_force = fuse and self.tiled and type(self).Force is FastPMStep.Force
         i = 0
         while i < len(actions):
             j = i
             while fuse and j < len(actions) and j - i < 2 and actions[j][0] == "K":
                 j += 1
-            if j > i and j < len(actions) and actions[j][0] == "D" and \
-                    self._kick_drift(state, actions[i:j], actions[j]):
-                i = j + 1
-                continue
+            if j > i and j < len(actions) and actions[j][0] == "D":
+                k = self._kick_drift_factors(state, actions[i:j], actions[j])
+                if k is not None:
+                    facs, fac, aP, aS = k
+                    force = actions[j + 1] if j + 1 < len(actions) else None
+                    if fuse_force and force is not None and force[0] == "F" and force[2] == aS:
+                        state.a["P"] = aP
+                        state.a["S"] = aS
+                        self._pending_kd = (state.P, state.F, facs, fac)
+                        try:
+                            self.run(*force, state, monitor)
+                        finally:
+                            self._pending_kd = None
+                        i = j + 2
+                        continue
+                    self._kick_drift(state, facs, fac)
+                    state.a["P"] = aP
+                    state.a["S"] = aS
+                    i = j + 1
+                    continue
             self.run(*actions[i], state, monitor)
             i += 1
 
-    def _kick_drift(self, state, kicks, drift):
-        """Fused Kick(s) + Drift; False (nothing done) if a time tag would
-        fail one of their assertions."""
+    def _kick_drift_factors(self, state, kicks, drift):
+        """(kick factors, drift factor, new P tag, new S tag) of Kick(s) +
+        Drift, or None if a time tag would fail one of their assertions."""
         aF, aP = state.a["F"], state.a["P"]
         facs = []
         for _, ai, ac, af in kicks:
             if ac != aF:
-                return False
+                return None
             facs.append(self._pt.kick_factor(ai, ac, af))
             aP = af
         _, ai, ac, af = drift
         if ac != aP:
-            return False
-        fac = self._pt.drift_factor(ai, ac, af)
+            return None
+        return facs, self._pt.drift_factor(ai, ac, af), aP, af
+
+    def _kick_drift(self, state, facs, fac):
+        """Fused Kick(s) + Drift in one pass (fpm_kick_drift)."""
         kc = (ctypes.c_double * 2)(*(facs + [0.0] * (2 - len(facs))))
         with self._mark("kick_drift"):
             _lib.call("fpm_kick_drift", state.dtype, state.S.data_ptr(), state.P.data_ptr(),
                       state.F.data_ptr(), state.S.numel(), kc, len(facs), float(fac), _stream())
-        state.a["P"] = aP
-        state.a["S"] = af
-        return True
 
     def Kick(self, state, ai, ac, af):
         assert ac == state.a["F"]
@@ -386,10 +408,12 @@ class FastPMStep:
             autotune_particles(self.pm, X1)
             self.solver._particles_tuned = True
         if self.tiled:
-            with self._mark("bin"):
+            kd, self._pending_kd = self._pending_kd, None
+            with self._mark("bin" if kd is None else "kick_drift_bin"):
                 # one pass into the previous Force's capacity layout (the
-                # particles moved little since)
-                self.pm.bin_particles(X1, rebin=True)
+                # particles moved little since), with the step boundary's
+                # kicks and drift when run_actions handed them over
+                self.pm.bin_particles(X1, rebin=True, kick_drift=kd)
             with self._mark("paint"):
                 self.pm.paint_tiled(X1, mass=1.0 / nbar, out=rho)
         else:

@@ -305,19 +305,39 @@ class ParticleMesh:
         ws = self._tile_ws
         return bool(ws[off + 4:off + 8].view(torch.int32).item())
 
-    def bin_particles(self, pos, resampler=None, rebin=False):
+    def bin_particles(self, pos, resampler=None, rebin=False, kick_drift=None):
         """Bins `pos` into the mesh's tile workspace for the tiled paint and
         readouts.  rebin=True lets the call place the records in one pass
         into the capacity layout the previous bin call on this mesh left
         (it falls back to exact placement by itself when that layout does
         not fit); it is used only when that call binned as many particles
-        with the same order into the same workspace."""
+        with the same order into the same workspace.  kick_drift =
+        (P, F, kicks, drift) first applies a step boundary to the momenta
+        P and to `pos`'s displacement S (P += k F for k in kicks, then
+        S += drift P, as fpm_kick_drift): inside the binning pass when it
+        rebins (fpm_kick_drift_rebin), else as a pass of its own."""
         parts = self._particles(pos)
         order = resampler_order(resampler or self.resampler)
         ws = self.tile_workspace(len(parts), order)
         key = (len(parts), order, ws.data_ptr())
-        fn = "fpm_rebin_particles_f32" if rebin and self._tile_binned == key else \
-            "fpm_bin_particles_f32"
+        can_rebin = rebin and self._tile_binned == key
+        if kick_drift is not None:
+            P, F, kicks, drift = kick_drift
+            kicks = [float(k) for k in kicks]
+            if parts.S is None or len(kicks) > 2:
+                raise ValueError("kick_drift needs a displacement and at most two kicks")
+            kc = (ctypes.c_double * 2)(*(kicks + [0.0] * (2 - len(kicks))))
+            if can_rebin:
+                _check_tensor(P, self.dtype, tuple(parts.S.shape))
+                _check_tensor(F, self.dtype, tuple(parts.S.shape))
+                _lib.call_plain("fpm_kick_drift_rebin_f32", parts.desc(self.dtype), self._desc,
+                                order, P.data_ptr(), F.data_ptr(), kc, len(kicks), float(drift),
+                                ws.data_ptr(), ws.numel(), _stream())
+                self._tile_binned = key
+                return parts
+            _lib.call("fpm_kick_drift", self.dtype, parts.S.data_ptr(), P.data_ptr(), F.data_ptr(),
+                      parts.S.numel(), kc, len(kicks), float(drift), _stream())
+        fn = "fpm_rebin_particles_f32" if can_rebin else "fpm_bin_particles_f32"
         _lib.call_plain(fn, parts.desc(self.dtype), self._desc, order, ws.data_ptr(), ws.numel(),
                         _stream())
         self._tile_binned = key

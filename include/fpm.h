@@ -170,7 +170,8 @@ int fpm_readout_f64(const fpm_particles* part, const fpm_mesh* mesh, int order, 
  * Every bin call also leaves in the workspace a capacity layout for the
  * next binning of as many particles: each sub-bin (tile and crossing mask)
  * gets count + count/4 + 32 slots of this call's count (count + count/2 + 64
- * after a call that had to place its records exactly).  fpm_rebin_particles
+ * for the four layouts after a call that had to place its records exactly).
+ * fpm_rebin_particles
  * places the records in one pass into that layout, straight at their
  * sub-bin's start plus their rank (no count/scatter pair); it needs the
  * workspace untouched since the bin or rebin call that left the layout.
@@ -193,6 +194,19 @@ int fpm_bin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int o
                           int64_t ws_bytes, fpm_stream_t stream);
 int fpm_rebin_particles_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* ws,
                             int64_t ws_bytes, fpm_stream_t stream);
+
+/* A leapfrog step boundary fused into the next Force's binning: equal to
+ * fpm_kick_drift_f32(part->s, P, F, 3 n, kicks, nkick, drift) followed by
+ * fpm_rebin_particles_f32(part, ...), bit for bit (the same updates of P and
+ * of the displacement S = part->s, which must not be NULL and is written;
+ * the same binning of the new positions), in one pass over the particles
+ * that bins each particle straight from its new displacement instead of
+ * reading it back.  Replaces the Kick, [Kick,] Drift of
+ * fastpm/core.py:168-180 and the binning of the Force's paint
+ * (fastpm/core.py:191) that follows them. */
+int fpm_kick_drift_rebin_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, void* P,
+                             const void* F, const double* kicks, int nkick, double drift,
+                             void* ws, int64_t ws_bytes, fpm_stream_t stream);
 int fpm_paint_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, double mass,
                         void* out, void* ws, int64_t ws_bytes, fpm_stream_t stream);
 int fpm_readout_tiled_f32(const fpm_particles* part, const fpm_mesh* mesh, int order, int ncomp,

@@ -4,7 +4,10 @@ every record straight at its sub-bin's capacity start plus its rank; a
 sub-bin that outgrows its capacity, or a workspace without that layout,
 falls back to exact placement.  Paint, weighted paint, the tiled readout and
 the stencil force readout must give bit for bit what an exact binning gives
-in every case."""
+in every case, also when a step boundary's kicks and drift run inside the
+binning pass."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -178,3 +181,64 @@ def test_capacity_other_count(gpu):
     fresh = ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4")
     fresh.bin_particles(small)
     assert np.array_equal(fresh.paint_tiled(small, mass=1.0).value.cpu().numpy(), rho)
+
+
+@pytest.mark.parametrize("resampler", ["cic", "tsc"])
+@pytest.mark.parametrize("explicit", [False, True])
+def test_kick_drift_rebin_bitwise(gpu, resampler, explicit):
+    """fpm_kick_drift_rebin (a step boundary's kicks and drift inside the
+    next binning's pass) leaves P, S and every tiled output equal bit for bit
+    to fpm_kick_drift followed by a rebin: for 0, 1 and 2 kicks, on a layout
+    that fits and on one that overflows (particles pulled into a cluster), on
+    lattice and explicit Lagrangian positions."""
+    from fastpm import _lib
+    from fastpm.pm import ParticleMesh, Particles
+
+    print("kick-drift rebin", resampler, explicit, flush=True)
+    nmesh, box = (64, 64, 64), (100.0, 100.0, 100.0)
+    order = 2 if resampler == "cic" else 3
+    rng = np.random.default_rng(21)
+    phi = None
+    meshes = [ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4", resampler=resampler)
+              for _ in range(2)]
+    phi = meshes[0].create("real")
+    phi.value.copy_(torch.from_numpy(rng.standard_normal(nmesh).astype("f4")))
+    fields = [meshes[0].create("real", value=torch.from_numpy(
+        rng.standard_normal(nmesh).astype("f4")).to(gpu)) for _ in range(2)]
+    base = _parts(meshes[0], rng, 1.0, gpu)
+    n = len(base)
+    Q = base.Q
+    if explicit:
+        Q = meshes[0].positions(Particles(base.Q, None)).clone()
+    S0 = base.S.clone()
+    P0 = torch.from_numpy(rng.normal(0, 0.5, size=(n, 3)).astype("f4")).to(gpu)
+    F0 = torch.from_numpy(rng.normal(0, 0.5, size=(n, 3)).astype("f4")).to(gpu)
+    # a pull towards the box centre that crowds the particles on the last case
+    X = O.lattice([int(m) // 2 for m in nmesh], box)
+    pull = torch.from_numpy((np.asarray(box) * 0.5 - X).astype("f4")).to(gpu)
+    w = torch.from_numpy(rng.standard_normal((n, 2)).astype("f4")).to(gpu)
+    cases = [([], 0.7), ([0.3], 0.5), ([0.2, -0.4], 1.1), ([1.0], 1.0)]
+    state = [[S0.clone(), P0.clone()] for _ in meshes]
+    for m, (S, P) in zip(meshes, state):
+        m.bin_particles(Particles(Q, S))  # the layout the rebins start from
+    for c, (kicks, drift) in enumerate(cases):
+        # last case: the kick sets P to the pull, the drift moves the
+        # particles into a cluster about the box centre
+        F = pull - state[0][1] if c == len(cases) - 1 else F0
+        (S1, P1), (S2, P2) = state
+        kc = (ctypes.c_double * 2)(*(kicks + [0.0] * (2 - len(kicks))))
+        _lib.call("fpm_kick_drift", "f4", S1.data_ptr(), P1.data_ptr(), F.data_ptr(),
+                  S1.numel(), kc, len(kicks), float(drift), None)
+        meshes[0].bin_particles(Particles(Q, S1), rebin=True)
+        meshes[1].bin_particles(Particles(Q, S2), rebin=True, kick_drift=(P2, F, kicks, drift))
+        assert torch.equal(S1, S2) and torch.equal(P1, P2), c
+        ex = [m.tile_layout_exact(n, order) for m in meshes]
+        assert ex[0] == ex[1], c
+        if c == len(cases) - 1:
+            assert ex[1]  # the pull overflowed the layout: exact placement
+        for m in meshes:
+            assert m.tile_misplaced(n, order) == 0
+        a = _outputs(meshes[0], Particles(Q, S1), phi, fields, w, gpu)
+        b = _outputs(meshes[1], Particles(Q, S2), phi, fields, w, gpu)
+        for x, y in zip(a, b):
+            assert torch.equal(x, y), c

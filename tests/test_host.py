@@ -139,37 +139,56 @@ def test_gpu_required():
 
 def test_run_actions_groups_kicks_and_drift():
     """FastPMStep.run_actions fuses each K[,K],D run when no monitor is given
-    (K D at the first step, K K D at every later boundary, the last K alone),
-    and runs everything one by one with a monitor, on an unfused solver, or
-    when a time tag would fail an assertion.  The step's methods are replaced
-    by recorders, so no GPU is needed."""
+    (K D at the first step, K K D at every later boundary, the last K alone);
+    on the tiled path each such run goes into the following Force's binning
+    (the Force runs with the run's factors pending).  It runs everything one
+    by one with a monitor, on an unfused solver, or when a time tag would
+    fail an assertion.  The step's methods are replaced by recorders, so no
+    GPU is needed."""
+    import types
     from fastpm.core import FastPMStep, leapfrog
 
     class Rec(FastPMStep):
-        def __init__(self, fused=True, refuse=False):
+        def __init__(self, fused=True, refuse=False, tiled=False):
             self.solver = type("S", (), {"fused": fused})()
             self.log = []
             self.refuse = refuse
+            self.tiled = tiled
+            self._pending_kd = None
 
         def run(self, action, ai, ac, af, state, monitor):
-            self.log.append(action)
+            kd = self._pending_kd
+            self.log.append(action if kd is None else "".join(kd[2]) + kd[3] + "+" + action)
 
-        def _kick_drift(self, state, kicks, drift):
+        def _kick_drift_factors(self, state, kicks, drift):
             if self.refuse:
-                return False
-            self.log.append("".join(k[0] for k in kicks) + drift[0])
-            return True
+                return None
+            return [k[0] for k in kicks], drift[0], kicks[-1][3], drift[3]
+
+        def _kick_drift(self, state, facs, fac):
+            self.log.append("".join(facs) + fac)
+
+    def state():
+        return types.SimpleNamespace(a={}, P=None, F=None)
 
     acts = list(leapfrog([0.1, 0.4, 0.7, 1.0]))
     s = Rec()
-    s.run_actions(acts, None)
+    s.run_actions(acts, state())
     assert s.log == ["F", "KD", "F", "KKD", "F", "KKD", "F", "K"]
-    for step in (Rec(fused=False), Rec(refuse=True)):
-        step.run_actions(iter(acts), None)
+    s = Rec(tiled=True)
+    st = state()
+    s.run_actions(acts, st)
+    assert s.log == ["F", "KD+F", "KKD+F", "KKD+F", "K"]
+    d = max(i for i, a in enumerate(acts) if a[0] == "D")
+    assert s._pending_kd is None and st.a == {"P": acts[d - 1][3], "S": acts[d][3]}
+    for step in (Rec(fused=False), Rec(refuse=True), Rec(fused=False, tiled=True),
+                 Rec(refuse=True, tiled=True)):
+        step.run_actions(iter(acts), state())
         assert step.log == [a[0] for a in acts]
-    s = Rec()
-    s.run_actions(acts, None, monitor=lambda *args: None)
-    assert s.log == [a[0] for a in acts]
+    for tiled in (False, True):
+        s = Rec(tiled=tiled)
+        s.run_actions(acts, state(), monitor=lambda *args: None)
+        assert s.log == [a[0] for a in acts]
 
 
 def test_reverse_pass_fuses_drift_and_kicks():

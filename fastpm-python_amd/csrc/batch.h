@@ -109,14 +109,11 @@ inline unsigned batch_grid(int64_t total) { return (unsigned)total; }
 
 // The per-unit parts.  Each builds its launches from the boxes into `out`
 // (tables placed in `arena`) and enqueues them.
-int kd_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
-                   std::vector<BatchLaunch>& out);
-int kd_batch_run(const std::vector<BatchLaunch>& ls, const int32_t* counter, hipStream_t st);
-
 int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
                      std::vector<BatchLaunch>& bin, std::vector<BatchLaunch>& paint,
                      std::vector<BatchLaunch>& readout);
-int tile_batch_run(const std::vector<BatchLaunch>& ls, hipStream_t st);
+// counter: the step row of the count pass's kick-drift (NULL: none)
+int tile_batch_run(const std::vector<BatchLaunch>& ls, hipStream_t st, const int32_t* counter);
 
 int fft_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
                     std::vector<BatchLaunch>& out);

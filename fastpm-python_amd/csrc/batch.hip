@@ -1,6 +1,6 @@
 // Batched step of many boxes (fpm_batch_*): the plan holds, per pass, the
-// launches the translation units built (kick-drift in particles.hip,
-// binning / paint / readout in tiles.hip, the transform in fft.hip), each
+// launches the translation units built (binning with the step boundary's
+// kick-drift / paint / readout in tiles.hip, the transform in fft.hip), each
 // with its argument table in the caller's device buffer.
 #include <cstring>
 #include <new>
@@ -27,7 +27,7 @@ void* DevArena::put(const void* host, int64_t bytes, int* rc) {
 }  // namespace fpm
 
 struct fpm_batch {
-  std::vector<fpm::BatchLaunch> kd, bin, paint, fft, readout;
+  std::vector<fpm::BatchLaunch> bin, paint, fft, readout;
 };
 
 using namespace fpm;
@@ -38,9 +38,8 @@ static int build_all(int nbox, const fpm_batch_box* boxes, DevArena* arena, fpm_
     if (boxes[i].order != 2 && boxes[i].order != 3) return FPM_ERR_ARG;
     if (boxes[i].part.n > 0 && boxes[i].part.s == nullptr) return FPM_ERR_ARG;
   }
-  int rc = kd_batch_build(nbox, boxes, arena, b->kd);
+  int rc = tile_batch_build(nbox, boxes, arena, b->bin, b->paint, b->readout);
   if (rc) return rc;
-  if ((rc = tile_batch_build(nbox, boxes, arena, b->bin, b->paint, b->readout))) return rc;
   return fft_batch_build(nbox, boxes, arena, b->fft);
 }
 
@@ -81,11 +80,11 @@ int fpm_batch_step(const fpm_batch* b, const int32_t* counter, fpm_stream_t stre
   if (!b) return FPM_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   int rc = FPM_OK;
-  if (counter && (rc = kd_batch_run(b->kd, counter, st))) return rc;
-  if ((rc = tile_batch_run(b->bin, st))) return rc;
-  if ((rc = tile_batch_run(b->paint, st))) return rc;
+  // the count pass applies the step boundary (counter) before binning
+  if ((rc = tile_batch_run(b->bin, st, counter))) return rc;
+  if ((rc = tile_batch_run(b->paint, st, nullptr))) return rc;
   if ((rc = fft_batch_run(b->fft, st))) return rc;
-  return tile_batch_run(b->readout, st);
+  return tile_batch_run(b->readout, st, nullptr);
 }
 
 void fpm_batch_destroy(fpm_batch* b) { delete b; }

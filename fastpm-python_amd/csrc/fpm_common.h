@@ -83,6 +83,12 @@ struct PartDesc {
   T inv_cell[3];      // Nmesh / BoxSize
 };
 
+// A particle's three components, moved as one 12-byte access.
+template <typename T>
+struct Vec3 {
+  T v[3];
+};
+
 // Lattice coordinates of lattice particle p (C order over P.lat).
 template <typename T>
 __device__ __forceinline__ void lattice_of(const PartDesc<T>& P, int64_t p, int32_t li[3]) {
@@ -102,27 +108,10 @@ __device__ __forceinline__ void lattice_of(const PartDesc<T>& P, int64_t p, int3
   }
 }
 
-// Cell and fraction of lattice particle p whose lattice coordinates li are
-// known.  No FMA contraction: every kernel that places a particle (binning,
-// paint, readout) must round its position identically.
-template <typename T>
-__device__ __forceinline__ void lattice_cell(const PartDesc<T>& P, int64_t p, const int32_t li[3],
-                                             int64_t cell[3], T frac[3]) {
-#pragma clang fp contract(off)
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double base = ((double)li[d] + P.lat_shift) * P.ratio[d];
-    const double fb = floor(base);
-    T t2 = (T)(base - fb);
-    if (P.s) t2 += P.s[3 * p + d] * P.inv_cell[d];
-    const T ft = floor(t2);
-    cell[d] = (int64_t)(int32_t)fb + (int64_t)(int32_t)ft;
-    frac[d] = t2 - ft;
-  }
-}
-
-// lattice_cell with the displacement given (sv[d], already in registers)
-// instead of read from P.s: the same arithmetic.
+// Cell and fraction of a lattice particle at lattice coordinates li whose
+// displacement is sv (in registers).  No FMA contraction: every kernel that
+// places a particle (binning, paint, readout) must round its position
+// identically.
 template <typename T>
 __device__ __forceinline__ void lattice_cell_s(const PartDesc<T>& P, const int32_t li[3],
                                                const T sv[3], int64_t cell[3], T frac[3]) {
@@ -133,6 +122,27 @@ __device__ __forceinline__ void lattice_cell_s(const PartDesc<T>& P, const int32
     const double fb = floor(base);
     T t2 = (T)(base - fb);
     t2 += sv[d] * P.inv_cell[d];
+    const T ft = floor(t2);
+    cell[d] = (int64_t)(int32_t)fb + (int64_t)(int32_t)ft;
+    frac[d] = t2 - ft;
+  }
+}
+
+// The same for lattice particle p, its displacement read from P.s (or zero).
+template <typename T>
+__device__ __forceinline__ void lattice_cell(const PartDesc<T>& P, int64_t p, const int32_t li[3],
+                                             int64_t cell[3], T frac[3]) {
+#pragma clang fp contract(off)
+  if (P.s) {  // the displacement as one 12-byte load
+    const Vec3<T> sv = reinterpret_cast<const Vec3<T>*>(P.s)[p];
+    lattice_cell_s(P, li, sv.v, cell, frac);
+    return;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double base = ((double)li[d] + P.lat_shift) * P.ratio[d];
+    const double fb = floor(base);
+    const T t2 = (T)(base - fb);
     const T ft = floor(t2);
     cell[d] = (int64_t)(int32_t)fb + (int64_t)(int32_t)ft;
     frac[d] = t2 - ft;
@@ -198,12 +208,6 @@ struct KickDrift {
   const T* F;
   T k0, k1, d;
   int nkick;
-};
-
-// A particle's three components, moved as one 12-byte access.
-template <typename T>
-struct Vec3 {
-  T v[3];
 };
 
 template <typename T>

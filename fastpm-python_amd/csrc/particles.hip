@@ -1,6 +1,6 @@
 // Particle-array updates: kick / drift (axpy with a double coefficient), the
 // 2LPT combination of displacement, momentum and force, and positions X = Q + S.
-#include "batch.h"
+#include "fpm_common.h"
 
 namespace fpm {
 
@@ -328,125 +328,6 @@ static int positions_impl(const fpm_particles* part, const fpm_mesh* m, void* ou
   hipLaunchKernelGGL(k_positions<T>, dim3(grid_elems(P.n)), dim3(256), 0, st, P, sp[0], sp[1],
                      sp[2], static_cast<T*>(out));
   FPM_LAUNCH_CHECK();
-  return FPM_OK;
-}
-
-// ---------------------------------------------------------------- batched boxes
-// The step-boundary update of fpm_kick_drift_table for many boxes in one
-// launch: each box's arrays and coefficient table in a device table; item i
-// is a 256-element chunk of its box; the arithmetic per element is
-// k_kick_drift_table's.
-template <typename T>
-struct KdBox {
-  T* S;
-  T* P;
-  const T* F;
-  int64_t n;
-  const double* table;
-  int rows, stride, close, open, drift;
-  int v4;  // S, P and F 16-byte aligned: four-element vector accesses
-};
-
-constexpr int kKdPer = 8;  // elements per thread of a batched kick-drift item
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_batch_kick_drift(const KdBox<T>* __restrict__ tab,
-                                                          const int64_t* __restrict__ pre,
-                                                          const int64_t* __restrict__ cnt, int nb,
-                                                          int64_t total,
-                                                          const int32_t* __restrict__ counter) {
-  const int32_t row = counter[0];
-  const int64_t it = blockIdx.x;
-  if (it >= total) return;
-  const int b = batch_box(pre, nb, it);
-  const int64_t j = it - pre[b];
-  if (j >= cnt[b]) return;
-  const KdBox<T> x = tab[b];
-  if (row < 0 || row >= x.rows) return;
-  const bool prev = row > 0;
-  const T k0 = prev ? (T)x.table[(int64_t)(row - 1) * x.stride + x.close] : (T)0;
-  const T k1 = (T)x.table[(int64_t)row * x.stride + x.open];
-  const T d = (T)x.table[(int64_t)row * x.stride + x.drift];
-  auto one = [&](int64_t i) {
-    const T f = x.F[i];
-    T p = x.P[i];
-    if (prev) p = madd(k0, f, p);
-    p = madd(k1, f, p);
-    x.P[i] = p;
-    x.S[i] = madd(d, p, x.S[i]);
-  };
-  if (x.v4) {
-    // the item's 256 kKdPer elements as kKdPer / 4 vectors per thread; the
-    // thread whose first vector is the n / 4-th takes the n % 4 tail
-    typedef T V4 __attribute__((ext_vector_type(4)));
-    V4* S4 = reinterpret_cast<V4*>(x.S);
-    V4* P4 = reinterpret_cast<V4*>(x.P);
-    const V4* F4 = reinterpret_cast<const V4*>(x.F);
-    const int64_t n4 = x.n / 4;
-#pragma unroll
-    for (int u = 0; u < kKdPer / 4; ++u) {
-      const int64_t i4 = (j * (kKdPer / 4) + u) * 256 + threadIdx.x;
-      if (i4 < n4) {
-        const V4 f = F4[i4];
-        V4 p = P4[i4], s = S4[i4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          if (prev) p[c] = madd(k0, f[c], p[c]);
-          p[c] = madd(k1, f[c], p[c]);
-          s[c] = madd(d, p[c], s[c]);
-        }
-        P4[i4] = p;
-        S4[i4] = s;
-      } else if (i4 == n4) {
-        for (int64_t i = n4 * 4; i < x.n; ++i) one(i);
-      }
-    }
-    return;
-  }
-#pragma unroll
-  for (int u = 0; u < kKdPer; ++u) {
-    const int64_t i = (j * kKdPer + u) * 256 + threadIdx.x;
-    if (i < x.n) one(i);
-  }
-}
-
-int kd_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
-                   std::vector<BatchLaunch>& out) {
-  std::vector<KdBox<float>> args(nbox);
-  for (int i = 0; i < nbox; ++i) {
-    const fpm_batch_box& bx = boxes[i];
-    const int64_t n = 3 * bx.part.n;
-    if (n > 0 && (!bx.part.s || !bx.P || !bx.force_out || bx.force_stride != 3 || !bx.table ||
-                  bx.rows < 1 || bx.stride < 1))
-      return FPM_ERR_ARG;
-    for (int k : {bx.close, bx.open, bx.drift})
-      if (n > 0 && (k < 0 || k >= bx.stride)) return FPM_ERR_ARG;
-    KdBox<float>& x = args[i];
-    x.S = static_cast<float*>(const_cast<void*>(bx.part.s));
-    x.P = static_cast<float*>(bx.P);
-    x.F = static_cast<const float*>(bx.force_out);
-    x.n = n;
-    x.table = bx.table;
-    x.rows = bx.rows;
-    x.stride = bx.stride;
-    x.close = bx.close;
-    x.open = bx.open;
-    x.drift = bx.drift;
-    auto al = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
-    x.v4 = al(x.S) && al(x.P) && al(x.F) ? 1 : 0;
-  }
-  return batch_groups(args, 1, [](const KdBox<float>&) { return BatchKey{0, 0, 0, 0}; },
-                      [](const KdBox<float>& x) { return (x.n + 256 * kKdPer - 1) / (256 * kKdPer); },
-                      arena, out);
-}
-
-int kd_batch_run(const std::vector<BatchLaunch>& ls, const int32_t* counter, hipStream_t st) {
-  for (const BatchLaunch& L : ls) {
-    hipLaunchKernelGGL(k_batch_kick_drift<float>, dim3(batch_grid(L.total)), dim3(256), 0, st,
-                       static_cast<const KdBox<float>*>(L.tab), L.pre, L.cnt, L.nb, L.total,
-                       counter);
-    FPM_LAUNCH_CHECK();
-  }
   return FPM_OK;
 }
 

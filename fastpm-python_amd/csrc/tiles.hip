@@ -2102,6 +2102,13 @@ struct TileBox {
   int sh;
   double inv_fx;
   GradRead<T> G;
+  // the step boundary before the force, applied by the count pass: rows of
+  // kick / drift coefficients as fpm_kick_drift_table reads them
+  T* S;
+  T* Pm;
+  const T* F;
+  const double* table;
+  int rows, stride, close, open, drift;
 };
 
 enum { kTbClear = 1, kTbCount, kTbScanLocal, kTbScanAdd, kTbScatter, kTbPaint, kTbReadout };
@@ -2126,14 +2133,34 @@ __global__ __launch_bounds__(256) void k_batch_bin_clear(const TileBox<T>* __res
 }
 
 template <typename T, int ORDER>
-__global__ __launch_bounds__(kBinThreads) void k_batch_bin_count(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total) {
+__global__ __launch_bounds__(kBinThreads) void k_batch_bin_count(const TileBox<T>* __restrict__ tab, const int64_t* __restrict__ pre, const int64_t* __restrict__ cnt, int nb, int64_t total, const int32_t* __restrict__ counter) {
   const int64_t it = blockIdx.x;
   if (it >= total) return;
   const int b = batch_box(pre, nb, it);
   const int64_t j = it - pre[b];
   if (j >= cnt[b]) return;
   const TileBox<T>& x = tab[b];
-  bin_count_body<T, ORDER>(x.P, x.g, x.w, x.o, j, cnt[b], KickDrift<T>{});
+  // step row counter[0]: kicks by the previous row's closing and this row's
+  // opening coefficient (only the opening one on row 0), then the drift, as
+  // fpm_kick_drift_table; a row outside [0, rows) leaves S and P unchanged
+  KickDrift<T> kd{};
+  const int32_t row = counter ? counter[0] : -1;
+  if (row >= 0 && row < x.rows) {
+    kd.S = x.S;
+    kd.P = x.Pm;
+    kd.F = x.F;
+    const T open = (T)x.table[(int64_t)row * x.stride + x.open];
+    if (row > 0) {
+      kd.k0 = (T)x.table[(int64_t)(row - 1) * x.stride + x.close];
+      kd.k1 = open;
+      kd.nkick = 2;
+    } else {
+      kd.k0 = open;
+      kd.nkick = 1;
+    }
+    kd.d = (T)x.table[(int64_t)row * x.stride + x.drift];
+  }
+  bin_count_body<T, ORDER>(x.P, x.g, x.w, x.o, j, cnt[b], kd);
 }
 
 template <typename T>
@@ -2201,6 +2228,20 @@ int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
     int rc = prep<float>(&bx.part, &bx.mesh, bx.order, bx.ws, bx.ws_bytes, x.P, x.g, x.w);
     if (rc) return rc;
     if (!bx.rho) return FPM_ERR_ARG;
+    if (x.P.n > 0 && (!bx.part.s || !bx.P || !bx.force_out || bx.force_stride != 3 ||
+                      !bx.table || bx.rows < 1 || bx.stride < 1))
+      return FPM_ERR_ARG;
+    for (int k : {bx.close, bx.open, bx.drift})
+      if (x.P.n > 0 && (k < 0 || k >= bx.stride)) return FPM_ERR_ARG;
+    x.S = static_cast<float*>(const_cast<void*>(bx.part.s));
+    x.Pm = static_cast<float*>(bx.P);
+    x.F = static_cast<const float*>(bx.force_out);
+    x.table = bx.table;
+    x.rows = bx.rows;
+    x.stride = bx.stride;
+    x.close = bx.close;
+    x.open = bx.open;
+    x.drift = bx.drift;
     // the plan owns the box's workspace from here: its first step places
     // the records exactly (no layout header yet), the later ones rebin
     x.w.allow_cap = 1;
@@ -2255,7 +2296,7 @@ int tile_batch_build(int nbox, const fpm_batch_box* boxes, DevArena* arena,
   return FPM_OK;
 }
 
-int tile_batch_run(const std::vector<BatchLaunch>& ls, hipStream_t st) {
+int tile_batch_run(const std::vector<BatchLaunch>& ls, hipStream_t st, const int32_t* counter) {
   for (const BatchLaunch& L : ls) {
     const auto* tab = static_cast<const TileBox<float>*>(L.tab);
     const dim3 grid(batch_grid(L.total));
@@ -2267,10 +2308,10 @@ int tile_batch_run(const std::vector<BatchLaunch>& ls, hipStream_t st) {
       case kTbCount:
         if (L.a == 2)
           hipLaunchKernelGGL((k_batch_bin_count<float, 2>), grid, dim3(kBinThreads), 0, st, tab,
-                             L.pre, L.cnt, L.nb, L.total);
+                             L.pre, L.cnt, L.nb, L.total, counter);
         else
           hipLaunchKernelGGL((k_batch_bin_count<float, 3>), grid, dim3(kBinThreads), 0, st, tab,
-                             L.pre, L.cnt, L.nb, L.total);
+                             L.pre, L.cnt, L.nb, L.total, counter);
         break;
       case kTbScanLocal:
         hipLaunchKernelGGL(k_batch_scan_local<float>, grid, dim3(kScanThreads), 0, st, tab, L.pre,

@@ -599,7 +599,9 @@ fpm_batch* fpm_batch_create(int nbox, const fpm_batch_box* boxes, void* dev, int
 /* Enqueues, for every box: with counter != NULL the step boundary
  * P += table[row-1][close] F (skipped for row 0), P += table[row][open] F,
  * S += table[row][drift] P (row = counter[0]; a row outside [0, rows)
- * leaves S and P unchanged), then the force: binning, paint of rho (mass
+ * leaves S and P unchanged; applied by the binning's pass over the
+ * particles, as fpm_kick_drift_rebin_f32 does), then the force: binning,
+ * paint of rho (mass
  * per particle), phi = c2r(common * r2c(rho)), RHO = rho_scale
  * readout(rho) and F = force_scale readout(D phi) with the four-point
  * gradient stencil.  The binning rebins each box into the capacity layout

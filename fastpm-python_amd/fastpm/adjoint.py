@@ -256,11 +256,12 @@ class ForwardModel:
         pending = []
         for act in self.actions:
             if act[0] == "F":
-                # the kicks and drift since the last Force in one pass
-                self.step.run_actions(pending, state)
+                # the kicks and drift since the last Force in one pass (on the
+                # tiled path the Force's binning pass); the Force leaves S as
+                # the drift left it, so the checkpoint is taken after it
+                self.step.run_actions(pending + [act], state)
                 pending = []
                 self.checkpoints.append(state.S.clone())
-                self.step.run(*act, state, None)
             else:
                 pending.append(act)
         self.step.run_actions(pending, state)

@@ -184,23 +184,26 @@ def test_capacity_other_count(gpu):
 
 
 @pytest.mark.parametrize("resampler", ["cic", "tsc"])
-@pytest.mark.parametrize("explicit", [False, True])
-def test_kick_drift_rebin_bitwise(gpu, resampler, explicit):
+@pytest.mark.parametrize("explicit,walk", [(False, 0), (False, 1), (False, 2), (True, 0)])
+def test_kick_drift_rebin_bitwise(gpu, resampler, explicit, walk):
     """fpm_kick_drift_rebin (a step boundary's kicks and drift inside the
     next binning's pass) leaves P, S and every tiled output equal bit for bit
     to fpm_kick_drift followed by a rebin: for 0, 1 and 2 kicks, on a layout
     that fits and on one that overflows (particles pulled into a cluster), on
-    lattice and explicit Lagrangian positions."""
+    explicit Lagrangian positions and on the lattice in each binning walk
+    (blocks, patches, index order)."""
     from fastpm import _lib
     from fastpm.pm import ParticleMesh, Particles
 
-    print("kick-drift rebin", resampler, explicit, flush=True)
+    print("kick-drift rebin", resampler, explicit, walk, flush=True)
     nmesh, box = (64, 64, 64), (100.0, 100.0, 100.0)
     order = 2 if resampler == "cic" else 3
     rng = np.random.default_rng(21)
     phi = None
     meshes = [ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4", resampler=resampler)
               for _ in range(2)]
+    for m in meshes:
+        m.set_variant(bin=walk)
     phi = meshes[0].create("real")
     phi.value.copy_(torch.from_numpy(rng.standard_normal(nmesh).astype("f4")))
     fields = [meshes[0].create("real", value=torch.from_numpy(
@@ -242,3 +245,38 @@ def test_kick_drift_rebin_bitwise(gpu, resampler, explicit):
         b = _outputs(meshes[1], Particles(Q, S2), phi, fields, w, gpu)
         for x, y in zip(a, b):
             assert torch.equal(x, y), c
+
+
+def test_kick_drift_bin_without_layout(gpu):
+    """bin_particles(kick_drift=...) on a workspace that cannot rebin (no
+    previous binning of these particles) applies the step boundary as a
+    pass of its own before an exact binning: the same P, S and outputs as
+    fpm_kick_drift followed by a bin."""
+    from fastpm import _lib
+    from fastpm.pm import ParticleMesh, Particles
+
+    print("kick-drift without layout", flush=True)
+    nmesh, box = (64, 64, 64), (100.0, 100.0, 100.0)
+    rng = np.random.default_rng(4)
+    meshes = [ParticleMesh(Nmesh=nmesh, BoxSize=box, dtype="f4") for _ in range(2)]
+    phi = meshes[0].create("real")
+    phi.value.copy_(torch.from_numpy(rng.standard_normal(nmesh).astype("f4")))
+    fields = [meshes[0].create("real", value=torch.from_numpy(
+        rng.standard_normal(nmesh).astype("f4")).to(gpu)) for _ in range(2)]
+    base = _parts(meshes[0], rng, 1.0, gpu)
+    n = len(base)
+    P0 = torch.from_numpy(rng.normal(0, 0.5, size=(n, 3)).astype("f4")).to(gpu)
+    F = torch.from_numpy(rng.normal(0, 0.5, size=(n, 3)).astype("f4")).to(gpu)
+    w = torch.from_numpy(rng.standard_normal((n, 2)).astype("f4")).to(gpu)
+    S1, S2, P1, P2 = base.S.clone(), base.S.clone(), P0.clone(), P0.clone()
+    kc = (ctypes.c_double * 2)(0.25, -0.5)
+    _lib.call("fpm_kick_drift", "f4", S1.data_ptr(), P1.data_ptr(), F.data_ptr(), S1.numel(),
+              kc, 2, 0.8, None)
+    meshes[0].bin_particles(Particles(base.Q, S1))
+    meshes[1].bin_particles(Particles(base.Q, S2), rebin=True, kick_drift=(P2, F, [0.25, -0.5], 0.8))
+    assert torch.equal(S1, S2) and torch.equal(P1, P2)
+    assert meshes[1].tile_layout_exact(n, 2)
+    a = _outputs(meshes[0], Particles(base.Q, S1), phi, fields, w, gpu)
+    b = _outputs(meshes[1], Particles(base.Q, S2), phi, fields, w, gpu)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
